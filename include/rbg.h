@@ -1,0 +1,172 @@
+/*
+ * rbg.h -- C ABI of librbg, the MI355X (gfx950) implementation of RabbitBin's
+ * graph-construction path: canonical 4-mer weighted ProbMinHash sketching,
+ * all-pairs register-match similarity, per-row top-N with calibration and the
+ * mutual filter, and Spearman / weighted-Jaccard edge weighting.
+ *
+ * Every entry point returns an rbg_status (0 = success) and never throws.
+ * Host buffers are caller-owned unless a function says otherwise; buffers the
+ * library allocates are released with the matching *_free call.
+ * One context owns one HIP device and one stream; calls on a context are not
+ * re-entrant (drive each context from one host thread).
+ *
+ * Reference seams replaced (paths relative to the RabbitBin source tree):
+ *   build_pmh_winners           src/rabbitbin.cpp:1436-1441   -> rbg_sketch_winners
+ *   estimate_pmh_baseline       src/rabbitbin.cpp:1340-1369   -> rbg_pmh_baseline
+ *   gen_fused_calib_graph       src/impl/rb_graph.cpp:501     -> rbg_build_graph (fused)
+ *   calibrate_sim_cutoff +      src/impl/rb_graph.cpp:1362,
+ *   build_similarity_graph      src/impl/rb_graph.cpp:199     -> rbg_build_graph (small)
+ *   graph dispatch              src/rabbitbin.cpp:5035-5054   -> rbg_build_graph (auto)
+ *   edge-weight loop            src/rabbitbin.cpp:5482-5509   -> rbg_edge_weights
+ *   depth features              src/rabbitbin.cpp:4784-4988   -> rbg_depth_features
+ */
+#ifndef RBG_H
+#define RBG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int rbg_status;
+#define RBG_OK 0
+#define RBG_ERR_HIP (-1)         /* a HIP runtime call failed */
+#define RBG_ERR_ARG (-2)         /* invalid argument */
+#define RBG_ERR_NOMEM (-3)       /* device or host allocation failed */
+#define RBG_ERR_UNSUPPORTED (-4) /* configuration outside the implemented path */
+#define RBG_ERR_BUDGET (-5)      /* request exceeds the HBM budget */
+
+typedef struct rbg_ctx rbg_ctx;
+
+/* Open device `device` (HIP ordinal) and create the context's stream. */
+rbg_status rbg_open(int device, rbg_ctx** out);
+void rbg_close(rbg_ctx* ctx);
+/* Message of the last failure on this context (static storage, never NULL). */
+const char* rbg_last_error(const rbg_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Stage (a): sketches.  Replaces build_pmh_winners (src/rabbitbin.cpp:1436)
+ * for k = 4 with the default GC-normalised weights (cap 20) and PMH seed
+ * `seed` (the reference uses 42).  seqs holds the n contigs back to back;
+ * contig r is seqs[offsets[r] .. offsets[r+1]).  Outputs (either may be NULL):
+ *   out_win32  n x m  folded 32-bit winners, identical to the reference's
+ *                     g_win_flat rows (src/rabbitbin.cpp:1582-1588)
+ *   out_win16  n x m  16-bit folds (fold_win16, src/rabbitbin.cpp:249-253)
+ * ------------------------------------------------------------------------- */
+rbg_status rbg_sketch_winners(rbg_ctx* ctx, const char* seqs, const uint64_t* offsets,
+                              uint32_t n, uint32_t m, uint64_t seed,
+                              uint32_t* out_win32, uint16_t* out_win16);
+
+/* ---------------------------------------------------------------------------
+ * Depth features (src/rabbitbin.cpp:4784-4988, src/ranker.h:55-133,
+ * src/impl/rb_abundance.cpp:94-114).  depth: n x S raw per-sample means.
+ * Any output may be NULL.
+ * ------------------------------------------------------------------------- */
+rbg_status rbg_depth_features(const float* depth, uint32_t n, uint32_t S, double min_cv,
+                              float* ranked, float* unit, float* raw, double* colnorm,
+                              uint8_t* anynz);
+
+/* ---------------------------------------------------------------------------
+ * Graph construction: the dispatch of src/rabbitbin.cpp:5035-5054 with the
+ * default options.  Sketches, b0, the pair pass, top-N, calibration, the
+ * mutual filter and (when S > 0) edge weights, in one call.
+ * ------------------------------------------------------------------------- */
+typedef struct rbg_graph_params {
+  uint32_t m;              /* --sketch-m (default 500) */
+  uint32_t max_edges;      /* --max-edges (default 200) */
+  double min_edge_weight;  /* --min-edge-score / 100 (default 0.70) */
+  double coverage;         /* --max-posterior / 100 (default 0.95) */
+  double min_cv;           /* --min-coverage (default 1.0) */
+  double neg_depth_thr;    /* RABBIT_NEG_DEPTH default -0.3 */
+  uint64_t pmh_seed;       /* g_pmh_seed (42) */
+  uint32_t rand_seed;      /* srand() seed (--seed); calibration consumes rand() */
+  int regime;              /* 0 auto (n > 25000 -> fused), 1 small, 2 fused */
+} rbg_graph_params;
+
+/* Fills `p` with the reference defaults listed above (rand_seed = 1). */
+void rbg_default_params(rbg_graph_params* p);
+
+typedef struct rbg_graph_result {
+  uint64_t n_edges;
+  uint32_t* from;          /* edges sorted by (from, to), from < to */
+  uint32_t* to;
+  float* scomp;            /* composition similarity sComp */
+  float* score;            /* edgeScore (0 for cut edges); NULL when S == 0 */
+  double sim_cutoff;       /* simCutoff (per-mille scale, as the reference) */
+  double b0;               /* PMH baseline */
+  int regime;              /* 1 small, 2 fused */
+  uint64_t gated_pairs;    /* pairs that reached the register compare */
+  /* stage timings in milliseconds, measured with HIP events on the stream */
+  double ms_sketch, ms_pairs, ms_select, ms_edges, ms_total;
+} rbg_graph_result;
+
+/* Optional per-row diagnostics for parity tests (any pointer may be NULL):
+ * top-N lists sorted by neighbour id, their lengths, and the per-row maximum
+ * match count used by calibration. */
+typedef struct rbg_graph_diag {
+  uint32_t* topk_id;       /* n x max_edges */
+  uint16_t* topk_cnt;      /* n x max_edges */
+  uint32_t* topk_len;      /* n */
+  uint32_t* rowmax_cnt;    /* n */
+  uint16_t* win16;         /* n x m folded winners */
+} rbg_graph_diag;
+
+/* seqs/offsets as rbg_sketch_winners; depth n x S raw means (S may be 0). */
+rbg_status rbg_build_graph(rbg_ctx* ctx, const char* seqs, const uint64_t* offsets,
+                           uint32_t n, const float* depth, uint32_t S,
+                           const rbg_graph_params* params, rbg_graph_result* out,
+                           rbg_graph_diag* diag);
+void rbg_graph_result_free(rbg_graph_result* r);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident assembly: the contigs and depth features are uploaded once
+ * (depth features are derived at upload), and rbg_asm_build_graph runs the
+ * whole graph construction on the resident copy.  rbg_build_graph is
+ * rbg_asm_create + rbg_asm_build_graph + rbg_asm_free.
+ * ------------------------------------------------------------------------- */
+typedef struct rbg_asm rbg_asm;
+rbg_status rbg_asm_create(rbg_ctx* ctx, const char* seqs, const uint64_t* offsets, uint32_t n,
+                          const float* depth, uint32_t S, double min_cv, rbg_asm** out);
+void rbg_asm_free(rbg_asm* a);
+rbg_status rbg_asm_build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* params,
+                               rbg_graph_result* out, rbg_graph_diag* diag);
+
+/* ---------------------------------------------------------------------------
+ * Stage (d) alone (src/rabbitbin.cpp:5482-5509): edge weights for a given
+ * edge list.  unit/raw are n x S, colnorm S.  Weights are computed in FP64
+ * and returned as float (score) and, if non-NULL, double (score64).
+ * ------------------------------------------------------------------------- */
+rbg_status rbg_edge_weights(rbg_ctx* ctx, const uint32_t* from, const uint32_t* to,
+                            const float* scomp, uint64_t ne, const float* unit,
+                            const float* raw, const double* colnorm, uint32_t n,
+                            uint32_t S, double w_comp, double min_edge_weight,
+                            double neg_depth_thr, float* score, double* score64);
+
+/* ---------------------------------------------------------------------------
+ * Host-side pieces, callable without a GPU.
+ * rbg_sketch_tables: the per-(element, register) tables the sketch kernel
+ *   consumes.  Element x is the x-th canonical 4-mer in ascending code order.
+ *   T[x*m+k]: value of x's proposal to register k at weight 1 (the weighted
+ *   value is (1/w)*T, src/probmh.cpp:651-677); W[x*m+k]: x's winner identity
+ *   at k (src/probmh.cpp:667); cls[k*136+x]: class byte (smallest element with
+ *   the same 16-bit fold at k, 0xFF when that fold is 0).  Any may be NULL.
+ * rbg_calibrate: simCutoff from per-row maximum match counts, as
+ *   calibrate_sim_cutoff(coverage, true) (regime 1, src/impl/rb_graph.cpp:1362)
+ *   or the 10 x 2500 sampled calibration of gen_fused_calib_graph (regime 2,
+ *   src/impl/rb_graph.cpp:502-525, 1243-1254), consuming the glibc rand()
+ *   stream that srand(rand_seed) starts.
+ * ------------------------------------------------------------------------- */
+rbg_status rbg_sketch_tables(uint32_t m, uint64_t seed, double* T, uint64_t* W, uint8_t* cls);
+rbg_status rbg_calibrate(const uint32_t* rowmax_cnt, uint32_t n, uint32_t m, double b0,
+                         double coverage, uint32_t rand_seed, int regime, double* sim_cutoff);
+
+/* Chance baseline b0 over 16-bit winners (src/rabbitbin.cpp:1340-1369). */
+rbg_status rbg_pmh_baseline(rbg_ctx* ctx, const uint16_t* win16, uint32_t n, uint32_t m,
+                            double* b0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RBG_H */

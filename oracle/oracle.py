@@ -1,0 +1,213 @@
+"""Python binding of the CPU oracle (liboracle.so) and of oracle/_ref.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref_probmh.so")
+
+u8p = C.POINTER(C.c_uint8)
+u16p = C.POINTER(C.c_uint16)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+
+
+class GraphIn(C.Structure):
+    _fields_ = [("win16", u16p), ("unit", f32p), ("anynz", u8p), ("n", C.c_uint64), ("m", C.c_uint32),
+                ("S", C.c_uint32), ("K", C.c_uint32), ("min_edge_weight", C.c_double),
+                ("coverage", C.c_double), ("w_comp", C.c_double), ("b0", C.c_double),
+                ("rand_seed", C.c_uint), ("threads", C.c_int), ("force_regime", C.c_int)]
+
+
+class GraphOut(C.Structure):
+    _fields_ = [("sim_cutoff", C.c_double), ("n_edges", C.c_uint64), ("from_", u32p), ("to", u32p),
+                ("sv", f32p), ("topk_id", u32p), ("topk_sv", f32p), ("topk_len", u32p),
+                ("rowmax_cnt", u32p), ("regime", C.c_int), ("gated_pairs", C.c_uint64)]
+
+
+_orc = None
+_ref = None
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        L = C.CDLL(ORACLE_LIB)
+        vp = C.c_void_p
+        L.orc_fmix64.restype = C.c_uint64
+        L.orc_fmix64.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_wyrand_seq.argtypes = [u64p, C.c_int, u64p]
+        L.orc_wy_uniform_int_seq.argtypes = [u64p, C.c_uint32, C.c_int, u32p]
+        L.orc_zig_seq.argtypes = [u64p, C.c_int, f64p]
+        L.orc_ted_seq.argtypes = [C.c_double, C.c_double, C.c_double, u64p, C.c_int, f64p]
+        L.orc_ted_params.argtypes = [C.c_uint32, f64p, f64p]
+        L.orc_pmh_new.restype = vp
+        L.orc_pmh_new.argtypes = [C.c_uint32, C.c_uint64]
+        L.orc_pmh_free.argtypes = [vp]
+        L.orc_pmh_reset.argtypes = [vp]
+        L.orc_pmh_add_hash.argtypes = [vp, C.c_uint64, C.c_double]
+        L.orc_pmh_get.argtypes = [vp, u64p, f64p]
+        L.orc_kmer4_hist.argtypes = [C.c_char_p, C.c_size_t, u32p, u64p, u64p]
+        L.orc_kmer4_weights.argtypes = [u32p, u64p, C.c_uint64, C.c_double, f64p]
+        L.orc_build_pmh_winners.argtypes = [C.c_char_p, C.c_size_t, C.c_uint32, C.c_uint64, u32p]
+        L.orc_sketch_batch.argtypes = [C.c_char_p, u64p, C.c_size_t, C.c_uint32, C.c_uint64, u32p, C.c_int]
+        L.orc_fold16.argtypes = [u32p, C.c_size_t, u16p]
+        L.orc_depth_features.argtypes = [f32p, C.c_size_t, C.c_size_t, C.c_double, f32p, f32p, f32p, f64p, u8p]
+        L.orc_match_count16.restype = C.c_uint32
+        L.orc_match_count16.argtypes = [u16p, u16p, C.c_uint32]
+        L.orc_pmh_baseline.restype = C.c_double
+        L.orc_pmh_baseline.argtypes = [u16p, C.c_size_t, C.c_uint32]
+        L.orc_unit_dot_f.restype = C.c_float
+        L.orc_unit_dot_f.argtypes = [f32p, f32p, C.c_uint32]
+        L.orc_sv_from_count.restype = C.c_double
+        L.orc_sv_from_count.argtypes = [C.c_uint32, C.c_uint32, C.c_double]
+        L.orc_rand_seq.argtypes = [C.c_uint, C.c_int, C.POINTER(C.c_int32)]
+        L.orc_build_graph.argtypes = [C.POINTER(GraphIn), C.POINTER(GraphOut)]
+        L.orc_graph_free.argtypes = [C.POINTER(GraphOut)]
+        L.orc_edge_weights.argtypes = [u32p, u32p, f32p, C.c_uint64, f32p, f32p, f64p, C.c_uint32, C.c_double,
+                                       C.c_double, C.c_double, C.c_int, f32p, f64p]
+        _orc = L
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_LIB)
+        vp = C.c_void_p
+        L.ref_pmh_new.restype = vp
+        L.ref_pmh_new.argtypes = [C.c_uint32, C.c_int, C.c_uint64]
+        L.ref_pmh_free.argtypes = [vp]
+        L.ref_pmh_reset.argtypes = [vp]
+        L.ref_pmh_add_hash.argtypes = [vp, C.c_uint64, C.c_double]
+        L.ref_pmh_get.argtypes = [vp, u64p, f64p]
+        L.ref_wyrand_seq.argtypes = [u64p, C.c_int, u64p]
+        L.ref_wy_uniform_int_seq.argtypes = [u64p, C.c_uint32, C.c_int, u32p]
+        L.ref_zig_seq.argtypes = [u64p, C.c_int, f64p]
+        L.ref_ted_seq.argtypes = [C.c_double, C.c_double, C.c_double, u64p, C.c_int, f64p]
+        L.ref_fmix64.restype = C.c_uint64
+        L.ref_fmix64.argtypes = [C.c_uint64, C.c_uint64]
+        L.ref_zig_tables.argtypes = [f64p, f64p]
+        _ref = L
+    return _ref
+
+
+# ---- convenience wrappers -------------------------------------------------
+
+def sketch_batch(seqs, offsets, m=500, seed=42, threads=0):
+    buf = bytes(np.ascontiguousarray(seqs, np.uint8)) if not isinstance(seqs, bytes) else seqs
+    off = np.ascontiguousarray(offsets, np.uint64)
+    n = len(off) - 1
+    out = np.zeros((n, m), np.uint32)
+    orc().orc_sketch_batch(buf, _p(off, u64p), n, m, seed, _p(out, u32p), threads)
+    return out
+
+
+def fold16(w32):
+    w = np.ascontiguousarray(w32, np.uint32)
+    out = np.zeros(w.shape, np.uint16)
+    orc().orc_fold16(_p(w, u32p), w.size, _p(out, u16p))
+    return out
+
+
+def depth_features(depth, min_cv=1.0):
+    d = np.ascontiguousarray(depth, np.float32)
+    n, S = d.shape
+    ranked = np.zeros_like(d)
+    unit = np.zeros_like(d)
+    raw = np.zeros_like(d)
+    colnorm = np.zeros(S, np.float64)
+    anynz = np.zeros(n, np.uint8)
+    orc().orc_depth_features(_p(d, f32p), n, S, min_cv, _p(ranked, f32p), _p(unit, f32p), _p(raw, f32p),
+                             _p(colnorm, f64p), _p(anynz, u8p))
+    return ranked, unit, raw, colnorm, anynz
+
+
+def pmh_baseline(win16):
+    w = np.ascontiguousarray(win16, np.uint16)
+    return orc().orc_pmh_baseline(_p(w, u16p), w.shape[0], w.shape[1])
+
+
+def w_comp_for(S):
+    # auto_w_comp (src/rabbitbin.cpp:470-473)
+    return 1.0 if S <= 2 else 0.0
+
+
+def build_graph(win16, unit, anynz, S, b0, rand_seed=1, K=200, min_edge_weight=0.70, coverage=0.95,
+                force_regime=0, threads=0):
+    w = np.ascontiguousarray(win16, np.uint16)
+    n, m = w.shape
+    u = np.ascontiguousarray(unit, np.float32) if unit is not None else None
+    nz = np.ascontiguousarray(anynz, np.uint8) if anynz is not None else None
+    gi = GraphIn(_p(w, u16p), _p(u, f32p), _p(nz, u8p), n, m, S, K, min_edge_weight, coverage,
+                 w_comp_for(S), b0, rand_seed, threads, force_regime)
+    go = GraphOut()
+    st = orc().orc_build_graph(C.byref(gi), C.byref(go))
+    if st != 0:
+        raise RuntimeError(f"orc_build_graph status {st}")
+    ne = go.n_edges
+    res = {
+        "sim_cutoff": go.sim_cutoff, "regime": go.regime, "gated_pairs": go.gated_pairs,
+        "from": np.ctypeslib.as_array(go.from_, (ne,)).copy() if ne else np.zeros(0, np.uint32),
+        "to": np.ctypeslib.as_array(go.to, (ne,)).copy() if ne else np.zeros(0, np.uint32),
+        "sv": np.ctypeslib.as_array(go.sv, (ne,)).copy() if ne else np.zeros(0, np.float32),
+        "topk_id": np.ctypeslib.as_array(go.topk_id, (n, K)).copy(),
+        "topk_sv": np.ctypeslib.as_array(go.topk_sv, (n, K)).copy(),
+        "topk_len": np.ctypeslib.as_array(go.topk_len, (n,)).copy(),
+        "rowmax_cnt": np.ctypeslib.as_array(go.rowmax_cnt, (n,)).copy(),
+    }
+    orc().orc_graph_free(C.byref(go))
+    return res
+
+
+def edge_weights(frm, to, scomp, unit, raw, colnorm, S, mode, min_edge_weight=0.70, neg_thr=-0.3):
+    frm = np.ascontiguousarray(frm, np.uint32)
+    to = np.ascontiguousarray(to, np.uint32)
+    sc = np.ascontiguousarray(scomp, np.float32)
+    u = np.ascontiguousarray(unit, np.float32) if unit is not None else None
+    r = np.ascontiguousarray(raw, np.float32) if raw is not None else None
+    cn = np.ascontiguousarray(colnorm, np.float64) if colnorm is not None else None
+    w = np.zeros(len(frm), np.float32)
+    w64 = np.zeros(len(frm), np.float64)
+    orc().orc_edge_weights(_p(frm, u32p), _p(to, u32p), _p(sc, f32p), len(frm), _p(u, f32p), _p(r, f32p),
+                           _p(cn, f64p), S, w_comp_for(S), min_edge_weight, neg_thr, mode, _p(w, f32p),
+                           _p(w64, f64p))
+    return w, w64
+
+
+def full_pipeline(seqs, offsets, depth, rand_seed=1, m=500, K=200, force_regime=0, threads=0):
+    """Oracle of the whole path: sketches, b0, graph, edge weights (reference arithmetic)."""
+    w32 = sketch_batch(seqs, offsets, m=m, threads=threads)
+    w16 = fold16(w32)
+    b0 = pmh_baseline(w16)
+    S = 0 if depth is None else depth.shape[1]
+    if S:
+        _, unit, raw, colnorm, anynz = depth_features(depth)
+    else:
+        unit = raw = colnorm = anynz = None
+    g = build_graph(w16, unit, anynz, S, b0, rand_seed=rand_seed, K=K, force_regime=force_regime,
+                    threads=threads)
+    g["b0"] = b0
+    g["win16"] = w16
+    if S:
+        g["score"], g["score64"] = edge_weights(g["from"], g["to"], g["sv"], unit, raw,
+                                                colnorm if S > 1 else None, S, mode=1)
+        g["score_ref"], _ = edge_weights(g["from"], g["to"], g["sv"], unit, raw,
+                                         colnorm if S > 1 else None, S, mode=0)
+    return g

@@ -1,0 +1,267 @@
+"""ctypes binding of librbg (include/rbg.h).
+
+Mirrors the reference seams the library replaces:
+  build_pmh_winners      (src/rabbitbin.cpp:1436)      -> Context.build_pmh_winners
+  estimate_pmh_baseline  (src/rabbitbin.cpp:1340)      -> Context.pmh_baseline
+  gen_fused_calib_graph / calibrate_sim_cutoff + build_similarity_graph
+                         (src/impl/rb_graph.cpp:501, 1362, 199) -> Context.build_graph
+  edge-weight loop       (src/rabbitbin.cpp:5482-5509) -> Context.edge_weights
+Errors from the library raise RbgError with the library's message.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import LIB_PATH
+
+u8p = C.POINTER(C.c_uint8)
+u16p = C.POINTER(C.c_uint16)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+
+
+class RbgError(RuntimeError):
+    pass
+
+
+class GraphParams(C.Structure):
+    _fields_ = [
+        ("m", C.c_uint32),
+        ("max_edges", C.c_uint32),
+        ("min_edge_weight", C.c_double),
+        ("coverage", C.c_double),
+        ("min_cv", C.c_double),
+        ("neg_depth_thr", C.c_double),
+        ("pmh_seed", C.c_uint64),
+        ("rand_seed", C.c_uint32),
+        ("regime", C.c_int),
+    ]
+
+
+class GraphResult(C.Structure):
+    _fields_ = [
+        ("n_edges", C.c_uint64),
+        ("from_", u32p),
+        ("to", u32p),
+        ("scomp", f32p),
+        ("score", f32p),
+        ("sim_cutoff", C.c_double),
+        ("b0", C.c_double),
+        ("regime", C.c_int),
+        ("gated_pairs", C.c_uint64),
+        ("ms_sketch", C.c_double),
+        ("ms_pairs", C.c_double),
+        ("ms_select", C.c_double),
+        ("ms_edges", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+
+class GraphDiag(C.Structure):
+    _fields_ = [
+        ("topk_id", u32p),
+        ("topk_cnt", u16p),
+        ("topk_len", u32p),
+        ("rowmax_cnt", u32p),
+        ("win16", u16p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load librbg.so; raise if it is not built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    try:
+        L = C.CDLL(LIB_PATH)
+    except OSError as e:
+        raise RbgError(f"librbg.so is not loadable ({e}); build it with __graft_entry__.build()")
+    vp = C.c_void_p
+    L.rbg_open.argtypes = [C.c_int, C.POINTER(vp)]
+    L.rbg_close.argtypes = [vp]
+    L.rbg_last_error.argtypes = [vp]
+    L.rbg_last_error.restype = C.c_char_p
+    L.rbg_default_params.argtypes = [C.POINTER(GraphParams)]
+    L.rbg_sketch_winners.argtypes = [vp, C.c_char_p, u64p, C.c_uint32, C.c_uint32, C.c_uint64, u32p, u16p]
+    L.rbg_depth_features.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_double, f32p, f32p, f32p, f64p, u8p]
+    L.rbg_build_graph.argtypes = [vp, C.c_char_p, u64p, C.c_uint32, f32p, C.c_uint32,
+                                  C.POINTER(GraphParams), C.POINTER(GraphResult), C.POINTER(GraphDiag)]
+    L.rbg_asm_create.argtypes = [vp, C.c_char_p, u64p, C.c_uint32, f32p, C.c_uint32, C.c_double, C.POINTER(vp)]
+    L.rbg_asm_free.argtypes = [vp]
+    L.rbg_asm_build_graph.argtypes = [vp, vp, C.POINTER(GraphParams), C.POINTER(GraphResult), C.POINTER(GraphDiag)]
+    L.rbg_graph_result_free.argtypes = [C.POINTER(GraphResult)]
+    L.rbg_edge_weights.argtypes = [vp, u32p, u32p, f32p, C.c_uint64, f32p, f32p, f64p, C.c_uint32,
+                                   C.c_uint32, C.c_double, C.c_double, C.c_double, f32p, f64p]
+    L.rbg_pmh_baseline.argtypes = [vp, u16p, C.c_uint32, C.c_uint32, f64p]
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def default_params(**kw):
+    p = GraphParams()
+    lib().rbg_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _as_seqs(seqs, offsets):
+    if isinstance(seqs, (bytes, bytearray)):
+        buf = bytes(seqs)
+    else:
+        buf = np.ascontiguousarray(seqs, dtype=np.uint8).tobytes()
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    return buf, off
+
+
+class Context:
+    """One HIP device + stream (rbg_open)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        self._h = C.c_void_p()
+        st = self._L.rbg_open(device, C.byref(self._h))
+        if st != 0:
+            raise RbgError(f"rbg_open({device}) failed with status {st}")
+
+    def close(self):
+        if self._h:
+            self._L.rbg_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st, what):
+        if st != 0:
+            raise RbgError(f"{what}: status {st}: {self._L.rbg_last_error(self._h).decode()}")
+
+    def build_pmh_winners(self, seqs, offsets, m=500, seed=42):
+        buf, off = _as_seqs(seqs, offsets)
+        n = len(off) - 1
+        w32 = np.zeros((n, m), np.uint32)
+        w16 = np.zeros((n, m), np.uint16)
+        self._check(self._L.rbg_sketch_winners(self._h, buf, _p(off, u64p), n, m, seed,
+                                               _p(w32, u32p), _p(w16, u16p)), "rbg_sketch_winners")
+        return w32, w16
+
+    def pmh_baseline(self, win16):
+        w = np.ascontiguousarray(win16, dtype=np.uint16)
+        b0 = C.c_double()
+        self._check(self._L.rbg_pmh_baseline(self._h, _p(w, u16p), w.shape[0], w.shape[1], C.byref(b0)),
+                    "rbg_pmh_baseline")
+        return b0.value
+
+    def _result(self, res, n, K, diag_arrays):
+        ne = res.n_edges
+        out = {
+            "from": np.ctypeslib.as_array(res.from_, (ne,)).copy() if ne else np.zeros(0, np.uint32),
+            "to": np.ctypeslib.as_array(res.to, (ne,)).copy() if ne else np.zeros(0, np.uint32),
+            "scomp": np.ctypeslib.as_array(res.scomp, (ne,)).copy() if ne else np.zeros(0, np.float32),
+            "score": (np.ctypeslib.as_array(res.score, (ne,)).copy() if (ne and res.score) else None),
+            "sim_cutoff": res.sim_cutoff, "b0": res.b0, "regime": res.regime,
+            "gated_pairs": res.gated_pairs,
+            "ms": {"sketch": res.ms_sketch, "pairs": res.ms_pairs, "select": res.ms_select,
+                   "edges": res.ms_edges, "total": res.ms_total},
+        }
+        if diag_arrays is not None:
+            out.update(diag_arrays)
+        self._L.rbg_graph_result_free(C.byref(res))
+        return out
+
+    def _diag(self, n, K, m, want):
+        if not want:
+            return None, None
+        arrs = {"topk_id": np.zeros((n, K), np.uint32), "topk_cnt": np.zeros((n, K), np.uint16),
+                "topk_len": np.zeros(n, np.uint32), "rowmax_cnt": np.zeros(n, np.uint32),
+                "win16": np.zeros((n, m), np.uint16)}
+        d = GraphDiag(_p(arrs["topk_id"], u32p), _p(arrs["topk_cnt"], u16p), _p(arrs["topk_len"], u32p),
+                      _p(arrs["rowmax_cnt"], u32p), _p(arrs["win16"], u16p))
+        return d, arrs
+
+    def build_graph(self, seqs, offsets, depth=None, params=None, diag=False):
+        buf, off = _as_seqs(seqs, offsets)
+        n = len(off) - 1
+        p = params or default_params()
+        if depth is None:
+            dep, S = None, 0
+        else:
+            dep = np.ascontiguousarray(depth, dtype=np.float32)
+            S = dep.shape[1]
+        d, arrs = self._diag(n, p.max_edges, p.m, diag)
+        res = GraphResult()
+        self._check(self._L.rbg_build_graph(self._h, buf, _p(off, u64p), n, _p(dep, f32p), S, C.byref(p),
+                                            C.byref(res), C.byref(d) if d else None), "rbg_build_graph")
+        return self._result(res, n, p.max_edges, arrs)
+
+    def edge_weights(self, frm, to, scomp, unit, raw, colnorm, S, w_comp, min_edge_weight=0.7,
+                     neg_depth_thr=-0.3):
+        frm = np.ascontiguousarray(frm, np.uint32)
+        to = np.ascontiguousarray(to, np.uint32)
+        sc = np.ascontiguousarray(scomp, np.float32)
+        u = np.ascontiguousarray(unit, np.float32) if unit is not None else None
+        r = np.ascontiguousarray(raw, np.float32) if raw is not None else None
+        cn = np.ascontiguousarray(colnorm, np.float64) if colnorm is not None else None
+        n = u.shape[0] if u is not None else int(max(frm.max(initial=0), to.max(initial=0)) + 1)
+        w = np.zeros(len(frm), np.float32)
+        w64 = np.zeros(len(frm), np.float64)
+        self._check(self._L.rbg_edge_weights(self._h, _p(frm, u32p), _p(to, u32p), _p(sc, f32p), len(frm),
+                                             _p(u, f32p), _p(r, f32p), _p(cn, f64p), n, S, w_comp,
+                                             min_edge_weight, neg_depth_thr, _p(w, f32p), _p(w64, f64p)),
+                    "rbg_edge_weights")
+        return w, w64
+
+
+class Assembly:
+    """Device-resident assembly (rbg_asm_create): contigs + depth features in HBM."""
+
+    def __init__(self, ctx, seqs, offsets, depth=None, min_cv=1.0):
+        self.ctx = ctx
+        buf, off = _as_seqs(seqs, offsets)
+        self.n = len(off) - 1
+        dep = None if depth is None else np.ascontiguousarray(depth, np.float32)
+        self.S = 0 if dep is None else dep.shape[1]
+        self._h = C.c_void_p()
+        ctx._check(ctx._L.rbg_asm_create(ctx._h, buf, _p(off, u64p), self.n, _p(dep, f32p), self.S, min_cv,
+                                         C.byref(self._h)), "rbg_asm_create")
+
+    def build_graph(self, params=None, diag=False):
+        p = params or default_params()
+        d, arrs = self.ctx._diag(self.n, p.max_edges, p.m, diag)
+        res = GraphResult()
+        self.ctx._check(self.ctx._L.rbg_asm_build_graph(self.ctx._h, self._h, C.byref(p), C.byref(res),
+                                                        C.byref(d) if d else None), "rbg_asm_build_graph")
+        return self.ctx._result(res, self.n, p.max_edges, arrs)
+
+    def free(self):
+        if self._h:
+            self.ctx._L.rbg_asm_free(self._h)
+            self._h = C.c_void_p()
+
+
+def depth_features(depth, min_cv=1.0):
+    """rbg_depth_features: ranked, unit, raw, colnorm, anynz."""
+    d = np.ascontiguousarray(depth, np.float32)
+    n, S = d.shape
+    ranked = np.zeros_like(d)
+    unit = np.zeros_like(d)
+    raw = np.zeros_like(d)
+    colnorm = np.zeros(S, np.float64)
+    anynz = np.zeros(n, np.uint8)
+    st = lib().rbg_depth_features(_p(d, f32p), n, S, min_cv, _p(ranked, f32p), _p(unit, f32p), _p(raw, f32p),
+                                  _p(colnorm, f64p), _p(anynz, u8p))
+    if st != 0:
+        raise RbgError(f"rbg_depth_features: status {st}")
+    return ranked, unit, raw, colnorm, anynz

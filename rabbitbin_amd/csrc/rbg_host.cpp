@@ -1,0 +1,409 @@
+// Host-side pieces of librbg (see rbg_host.h).  Compiled with
+// -ffp-contract=off: every multiply-add that the reference build fuses is an
+// explicit std::fma here, every other operation is a separate IEEE op.
+#include "rbg_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <random>
+
+#include "zig_tables.h"
+
+namespace rbg {
+namespace {
+
+// ---- wyrand stream (src/probmh.cpp:60-87) --------------------------------
+inline uint64_t wy_next(uint64_t& s) {
+  s += 0xa0761d6478bd642fULL;
+  const __uint128_t r = (__uint128_t)s * (s ^ 0xe7037ed1a0b428dbULL);
+  return (uint64_t)(r >> 64) ^ (uint64_t)r;
+}
+inline double to_unit(uint64_t bits) { return (double)(bits >> 11) * 0x1p-53; }
+
+inline uint32_t wy_below(uint32_t bound, uint64_t& s) {
+  uint64_t prod = (wy_next(s) >> 32) * (uint64_t)bound;
+  if ((uint32_t)prod < bound) {
+    const uint32_t reject_below = (uint32_t)(0u - bound) % bound;
+    while ((uint32_t)prod < reject_below) prod = (wy_next(s) >> 32) * (uint64_t)bound;
+  }
+  return (uint32_t)(prod >> 32);
+}
+
+// ---- samplers -------------------------------------------------------------
+struct TedRow {
+  double boundary, gap, c1, c2, c3;
+};
+
+// Truncated exponential on [0,1) (src/probmh.cpp:344-357).
+double trunc_exp(const TedRow& t, uint64_t& s) {
+  double x = to_unit(wy_next(s)) * t.c1;
+  if (x < 1.0) return x;
+  for (;;) {
+    x = to_unit(wy_next(s));
+    if (x <= t.c2) return x;
+    double y = to_unit(wy_next(s)) * 0.5;
+    if (y > 1.0 - x) {
+      x = 1.0 - x;
+      y = 1.0 - y;
+    }
+    if (x <= t.c3 * (1.0 - y)) return x;
+    if (t.c1 * y <= 1.0 - x) return x;
+  }
+}
+
+// Exp(1) by the 256-layer ziggurat (src/probmh.cpp:319-338); the slow-path
+// interpolations are single fused operations in the reference build.
+double exp1(uint64_t& s) {
+  double shift = 0.0;
+  for (;;) {
+    const uint64_t bits = wy_next(s);
+    const int layer = (int)(bits & 0xFF);
+    const double x = to_unit(bits) * kZigX[layer];
+    if (x < kZigX[layer + 1]) return shift + x;
+    if (layer == 0) {
+      shift += kZigX[1];
+      continue;
+    }
+    const double y01 = to_unit(wy_next(s));
+    const double y = std::fma(y01, kZigY[layer + 1] - kZigY[layer], kZigY[layer]);
+    const double above_upper =
+        std::fma(kZigX[layer] - kZigX[layer + 1], y01, -(kZigX[layer] - x));
+    const double above_lower =
+        y - std::fma(kZigX[layer + 1] - x, kZigY[layer + 1], kZigY[layer + 1]);
+    if (above_upper < 0 && (above_lower < 0 || y < std::exp(-x))) return x + shift;
+  }
+}
+
+// Boundaries and sampler constants of the m-1 proposal intervals
+// (src/probmh.cpp:506-572).
+void ted_rows(uint32_t m, std::vector<TedRow>& rows, double& first_inv) {
+  rows.assign(m - 1, TedRow{});
+  const double first = std::log1p(1.0 / (double)(m - 1));
+  first_inv = 1.0 / first;
+  auto consts = [](double rate, TedRow& r) {
+    r.c1 = (rate != 0.0) ? std::expm1(rate) / rate : 1.0;
+    r.c2 = (rate != 0.0) ? -std::log1p(std::expm1(-rate) * 0.5) / rate : 0.5;
+    r.c3 = (rate != 0.0) ? -std::expm1(-rate) / rate : 1.0;
+  };
+  rows[0].boundary = 1.0;
+  rows[0].gap = 1.0;
+  consts(first, rows[0]);
+  double prev = first;
+  for (uint32_t i = 1; i + 1 < m; ++i) {
+    const double b = std::log1p((double)(i + 1) / (double)(m - i - 1));
+    rows[i].boundary = b / first;
+    rows[i].gap = rows[i].boundary - rows[i - 1].boundary;
+    consts(b - prev, rows[i]);
+    prev = b;
+  }
+}
+
+// All m proposals of one element against an empty sketch: proposal p goes to
+// the p-th register of a Fisher-Yates permutation drawn from the element's
+// own stream; its winner identity is the stream state right after the draw.
+void element_proposals(uint32_t m, uint64_t rng, const std::vector<TedRow>& ted,
+                       double first_inv, double* T, uint64_t* W) {
+  std::vector<uint32_t> perm(m);
+  std::iota(perm.begin(), perm.end(), 0u);
+  const double u0 = to_unit(rng) * ted[0].c1;
+  double hv = (u0 < 1.0) ? u0 : trunc_exp(ted[0], rng);
+  for (uint32_t p = 1; p <= m; ++p) {
+    const uint32_t j = (p - 1) + wy_below(m - (p - 1), rng);
+    const uint32_t reg = perm[j];
+    perm[j] = perm[p - 1];
+    T[reg] = hv;
+    W[reg] = rng;
+    if (p + 1 < m)
+      hv = std::fma(ted[p].gap, trunc_exp(ted[p], rng), ted[p - 1].boundary);
+    else if (p + 1 == m)
+      hv = std::fma(first_inv, exp1(rng), ted[m - 2].boundary);
+  }
+}
+
+inline uint64_t fmix64(uint64_t x, uint64_t seed) {
+  x ^= seed;
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+inline uint16_t fold16(uint32_t w) {
+  if (w == 0) return 0;
+  const uint16_t h = (uint16_t)(w ^ (w >> 16));
+  return h ? h : (uint16_t)1;
+}
+
+struct CodeTables {
+  uint8_t codes[kElems];
+  int16_t elem[256];
+  CodeTables() {
+    int x = 0;
+    for (int v = 0; v < 256; ++v) {
+      int rc = 0;
+      for (int d = 0; d < 4; ++d) rc |= (3 - ((v >> (2 * d)) & 3)) << (2 * (3 - d));
+      if (v <= rc) {
+        codes[x] = (uint8_t)v;
+        elem[v] = (int16_t)x++;
+      } else {
+        elem[v] = -1;
+      }
+    }
+  }
+};
+const CodeTables& code_tables() {
+  static const CodeTables t;
+  return t;
+}
+
+}  // namespace
+
+const uint8_t* canonical_codes() { return code_tables().codes; }
+const int16_t* element_of_code() { return code_tables().elem; }
+
+const ProposalTables& proposal_tables(uint32_t m, uint64_t seed) {
+  static std::mutex mu;
+  static std::map<std::pair<uint32_t, uint64_t>, ProposalTables> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_pair(m, seed);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  ProposalTables t;
+  t.m = m;
+  t.seed = seed;
+  t.T.resize((size_t)kElems * m);
+  t.W.resize((size_t)kElems * m);
+  t.W32.resize((size_t)kElems * m);
+  t.W16.resize((size_t)kElems * m);
+  t.cls.resize((size_t)m * kElems);
+  std::vector<TedRow> ted;
+  double first_inv;
+  ted_rows(m, ted, first_inv);
+  const uint8_t* codes = canonical_codes();
+  for (int x = 0; x < kElems; ++x)
+    element_proposals(m, fmix64(codes[x], seed), ted, first_inv, &t.T[(size_t)x * m],
+                      &t.W[(size_t)x * m]);
+  for (size_t q = 0; q < t.W.size(); ++q) {
+    t.W32[q] = (uint32_t)(t.W[q] ^ (t.W[q] >> 32));
+    t.W16[q] = fold16(t.W32[q]);
+  }
+  for (uint32_t k = 0; k < m; ++k)
+    for (int x = 0; x < kElems; ++x) {
+      const uint16_t f = t.W16[(size_t)x * m + k];
+      uint8_t c = 0xFF;
+      if (f != 0)
+        for (int y = 0; y <= x; ++y)
+          if (t.W16[(size_t)y * m + k] == f) {
+            c = (uint8_t)y;
+            break;
+          }
+      t.cls[(size_t)k * kElems + x] = c;
+    }
+  return cache.emplace(key, std::move(t)).first->second;
+}
+
+void sv_tables(uint32_t m, double b0, std::vector<double>& svd, std::vector<float>& svf,
+               std::vector<uint16_t>& level) {
+  svd.resize(m + 1);
+  svf.resize(m + 1);
+  level.resize(m + 1);
+  const double inv_m = 1.0 / (double)m;
+  const bool rescale = b0 > 0.0 && b0 < 1.0;
+  const double inv1mb0 = rescale ? 1.0 / (1.0 - b0) : 1.0;
+  for (uint32_t c = 0; c <= m; ++c) {
+    double j = (double)c * inv_m;
+    if (rescale) {
+      j = (j - b0) * inv1mb0;
+      if (j < 0.0) j = 0.0;
+    }
+    if (j > 1.0 - 1e-6) j = 1.0 - 1e-6;
+    svd[c] = j;
+    svf[c] = (float)j;
+  }
+  // level: dense rank of the distinct float similarities (monotone in c)
+  uint16_t lv = 0;
+  for (uint32_t c = 0; c <= m; ++c) {
+    if (c > 0 && svf[c] != svf[c - 1]) ++lv;
+    level[c] = lv;
+  }
+}
+
+void depth_features(const float* depth, size_t n, size_t S, double min_cv, float* ranked,
+                    float* unit, float* raw, double* colnorm, uint8_t* anynz) {
+  std::vector<float> rk(depth, depth + n * S);
+  if (S > 1) {
+    if (raw) std::memcpy(raw, depth, n * S * sizeof(float));
+    if (colnorm)
+      for (size_t c = 0; c < S; ++c) {
+        double sum = 0.0;
+        for (size_t r = 0; r < n; ++r) sum += depth[r * S + c];
+        const double mean = sum / (double)std::max<size_t>(n, 1);
+        colnorm[c] = (mean > 1e-12) ? 1.0 / mean : 0.0;
+      }
+    // average ranks, ties share the mean position (src/ranker.h:55-66)
+#pragma omp parallel for schedule(static)
+    for (size_t r = 0; r < n; ++r) {
+      float* row = &rk[r * S];
+      std::vector<uint32_t> order(S);
+      std::iota(order.begin(), order.end(), 0u);
+      std::sort(order.begin(), order.end(),
+                [row](uint32_t a, uint32_t b) { return row[a] < row[b]; });
+      std::vector<float> rank(S);
+      size_t c = 0;
+      while (c < S) {
+        size_t tie = 1;
+        while (c + tie < S && row[order[c]] == row[order[c + tie]]) ++tie;
+        const float v = (float)(2 * c + tie - 1) / 2 + 1;
+        for (size_t q = 0; q < tie; ++q) rank[order[c + q]] = v;
+        c += tie;
+      }
+      std::copy(rank.begin(), rank.end(), row);
+    }
+  }
+  if (ranked) std::memcpy(ranked, rk.data(), n * S * sizeof(float));
+  if (unit) {
+    std::fill(unit, unit + n * S, 0.0f);
+    if (S > 1) {
+#pragma omp parallel for schedule(static)
+      for (size_t r = 0; r < n; ++r) {
+        const float* row = &rk[r * S];
+        double mean = 0.0;
+        for (size_t k = 0; k < S; ++k) mean += row[k];
+        mean /= (double)S;
+        double ss = 0.0;
+        for (size_t k = 0; k < S; ++k) {
+          const double d = (double)row[k] - mean;
+          ss = std::fma(d, d, ss);  // fused in the reference build
+        }
+        if (ss > 0.0) {
+          const double inv = 1.0 / std::sqrt(ss);
+          for (size_t k = 0; k < S; ++k) unit[r * S + k] = (float)(((double)row[k] - mean) * inv);
+        }
+      }
+    }
+  }
+  if (anynz)
+    for (size_t r = 0; r < n; ++r) {
+      uint8_t any = 0;
+      for (size_t c = 0; c < S && !any; ++c) any = (double)rk[r * S + c] > min_cv;
+      anynz[r] = any;
+    }
+}
+
+RandStream::RandStream(unsigned seed) {
+  std::memset(&rd_, 0, sizeof(rd_));
+  initstate_r(seed, state_, sizeof(state_), &rd_);
+}
+int RandStream::next() {
+  int32_t r;
+  random_r(&rd_, &r);
+  return r;
+}
+
+namespace {
+// calib_converge: walk p down from 999 until coverage is reached.
+size_t converge(const std::vector<double>& maxsim, double coverage, RandStream& rs) {
+  const size_t n = maxsim.size();
+  size_t p = 999, prev_p = 1000;
+  double cov = 0, prev_cov = 0;
+  while (p > 50) {
+    const double cut = (double)p / 1000.;
+    const size_t on = (size_t)std::count_if(maxsim.begin(), maxsim.end(),
+                                            [cut](double v) { return v >= cut; });
+    cov = (double)on / n;
+    if (cov >= coverage) {
+      if (cov - coverage > coverage - prev_cov) p = prev_p;
+      break;
+    }
+    prev_p = p;
+    prev_cov = cov;
+    const int r = rs.next() % 3;
+    p -= (size_t)(p > 990 ? r + 1 : p > 900 ? r + 3 : r + 9);
+  }
+  return p;
+}
+
+void draw_prefix(std::vector<size_t>& v, size_t count, RandStream& rs) {
+  size_t left = v.size();
+  for (size_t s = 0; s < count; ++s, --left) {
+    const size_t r = (size_t)rs.next() % left;
+    std::swap(v[s], v[s + r]);
+  }
+}
+}  // namespace
+
+double calibrate_small(const std::vector<uint32_t>& rowmax, const std::vector<double>& svd,
+                       double coverage, unsigned rand_seed) {
+  const size_t n = rowmax.size();
+  RandStream rs(rand_seed);
+  std::vector<size_t> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  draw_prefix(idx, n, rs);  // consumes n draws; the order does not matter below
+  std::vector<double> maxsim(n);
+  for (size_t t = 0; t < n; ++t) maxsim[t] = n > 1 ? svd[rowmax[idx[t]]] : 0.0;
+  return (double)converge(maxsim, coverage, rs);
+}
+
+double calibrate_fused(const std::vector<uint32_t>& rowmax, const std::vector<float>& svf,
+                       double coverage, unsigned rand_seed) {
+  const size_t n = rowmax.size();
+  constexpr size_t R = 10, SAMP = 2500;
+  RandStream rs(rand_seed);
+  std::vector<int8_t> round_of(n, -1);
+  std::vector<uint32_t> local_of(n, 0);
+  for (size_t r = 0; r < R; ++r) {
+    std::vector<size_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    draw_prefix(idx, SAMP, rs);
+    for (size_t li = 0; li < SAMP; ++li)
+      if (round_of[idx[li]] < 0) {  // a contig counts only in its lowest round
+        round_of[idx[li]] = (int8_t)r;
+        local_of[idx[li]] = (uint32_t)li;
+      }
+  }
+  std::vector<std::vector<double>> maxsim(R, std::vector<double>(SAMP, 0.0));
+  for (size_t c = 0; c < n; ++c)
+    if (round_of[c] >= 0) maxsim[round_of[c]][local_of[c]] = (double)svf[rowmax[c]];
+  double sum_p = 0;
+  for (size_t r = 0; r < R; ++r) {
+    double p = (double)converge(maxsim[r], coverage, rs);
+    if (p < 51.0) p = 50.0;
+    sum_p += p;
+    if (r == 1 && sum_p / 2 < 51.0) {
+      sum_p = 50.0 * R;
+      break;
+    }
+  }
+  return sum_p / (double)R;
+}
+
+void baseline_pairs(size_t n, std::vector<uint32_t>& pi, std::vector<uint32_t>& pj) {
+  pi.clear();
+  pj.clear();
+  if (n < 4) return;
+  std::mt19937_64 rng(0x9E3779B97F4A7C15ULL);
+  std::uniform_int_distribution<size_t> uni(0, n - 1);
+  for (size_t s = 0; s < 30000; ++s) {
+    const size_t i = uni(rng), j = uni(rng);
+    if (i == j) continue;
+    pi.push_back((uint32_t)i);
+    pj.push_back((uint32_t)j);
+  }
+}
+
+double baseline_median(const std::vector<uint32_t>& counts, uint32_t m) {
+  if (counts.empty()) return 0.0;
+  const double inv_m = 1.0 / (double)m;
+  std::vector<float> vals(counts.size());
+  for (size_t t = 0; t < counts.size(); ++t) vals[t] = (float)((double)counts[t] * inv_m);
+  std::nth_element(vals.begin(), vals.begin() + vals.size() / 2, vals.end());
+  return (double)vals[vals.size() / 2];
+}
+
+}  // namespace rbg

@@ -1,0 +1,47 @@
+// Kernel launchers (internal to librbg).  All launch on `st` and return the
+// launch status; shapes are checked by the callers in rbg_api.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rbg {
+
+hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
+                         uint32_t W, const double* T, const uint8_t* cls, const uint8_t* codes,
+                         uint8_t* wx, uint32_t mpad, uint32_t* planes, hipStream_t st);
+hipError_t launch_list_counts(const uint32_t* planes, uint32_t W, const uint32_t* pi,
+                              const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st);
+hipError_t launch_win16_counts(const uint16_t* win16, uint32_t m, const uint32_t* pi,
+                               const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st);
+hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, uint32_t n, uint16_t* out,
+                               hipStream_t st);
+hipError_t launch_gate(const float* unit, uint32_t NC, const uint8_t* anynz, uint32_t n, float thr,
+                       uint32_t acc1_end, uint64_t* surv, unsigned long long* nsurv, uint64_t cap,
+                       hipStream_t st);
+hipError_t launch_survivor_counts(const uint32_t* planes, uint32_t W, const uint64_t* surv,
+                                  uint64_t ns, uint16_t* cnt, uint32_t* deg, uint32_t* rowmax,
+                                  hipStream_t st);
+hipError_t launch_scatter(const uint64_t* surv, const uint16_t* cnt, uint64_t ns,
+                          const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st);
+hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32_t n,
+                               const uint16_t* level, uint32_t nlevels, uint32_t K,
+                               uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
+                               uint32_t* rowmax, hipStream_t st);
+hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_t n,
+                              const uint16_t* level, uint32_t nlevels, uint32_t K,
+                              uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
+                              uint32_t* rowmax, hipStream_t st);
+hipError_t launch_emit(const uint32_t* topk_id, const uint16_t* topk_cnt, const uint32_t* topk_len,
+                       uint32_t n, uint32_t K, uint32_t cmin, int mutual, int pass,
+                       uint32_t* row_edges, const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
+                       uint16_t* ecnt, hipStream_t st);
+hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const float* scomp,
+                               uint64_t ne, const float* unit, const float* raw,
+                               const double* colnorm, uint32_t S, double w_comp, double min_edge,
+                               double neg_thr, float* score, double* score64, hipStream_t st);
+hipError_t launch_cnt_to_sv(const uint16_t* c, uint64_t ne, const float* svf, float* out,
+                            hipStream_t st);
+hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n, void* tmp,
+                                     size_t& tmp_bytes, hipStream_t st);
+
+}  // namespace rbg

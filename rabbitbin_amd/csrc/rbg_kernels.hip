@@ -1,0 +1,914 @@
+// HIP kernels for gfx950 (MI355X, CDNA4, wave64).
+//
+// Data layout in HBM (all row-major, one row per contig):
+//   seqs     contig bytes back to back, offsets[n+1]
+//   wx       n x mpad  u8   winning element (0..135) per register, 0xFF unset
+//   planes   n x 8W    u32  register classes bit-sliced: word (w, p) holds bit
+//                           p of the class byte of registers 32w..32w+31.
+//                           Unset registers and the pad registers >= m hold
+//                           0xFF; a class is < 136, so 0xFF never equals one.
+//   unit     n x 16NC  f32  centred, L2-normalised rank rows, zero padded
+//
+// Register-match count of a pair (row a, column b):
+//   cnt = sum_w popcount( ~OR_p (a[w][p] ^ b'[w][p]) )
+// where b' is b with plane 0 cleared at b's unset/pad registers, so an unset
+// register shows 0xFF on the row side and 0xFE on the column side and never
+// counts.  A class byte is equal exactly when the reference's 16-bit folded
+// winners are equal and non-zero (src/rabbitbin.cpp:1261-1293), because the
+// class of (element x, register k) is the smallest element with the same
+// 16-bit fold at k.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rbg_kernels.h"
+
+namespace rbg {
+namespace {
+
+constexpr int kElemCount = 136;
+
+__device__ __forceinline__ uint32_t base_code(uint32_t c) {
+  const uint32_t u = c & 0xDFu;  // fold lower case; only a/A..t/T map onto A..T
+  return u == 'A' ? 0u : u == 'C' ? 1u : u == 'G' ? 2u : u == 'T' ? 3u : 255u;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Column-side transform of one 32-register word group: clear plane 0 where
+// the column register is unset (all eight bits set).
+__device__ __forceinline__ void column_word(const uint32_t* __restrict__ src, uint32_t* dst) {
+  uint32_t all = src[0];
+#pragma unroll
+  for (int p = 1; p < 8; ++p) all &= src[p];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) dst[p] = src[p];
+  dst[0] &= ~all;
+}
+
+__device__ __forceinline__ uint32_t word_mismatch(const uint32_t* a, const uint32_t* b) {
+  uint32_t mm = a[0] ^ b[0];
+#pragma unroll
+  for (int p = 1; p < 8; ++p) mm |= a[p] ^ b[p];
+  return mm;
+}
+
+__device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__ ra,
+                                                    const uint32_t* __restrict__ rb,
+                                                    uint32_t W) {
+  uint32_t mism = 0;
+  for (uint32_t w = 0; w < W; ++w) {
+    uint32_t a[8], b[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) a[p] = ra[8 * w + p];
+    column_word(rb + 8 * w, b);
+    mism += __popc(word_mismatch(a, b));
+  }
+  return 32u * W - mism;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a): sketch.  One workgroup sketches CPB contigs.
+// Phase 1 (per contig): canonical 4-mer histogram in LDS; a non-ACGT byte
+// breaks every 4-mer that covers it (src/rabbitbin.cpp:1467-1484).  Then the
+// GC-normalised weights in FP64, in the reference's operation order
+// (src/rabbitbin.cpp:1491-1516), kept as 1/w per element.
+// Phase 2: thread t owns registers k = t, t+256, ...; for each register the
+// winner is argmin_x fl((1/w_x) * T[x][k]) over present elements, ties to the
+// smaller x (the sketch's strict-less update, src/probmh.cpp:417-431).
+// The proposal table T is read once per workgroup for all CPB contigs.
+// ---------------------------------------------------------------------------
+template <int CPB>
+__global__ __launch_bounds__(256) void sketch_kernel(
+    const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
+    uint32_t W, const double* __restrict__ T, const uint8_t* __restrict__ cls,
+    const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx, uint32_t mpad,
+    uint32_t* __restrict__ planes) {
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t bases[4];
+  __shared__ uint32_t total;
+  __shared__ double winv[CPB][kElemCount];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t c0 = blockIdx.x * CPB;
+
+  for (int q = 0; q < CPB; ++q) {
+    const uint32_t c = c0 + q;
+    for (uint32_t v = tid; v < 256; v += 256) hist[v] = 0;
+    if (tid < 4) bases[tid] = 0;
+    if (tid == 0) total = 0;
+    __syncthreads();
+    if (c < n) {
+      const uint64_t beg = offs[c];
+      const int64_t len = (int64_t)(offs[c + 1] - beg);
+      const uint8_t* s = seqs + beg;
+      uint32_t lb0 = 0, lb1 = 0, lb2 = 0, lb3 = 0, kmers = 0;
+      for (int64_t p0 = (int64_t)tid * 4; p0 < len; p0 += 1024) {
+        uint32_t e[7];
+#pragma unroll
+        for (int t = 0; t < 7; ++t) {
+          const int64_t pos = p0 - 3 + t;
+          e[t] = (pos >= 0 && pos < len) ? base_code(s[pos]) : 255u;
+        }
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) {
+          const uint32_t ep = e[3 + q2];
+          if (p0 + q2 >= len || ep == 255u) continue;
+          lb0 += ep == 0u;
+          lb1 += ep == 1u;
+          lb2 += ep == 2u;
+          lb3 += ep == 3u;
+          if (e[q2] != 255u && e[q2 + 1] != 255u && e[q2 + 2] != 255u) {
+            const uint32_t fwd = (e[q2] << 6) | (e[q2 + 1] << 4) | (e[q2 + 2] << 2) | ep;
+            const uint32_t rev =
+                ((3u - ep) << 6) | ((3u - e[q2 + 2]) << 4) | ((3u - e[q2 + 1]) << 2) | (3u - e[q2]);
+            atomicAdd(&hist[fwd <= rev ? fwd : rev], 1u);
+            ++kmers;
+          }
+        }
+      }
+      if (lb0) atomicAdd(&bases[0], lb0);
+      if (lb1) atomicAdd(&bases[1], lb1);
+      if (lb2) atomicAdd(&bases[2], lb2);
+      if (lb3) atomicAdd(&bases[3], lb3);
+      if (kmers) atomicAdd(&total, kmers);
+    }
+    __syncthreads();
+    if (tid < kElemCount) {
+      double wi = 0.0;
+      const uint32_t v = codes[tid];
+      const uint32_t cnt = hist[v];
+      if (c < n && cnt != 0 && total != 0) {
+        const uint64_t vb = (uint64_t)bases[0] + bases[1] + bases[2] + bases[3];
+        double p[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) p[b] = vb > 0 ? (double)bases[b] / (double)vb : 0.25;
+        const double inv = 1.0 / (double)total;
+        double pf = 1.0, pr = 1.0;
+        uint32_t code = v;
+#pragma unroll
+        for (int pos = 0; pos < 4; ++pos) {
+          const uint32_t b = code & 3u;
+          pf *= p[b];
+          pr *= p[3u - b];
+          code >>= 2;
+        }
+        const double obs = (double)cnt * inv;
+        const double ex = pf + pr;
+        double w = 0.0;
+        if (ex > 1e-14) {
+          w = obs / ex;
+          if (w > 20.0) w = 20.0;
+        }
+        if (w > 0.0) wi = 1.0 / w;
+      }
+      winv[q][tid] = wi;
+    }
+    __syncthreads();
+  }
+
+  const uint32_t kspan = ((32u * W + 255u) / 256u) * 256u;
+  for (uint32_t k = tid; k < kspan; k += 256) {
+    double best[CPB];
+    int bx[CPB];
+#pragma unroll
+    for (int q = 0; q < CPB; ++q) {
+      best[q] = __builtin_huge_val();
+      bx[q] = -1;
+    }
+    if (k < m) {
+      for (int x = 0; x < kElemCount; ++x) {
+        const double t = T[(size_t)x * m + k];
+#pragma unroll
+        for (int q = 0; q < CPB; ++q) {
+          const double wv = winv[q][x];
+          if (wv != 0.0) {
+            const double hv = wv * t;
+            if (hv < best[q]) {
+              best[q] = hv;
+              bx[q] = x;
+            }
+          }
+        }
+      }
+    }
+    const uint32_t w0 = k / 32u;  // lanes 0..31 -> w0, lanes 32..63 -> w0+1
+#pragma unroll
+    for (int q = 0; q < CPB; ++q) {
+      const uint32_t c = c0 + q;
+      uint32_t code = 0xFFu;
+      if (k < m && bx[q] >= 0) code = cls[(size_t)k * kElemCount + bx[q]];
+      if (c < n && wx != nullptr && k < mpad)
+        wx[(size_t)c * mpad + k] = (k < m && bx[q] >= 0) ? (uint8_t)bx[q] : (uint8_t)0xFF;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        const uint64_t bits = __ballot((code >> p) & 1u);
+        const uint32_t ln = lane_id();
+        if (c < n && (ln == 0 || ln == 32) && w0 < W)
+          planes[(size_t)c * 8u * W + 8u * w0 + p] = ln == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
+      }
+    }
+  }
+}
+
+// Counts for an explicit list of pairs (b0 sampling).
+__global__ __launch_bounds__(256) void list_counts_kernel(const uint32_t* __restrict__ planes,
+                                                          uint32_t W, const uint32_t* __restrict__ pi,
+                                                          const uint32_t* __restrict__ pj,
+                                                          uint32_t np, uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= np) return;
+  out[t] = pair_count_rows(planes + (size_t)pi[t] * 8u * W, planes + (size_t)pj[t] * 8u * W, W);
+}
+
+// 16-bit winner compare for a pair list (src/rabbitbin.cpp:1261-1293).
+__global__ __launch_bounds__(256) void win16_counts_kernel(const uint16_t* __restrict__ w16,
+                                                           uint32_t m, const uint32_t* __restrict__ pi,
+                                                           const uint32_t* __restrict__ pj,
+                                                           uint32_t np, uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= np) return;
+  const uint16_t* a = w16 + (size_t)pi[t] * m;
+  const uint16_t* b = w16 + (size_t)pj[t] * m;
+  uint32_t c = 0;
+  for (uint32_t k = 0; k < m; ++k) c += (a[k] != 0 && a[k] == b[k]) ? 1u : 0u;
+  out[t] = c;
+}
+
+// ---------------------------------------------------------------------------
+// Small regime (n <= 25000): every ordered pair's count into an n x n u16
+// matrix.  Thread = column j with its bit planes in VGPRs; rows are uniform
+// across the wave and come in through scalar loads.
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __restrict__ planes,
+                                                           uint32_t n, uint32_t rows_per_block,
+                                                           uint16_t* __restrict__ out) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t r0 = blockIdx.y * rows_per_block;
+  const uint32_t r1 = min(n, r0 + rows_per_block);
+  uint32_t b[8 * W];
+  const uint32_t jj = j < n ? j : n - 1;
+#pragma unroll
+  for (int w = 0; w < W; ++w) column_word(planes + (size_t)jj * 8 * W + 8 * w, b + 8 * w);
+  for (uint32_t r = r0; r < r1; ++r) {
+    const uint32_t* __restrict__ a = planes + (size_t)r * 8 * W;
+    uint32_t mism = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) mism += __popc(word_mismatch(a + 8 * w, b + 8 * w));
+    if (j < n) out[(size_t)r * n + j] = (uint16_t)(32u * W - mism);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused regime, abundance gate (src/impl/rb_graph.cpp:1108-1116): keep pair
+// (i < j) iff is_nz(i, j) and unit_dot_f(u_i, u_j) >= thr, where unit_dot_f
+// is evaluated in the AVX-512 order of the reference build
+// (src/rabbitbin.cpp:754-776): 16 lanes; chunk c (16 elements, zero padded)
+// accumulates into acc1 when c is odd and lies inside a full 32-wide chunk
+// (c < acc1_end), else into acc0; each lane's chain is one FMA per chunk in
+// chunk order; then acc0+acc1 and GCC's _mm512_reduce_add_ps tree.
+// Padding lanes add fma(0,0,acc) = acc, matching the masked tail.
+// Survivors (i, j) are appended to `surv`.
+// ---------------------------------------------------------------------------
+template <int NC>
+__global__ __launch_bounds__(256) void gate_kernel(const float* __restrict__ unit,
+                                                   const uint8_t* __restrict__ anynz, uint32_t n,
+                                                   float thr, uint32_t acc1_end,
+                                                   uint32_t rows_per_block,
+                                                   uint64_t* __restrict__ surv,
+                                                   unsigned long long* __restrict__ nsurv,
+                                                   uint64_t cap) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t r0 = blockIdx.y * rows_per_block;
+  const uint32_t jmax = min(n, blockIdx.x * 256 + 256);
+  if (r0 + 1 >= jmax) return;  // block lies on or below the diagonal
+  const uint32_t r1 = min(jmax - 1, r0 + rows_per_block);
+  const uint32_t jj = j < n ? j : n - 1;
+  float b[16 * NC];
+#pragma unroll
+  for (int t = 0; t < 16 * NC; ++t) b[t] = unit[(size_t)jj * 16 * NC + t];
+  const bool nz_j = anynz ? anynz[jj] != 0 : true;
+  for (uint32_t i = r0; i < r1; ++i) {
+    const float* __restrict__ a = unit + (size_t)i * 16 * NC;
+    float acc0[16], acc1[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+      acc0[l] = 0.0f;
+      acc1[l] = 0.0f;
+    }
+    bool used1 = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if ((c & 1) && (uint32_t)c < acc1_end) {
+        used1 = true;
+#pragma unroll
+        for (int l = 0; l < 16; ++l) acc1[l] = __builtin_fmaf(a[16 * c + l], b[16 * c + l], acc1[l]);
+      } else {
+#pragma unroll
+        for (int l = 0; l < 16; ++l) acc0[l] = __builtin_fmaf(a[16 * c + l], b[16 * c + l], acc0[l]);
+      }
+    }
+    if (used1) {
+#pragma unroll
+      for (int l = 0; l < 16; ++l) acc0[l] = acc0[l] + acc1[l];
+    }
+    float t8[8], t4[4];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) t8[l] = acc0[l + 8] + acc0[l];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) t4[l] = t8[l + 4] + t8[l];
+    const float dot = (t4[0] + t4[2]) + (t4[1] + t4[3]);
+    const bool nz = nz_j || (anynz ? anynz[i] != 0 : true);
+    const bool pass = j < n && i < j && nz && !(dot < thr);
+    const uint64_t mask = __ballot(pass);
+    if (mask) {
+      unsigned long long base = 0;
+      if (lane_id() == 0) base = atomicAdd(nsurv, (unsigned long long)__popcll(mask));
+      base = __shfl(base, 0);
+      if (pass) {
+        const uint64_t slot = base + lanes_below(mask);
+        if (slot < cap) surv[slot] = ((uint64_t)i << 32) | j;
+      }
+    }
+  }
+}
+
+// Register compare of the gated pairs; per-row degree and maximum count.
+__global__ __launch_bounds__(256) void survivor_counts_kernel(
+    const uint32_t* __restrict__ planes, uint32_t W, const uint64_t* __restrict__ surv,
+    uint64_t ns, uint16_t* __restrict__ cnt, uint32_t* __restrict__ deg,
+    uint32_t* __restrict__ rowmax) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= ns) return;
+  const uint32_t i = (uint32_t)(surv[t] >> 32), j = (uint32_t)surv[t];
+  const uint32_t c =
+      pair_count_rows(planes + (size_t)i * 8u * W, planes + (size_t)j * 8u * W, W);
+  cnt[t] = (uint16_t)c;
+  atomicAdd(&deg[i], 1u);
+  atomicAdd(&deg[j], 1u);
+  atomicMax(&rowmax[i], c);
+  atomicMax(&rowmax[j], c);
+}
+
+// Scatter each gated pair into both rows' candidate lists: entry = (cnt<<32)|id.
+__global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict__ surv,
+                                                      const uint16_t* __restrict__ cnt, uint64_t ns,
+                                                      const uint64_t* __restrict__ off,
+                                                      uint32_t* __restrict__ cursor,
+                                                      uint64_t* __restrict__ list) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= ns) return;
+  const uint32_t i = (uint32_t)(surv[t] >> 32), j = (uint32_t)surv[t];
+  const uint64_t c = (uint64_t)cnt[t] << 32;
+  list[off[i] + atomicAdd(&cursor[i], 1u)] = c | j;
+  list[off[j] + atomicAdd(&cursor[j], 1u)] = c | i;
+}
+
+// ---------------------------------------------------------------------------
+// Per-row top-K under (sv desc, id asc) (src/rabbitbin.h:418-429).
+// sv is a monotone function of the count, so the order is (level desc, id
+// asc) with level = dense rank of the distinct float similarities.  A row's
+// candidates are a dense matrix row (small regime, partners j != i passing
+// is_nz) or a candidate list (fused regime).  Selection: level histogram ->
+// threshold level L*; among candidates at L* the smallest ids are chosen by a
+// two-step radix select on the id (12 + 12 bits).  The kept entries are then
+// sorted by id.  Also reports the row's maximum count (small regime: over all
+// partners j != i, as calibrate_sim_cutoff; fused: over the gated partners).
+// ---------------------------------------------------------------------------
+struct RowSource {
+  // dense
+  const uint16_t* mat;
+  const uint8_t* anynz;
+  uint32_t n;
+  // list
+  const uint64_t* list;
+  const uint64_t* off;
+  bool dense;
+};
+
+__device__ __forceinline__ uint32_t row_len(const RowSource& s, uint32_t i) {
+  return s.dense ? s.n : (uint32_t)(s.off[i + 1] - s.off[i]);
+}
+// Returns false when position t of row i is not a candidate.
+__device__ __forceinline__ bool row_entry(const RowSource& s, uint32_t i, uint32_t t, uint32_t& id,
+                                          uint32_t& c, bool& for_max) {
+  if (s.dense) {
+    id = t;
+    c = s.mat[(size_t)i * s.n + t];
+    for_max = t != i;
+    return t != i && (!s.anynz || s.anynz[i] || s.anynz[t]);
+  }
+  const uint64_t e = s.list[s.off[i] + t];
+  id = (uint32_t)e;
+  c = (uint32_t)(e >> 32);
+  for_max = true;
+  return true;
+}
+
+// Block-parallel search over an LDS histogram: the first position q of the
+// scan order (ascending bins, or descending when rev) at which the inclusive
+// running count reaches target.  Thread t sums a contiguous chunk of the
+// scan order, a block scan gives each chunk's starting count, and the one
+// thread whose chunk crosses target walks it.  Results go to *bin / *before
+// (count strictly before that bin).  Requires target <= total; 256 threads.
+__device__ void find_reaching(const uint32_t* h, uint32_t nb, bool rev, uint32_t target,
+                              uint32_t* wave_tot, uint32_t* bin, uint32_t* before) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t chunk = (nb + 255u) / 256u;
+  const uint32_t q0 = min(nb, tid * chunk), q1 = min(nb, q0 + chunk);
+  uint32_t local = 0;
+  for (uint32_t q = q0; q < q1; ++q) local += h[rev ? nb - 1 - q : q];
+  uint32_t incl = local;
+  const uint32_t ln = tid & 63u;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(incl, d);
+    if (ln >= d) incl += t;
+  }
+  if (ln == 63) wave_tot[tid >> 6] = incl;
+  __syncthreads();
+  uint32_t excl = incl - local;
+  for (uint32_t w = 0; w < (tid >> 6); ++w) excl += wave_tot[w];
+  if (local != 0 && excl < target && target <= excl + local) {
+    uint32_t acc = excl;
+    for (uint32_t q = q0; q < q1; ++q) {
+      const uint32_t b = rev ? nb - 1 - q : q;
+      if (acc + h[b] >= target) {
+        *bin = b;
+        *before = acc;
+        break;
+      }
+      acc += h[b];
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void select_topk_kernel(
+    RowSource src, uint32_t n, const uint16_t* __restrict__ level, uint32_t nlevels, uint32_t K,
+    uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
+    uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out) {
+  __shared__ uint32_t hist[4096];
+  __shared__ uint32_t sel_id[1024];
+  __shared__ uint16_t sel_cnt[1024];
+  __shared__ uint32_t nsel, tot, tmp_bin, tmp_below, rmax;
+  __shared__ uint32_t wave_tot[4];
+  const uint32_t i = blockIdx.x;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t len = row_len(src, i);
+
+  for (uint32_t b = tid; b < 4096; b += 256) hist[b] = 0;
+  if (tid == 0) {
+    nsel = 0;
+    rmax = 0;
+  }
+  __syncthreads();
+  uint32_t my_max = 0;
+  for (uint32_t t = tid; t < len; t += 256) {
+    uint32_t id, c;
+    bool fm;
+    const bool ok = row_entry(src, i, t, id, c, fm);
+    if (fm && c > my_max) my_max = c;
+    if (ok) atomicAdd(&hist[level[c]], 1u);
+  }
+  if (my_max) atomicMax(&rmax, my_max);
+  __syncthreads();
+  // threshold level L*: the largest level whose suffix count reaches K
+  if (tid == 0) tot = 0;
+  __syncthreads();
+  {
+    uint32_t part = 0;
+    for (uint32_t b = tid; b < nlevels; b += 256) part += hist[b];
+    if (part) atomicAdd(&tot, part);
+  }
+  __syncthreads();
+  const bool take_all = tot <= K;
+  uint32_t Ls = 0xFFFFFFFFu, need_here = 0;
+  if (!take_all) {
+    find_reaching(hist, nlevels, true, K, wave_tot, &tmp_bin, &tmp_below);
+    Ls = tmp_bin;
+    need_here = K - tmp_below;
+  }
+  uint32_t idthr = 0xFFFFFFFFu;  // keep ids <= idthr at level L*
+  if (!take_all && need_here < hist[Ls]) {
+    // radix select on the id among level == L*: high 12 bits, then low 12
+    __syncthreads();
+    for (uint32_t b = tid; b < 4096; b += 256) hist[b] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < len; t += 256) {
+      uint32_t id, c;
+      bool fm;
+      if (row_entry(src, i, t, id, c, fm) && level[c] == Ls) atomicAdd(&hist[(id >> 12) & 4095u], 1u);
+    }
+    __syncthreads();
+    find_reaching(hist, 4096, false, need_here, wave_tot, &tmp_bin, &tmp_below);
+    const uint32_t hb = tmp_bin;
+    const uint32_t need2 = need_here - tmp_below;
+    __syncthreads();
+    for (uint32_t b = tid; b < 4096; b += 256) hist[b] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < len; t += 256) {
+      uint32_t id, c;
+      bool fm;
+      if (row_entry(src, i, t, id, c, fm) && level[c] == Ls && ((id >> 12) & 4095u) == hb)
+        atomicAdd(&hist[id & 4095u], 1u);
+    }
+    __syncthreads();
+    find_reaching(hist, 4096, false, need2, wave_tot, &tmp_bin, &tmp_below);
+    idthr = (hb << 12) | tmp_bin;
+  }
+  // collect
+  for (uint32_t t = tid; t < len; t += 256) {
+    uint32_t id, c;
+    bool fm;
+    if (!row_entry(src, i, t, id, c, fm)) continue;
+    const uint32_t lv = level[c];
+    bool keep;
+    if (take_all)
+      keep = true;
+    else
+      keep = lv > Ls || (lv == Ls && id <= idthr);
+    if (keep) {
+      const uint32_t s = atomicAdd(&nsel, 1u);
+      if (s < 1024) {
+        sel_id[s] = id;
+        sel_cnt[s] = (uint16_t)c;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t ns = min(nsel, K);
+  // bitonic sort by id over the next power of two >= ns
+  uint32_t P = 1;
+  while (P < ns) P <<= 1;
+  for (uint32_t t = ns + tid; t < P; t += 256) {
+    sel_id[t] = 0xFFFFFFFFu;
+    sel_cnt[t] = 0;
+  }
+  __syncthreads();
+  for (uint32_t k2 = 2; k2 <= P; k2 <<= 1) {
+    for (uint32_t jj = k2 >> 1; jj > 0; jj >>= 1) {
+      for (uint32_t t = tid; t < P; t += 256) {
+        const uint32_t u = t ^ jj;
+        if (u > t) {
+          const bool asc = (t & k2) == 0;
+          const uint32_t a = sel_id[t], bb = sel_id[u];
+          if ((a > bb) == asc) {
+            sel_id[t] = bb;
+            sel_id[u] = a;
+            const uint16_t ca = sel_cnt[t];
+            sel_cnt[t] = sel_cnt[u];
+            sel_cnt[u] = ca;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t t = tid; t < ns; t += 256) {
+    topk_id[(size_t)i * K + t] = sel_id[t];
+    topk_cnt[(size_t)i * K + t] = sel_cnt[t];
+  }
+  if (tid == 0) {
+    topk_len[i] = ns;
+    rowmax_out[i] = rmax;
+  }
+}
+
+// Edge emission.  Small regime: (i, j) for i < j, j in top(i), count >= cmin.
+// Fused regime additionally requires i in top(j) (mutual, default on,
+// src/impl/rb_graph.cpp:1283-1348).  Pass 0 counts per row, pass 1 writes at
+// the row's offset, so the edges come out sorted by (i, j).
+__global__ __launch_bounds__(256) void emit_kernel(const uint32_t* __restrict__ topk_id,
+                                                   const uint16_t* __restrict__ topk_cnt,
+                                                   const uint32_t* __restrict__ topk_len, uint32_t n,
+                                                   uint32_t K, uint32_t cmin, int mutual, int pass,
+                                                   uint32_t* __restrict__ row_edges,
+                                                   const uint64_t* __restrict__ row_off,
+                                                   uint32_t* __restrict__ efrom,
+                                                   uint32_t* __restrict__ eto,
+                                                   uint16_t* __restrict__ ecnt) {
+  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t ln = threadIdx.x & 63u;
+  if (i >= n) return;
+  const uint32_t len = topk_len[i];
+  uint32_t written = 0;
+  for (uint32_t t0 = 0; t0 < len; t0 += 64) {
+    const uint32_t t = t0 + ln;
+    bool keep = false;
+    uint32_t j = 0, c = 0;
+    if (t < len) {
+      j = topk_id[(size_t)i * K + t];
+      c = topk_cnt[(size_t)i * K + t];
+      keep = j > i && c >= cmin;
+      if (keep && mutual) {
+        const uint32_t* lst = topk_id + (size_t)j * K;
+        uint32_t lo = 0, hi = topk_len[j];
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (lst[mid] < i)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        keep = lo < topk_len[j] && lst[lo] == i;
+      }
+    }
+    const uint64_t mask = __ballot(keep);
+    if (pass == 1 && keep) {
+      const uint64_t pos = row_off[i] + written + lanes_below(mask);
+      efrom[pos] = i;
+      eto[pos] = j;
+      ecnt[pos] = (uint16_t)c;
+    }
+    written += (uint32_t)__popcll(mask);
+  }
+  if (pass == 0 && ln == 0) row_edges[i] = written;
+}
+
+// Stage (d): per-edge weight in FP64 (the reference's depth_edge_term with
+// the default fuse = min, src/rabbitbin.cpp:930-970, and the combination and
+// cutoff of src/rabbitbin.cpp:5500-5507).
+__global__ __launch_bounds__(256) void edge_weight_kernel(
+    const uint32_t* __restrict__ efrom, const uint32_t* __restrict__ eto,
+    const float* __restrict__ scomp, uint64_t ne, const float* __restrict__ unit,
+    const float* __restrict__ raw, const double* __restrict__ colnorm, uint32_t S,
+    double w_comp, double min_edge, double neg_thr, float* __restrict__ score,
+    double* __restrict__ score64) {
+  const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  const uint32_t i = efrom[e], j = eto[e];
+  double w;
+  if (S == 0) {
+    w = (double)scomp[e] < min_edge ? 0.0 : (double)scomp[e];
+  } else if (S == 1) {
+    w = scomp[e];
+  } else {
+    const float* a = unit + (size_t)i * S;
+    const float* b = unit + (size_t)j * S;
+    double corr = 0.0;
+    for (uint32_t k = 0; k < S; ++k) corr += (double)a[k] * (double)b[k];
+    const bool finite = corr == corr && corr != __builtin_huge_val() && corr != -__builtin_huge_val();
+    if (!finite || (neg_thr > -0.99 && corr < neg_thr)) {
+      w = 0.0;
+    } else {
+      const double cp = corr < 0.0 ? 0.0 : corr;
+      const float* ra = raw + (size_t)i * S;
+      const float* rb = raw + (size_t)j * S;
+      double mn = 0.0, mx = 0.0;
+      for (uint32_t k = 0; k < S; ++k) {
+        double x = ra[k], y = rb[k];
+        if (colnorm) {
+          x *= colnorm[k];
+          y *= colnorm[k];
+        }
+        if (x < y) {
+          mn += x;
+          mx += y;
+        } else {
+          mn += y;
+          mx += x;
+        }
+      }
+      double wj = mx <= 0.0 ? 0.0 : mn / mx;
+      if (!(wj == wj)) wj = 0.0;
+      wj = wj < 0.0 ? 0.0 : (wj > 1.0 ? 1.0 : wj);
+      const double d = cp < wj ? cp : wj;
+      w = w_comp * (double)scomp[e] + (1.0 - w_comp) * d;
+      if (!(w == w) || w < min_edge) w = 0.0;
+    }
+  }
+  score[e] = (float)w;
+  if (score64) score64[e] = w;
+}
+
+__global__ void u16_to_sv_kernel(const uint16_t* __restrict__ c, uint64_t ne,
+                                 const float* __restrict__ svf, float* __restrict__ out) {
+  const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < ne) out[e] = svf[c[e]];
+}
+
+
+// Exclusive prefix sum u32 -> u64 in three steps: per-block sums (1024
+// elements per block), a one-block scan of the block sums, then a per-block
+// scan with the block's offset.  out has n + 1 entries (out[n] = total).
+__device__ uint64_t block_scan_excl_u64(uint64_t v, uint64_t* wave_tot, uint64_t* total) {
+  const uint32_t tid = threadIdx.x, ln = tid & 63u;
+  uint64_t incl = v;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint64_t t = __shfl_up(incl, d);
+    if (ln >= d) incl += t;
+  }
+  if (ln == 63) wave_tot[tid >> 6] = incl;
+  __syncthreads();
+  uint64_t excl = incl - v;
+  for (uint32_t w = 0; w < (tid >> 6); ++w) excl += wave_tot[w];
+  if (tid == 255) *total = excl + v;
+  __syncthreads();
+  return excl;
+}
+
+__global__ __launch_bounds__(256) void scan_block_sums_kernel(const uint32_t* __restrict__ in,
+                                                              uint32_t n,
+                                                              uint64_t* __restrict__ sums) {
+  __shared__ uint64_t wt[4], tot;
+  const uint64_t base = (uint64_t)blockIdx.x * 1024;
+  uint64_t v = 0;
+  for (uint32_t q = 0; q < 4; ++q) {
+    const uint64_t idx = base + threadIdx.x * 4 + q;
+    if (idx < n) v += in[idx];
+  }
+  block_scan_excl_u64(v, wt, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void scan_sums_kernel(uint64_t* __restrict__ sums, uint32_t nb) {
+  __shared__ uint64_t wt[4], tot;
+  uint64_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
+    const uint32_t b = b0 + threadIdx.x;
+    const uint64_t v = b < nb ? sums[b] : 0;
+    const uint64_t ex = block_scan_excl_u64(v, wt, &tot);
+    if (b < nb) sums[b] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void scan_final_kernel(const uint32_t* __restrict__ in, uint32_t n,
+                                                         const uint64_t* __restrict__ sums,
+                                                         uint64_t* __restrict__ out) {
+  __shared__ uint64_t wt[4], tot;
+  const uint64_t base = (uint64_t)blockIdx.x * 1024;
+  uint32_t x[4];
+  uint64_t v = 0;
+  for (uint32_t q = 0; q < 4; ++q) {
+    const uint64_t idx = base + threadIdx.x * 4 + q;
+    x[q] = idx < n ? in[idx] : 0u;
+    v += x[q];
+  }
+  uint64_t run = sums[blockIdx.x] + block_scan_excl_u64(v, wt, &tot);
+  for (uint32_t q = 0; q < 4; ++q) {
+    const uint64_t idx = base + threadIdx.x * 4 + q;
+    if (idx <= n) out[idx] = run;  // idx == n writes the grand total
+    run += x[q];
+  }
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
+                         uint32_t W, const double* T, const uint8_t* cls, const uint8_t* codes,
+                         uint8_t* wx, uint32_t mpad, uint32_t* planes, hipStream_t st) {
+  constexpr int CPB = 4;
+  const uint32_t blocks = (n + CPB - 1) / CPB;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(sketch_kernel<CPB>, dim3(blocks), dim3(256), 0, st, seqs, offs, n, m, W, T,
+                     cls, codes, wx, mpad, planes);
+  return hipGetLastError();
+}
+
+hipError_t launch_list_counts(const uint32_t* planes, uint32_t W, const uint32_t* pi,
+                              const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st) {
+  if (np == 0) return hipSuccess;
+  hipLaunchKernelGGL(list_counts_kernel, dim3((np + 255) / 256), dim3(256), 0, st, planes, W, pi,
+                     pj, np, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_win16_counts(const uint16_t* win16, uint32_t m, const uint32_t* pi,
+                               const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st) {
+  if (np == 0) return hipSuccess;
+  hipLaunchKernelGGL(win16_counts_kernel, dim3((np + 255) / 256), dim3(256), 0, st, win16, m, pi, pj,
+                     np, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, uint32_t n, uint16_t* out,
+                               hipStream_t st) {
+  const uint32_t rows = 64;
+  dim3 grid((n + 255) / 256, (n + rows - 1) / rows);
+  switch (W) {
+    case 16: hipLaunchKernelGGL(dense_counts_kernel<16>, grid, dim3(256), 0, st, planes, n, rows, out); break;
+    case 8: hipLaunchKernelGGL(dense_counts_kernel<8>, grid, dim3(256), 0, st, planes, n, rows, out); break;
+    case 4: hipLaunchKernelGGL(dense_counts_kernel<4>, grid, dim3(256), 0, st, planes, n, rows, out); break;
+    case 2: hipLaunchKernelGGL(dense_counts_kernel<2>, grid, dim3(256), 0, st, planes, n, rows, out); break;
+    case 1: hipLaunchKernelGGL(dense_counts_kernel<1>, grid, dim3(256), 0, st, planes, n, rows, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gate(const float* unit, uint32_t NC, const uint8_t* anynz, uint32_t n, float thr,
+                       uint32_t acc1_end, uint64_t* surv, unsigned long long* nsurv, uint64_t cap,
+                       hipStream_t st) {
+  const uint32_t rows = 256;
+  dim3 grid((n + 255) / 256, (n + rows - 1) / rows);
+  switch (NC) {
+    case 1: hipLaunchKernelGGL(gate_kernel<1>, grid, dim3(256), 0, st, unit, anynz, n, thr, acc1_end, rows, surv, nsurv, cap); break;
+    case 2: hipLaunchKernelGGL(gate_kernel<2>, grid, dim3(256), 0, st, unit, anynz, n, thr, acc1_end, rows, surv, nsurv, cap); break;
+    case 3: hipLaunchKernelGGL(gate_kernel<3>, grid, dim3(256), 0, st, unit, anynz, n, thr, acc1_end, rows, surv, nsurv, cap); break;
+    case 4: hipLaunchKernelGGL(gate_kernel<4>, grid, dim3(256), 0, st, unit, anynz, n, thr, acc1_end, rows, surv, nsurv, cap); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_survivor_counts(const uint32_t* planes, uint32_t W, const uint64_t* surv,
+                                  uint64_t ns, uint16_t* cnt, uint32_t* deg, uint32_t* rowmax,
+                                  hipStream_t st) {
+  if (ns == 0) return hipSuccess;
+  hipLaunchKernelGGL(survivor_counts_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
+                     planes, W, surv, ns, cnt, deg, rowmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(const uint64_t* surv, const uint16_t* cnt, uint64_t ns,
+                          const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st) {
+  if (ns == 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, surv, cnt,
+                     ns, off, cursor, list);
+  return hipGetLastError();
+}
+
+hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32_t n,
+                               const uint16_t* level, uint32_t nlevels, uint32_t K,
+                               uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
+                               uint32_t* rowmax, hipStream_t st) {
+  RowSource s{};
+  s.mat = mat;
+  s.anynz = anynz;
+  s.n = n;
+  s.dense = true;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(select_topk_kernel, dim3(n), dim3(256), 0, st, s, n, level, nlevels, K, topk_id,
+                     topk_cnt, topk_len, rowmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_t n,
+                              const uint16_t* level, uint32_t nlevels, uint32_t K,
+                              uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
+                              uint32_t* rowmax, hipStream_t st) {
+  RowSource s{};
+  s.list = list;
+  s.off = off;
+  s.n = n;
+  s.dense = false;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(select_topk_kernel, dim3(n), dim3(256), 0, st, s, n, level, nlevels, K, topk_id,
+                     topk_cnt, topk_len, rowmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_emit(const uint32_t* topk_id, const uint16_t* topk_cnt, const uint32_t* topk_len,
+                       uint32_t n, uint32_t K, uint32_t cmin, int mutual, int pass,
+                       uint32_t* row_edges, const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
+                       uint16_t* ecnt, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(emit_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt, topk_len, n,
+                     K, cmin, mutual, pass, row_edges, row_off, efrom, eto, ecnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const float* scomp,
+                               uint64_t ne, const float* unit, const float* raw,
+                               const double* colnorm, uint32_t S, double w_comp, double min_edge,
+                               double neg_thr, float* score, double* score64, hipStream_t st) {
+  if (ne == 0) return hipSuccess;
+  hipLaunchKernelGGL(edge_weight_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, efrom,
+                     eto, scomp, ne, unit, raw, colnorm, S, w_comp, min_edge, neg_thr, score, score64);
+  return hipGetLastError();
+}
+
+hipError_t launch_cnt_to_sv(const uint16_t* c, uint64_t ne, const float* svf, float* out,
+                            hipStream_t st) {
+  if (ne == 0) return hipSuccess;
+  hipLaunchKernelGGL(u16_to_sv_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, c, ne,
+                     svf, out);
+  return hipGetLastError();
+}
+
+hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n, void* tmp,
+                                     size_t& tmp_bytes, hipStream_t st) {
+  // n + 1 outputs; blocks cover 1024 inputs each (the last block also writes out[n]).
+  const uint32_t nb = n / 1024 + 1;
+  if (tmp == nullptr) {
+    tmp_bytes = (size_t)nb * sizeof(uint64_t);
+    return hipSuccess;
+  }
+  uint64_t* sums = static_cast<uint64_t*>(tmp);
+  hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nb), dim3(256), 0, st, in, n, sums);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, st, sums, nb);
+  hipLaunchKernelGGL(scan_final_kernel, dim3(nb), dim3(256), 0, st, in, n, sums, out);
+  return hipGetLastError();
+}
+
+}  // namespace rbg

@@ -1,0 +1,59 @@
+"""GPU parity: librbg (through the C ABI) against the CPU oracle on the same
+seeded inputs.  Integer/index outputs must be bit-exact; edge weights within
+1e-6 of the FP64 oracle."""
+import numpy as np
+import pytest
+
+from rabbitbin_amd import api, synth
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _edges(g, key="scomp"):
+    return {(int(a), int(b)): float(s) for a, b, s in zip(g["from"], g["to"], g[key])}
+
+
+def _check_graph(g, ref):
+    assert np.array_equal(g["win16"], ref["win16"])
+    assert g["b0"] == ref["b0"]
+    assert np.array_equal(g["rowmax_cnt"], ref["rowmax_cnt"])
+    assert np.array_equal(g["topk_len"], ref["topk_len"])
+    n, K = g["topk_id"].shape
+    for i in range(n):
+        L = g["topk_len"][i]
+        assert set(g["topk_id"][i, :L].tolist()) == set(ref["topk_id"][i, :L].tolist()), i
+    assert g["sim_cutoff"] == ref["sim_cutoff"]
+    eg = _edges(g)
+    er = {(int(a), int(b)): float(s) for a, b, s in zip(ref["from"], ref["to"], ref["sv"])}
+    assert eg == er
+    if g["score"] is not None:
+        d = np.abs(g["score"].astype(np.float64) - ref["score64"])
+        assert d.max() <= 1e-6
+
+
+def test_sketch_winners_exact(ctx):
+    seqs, off, _, _ = synth.make_assembly(300, 6, np.log(8000), 0.8, 0, seed=7)
+    w32, w16 = ctx.build_pmh_winners(seqs, off, m=500, seed=42)
+    ref32 = orc.sketch_batch(seqs, off, m=500, seed=42)
+    assert np.array_equal(w32, ref32)
+    assert np.array_equal(w16, orc.fold16(ref32))
+
+
+def test_graph_small_regime_2k(ctx):
+    seqs, off, depth, _ = synth.workload("asm2k")
+    p = api.default_params(rand_seed=1)
+    g = ctx.build_graph(seqs, off, depth, p, diag=True)
+    ref = orc.full_pipeline(seqs, off, depth, rand_seed=1)
+    assert g["regime"] == 1
+    _check_graph(g, ref)
+
+
+def test_graph_fused_regime_forced_3k(ctx):
+    seqs, off, depth, _ = synth.make_assembly(3000, 30, np.log(8000), 0.8, 10, seed=99)
+    p = api.default_params(rand_seed=3, regime=2)
+    g = ctx.build_graph(seqs, off, depth, p, diag=True)
+    ref = orc.full_pipeline(seqs, off, depth, rand_seed=3, force_regime=2)
+    assert g["regime"] == 2
+    assert g["gated_pairs"] == ref["gated_pairs"]
+    _check_graph(g, ref)
