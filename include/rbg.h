@@ -83,7 +83,14 @@ typedef struct rbg_graph_params {
   uint64_t pmh_seed;       /* g_pmh_seed (42) */
   uint32_t rand_seed;      /* srand() seed (--seed); calibration consumes rand() */
   int regime;              /* 0 auto (n > 25000 -> fused), 1 small, 2 fused */
+  uint32_t flags;          /* RBG_EDGES_ON_DEVICE: leave the edge list in HBM */
 } rbg_graph_params;
+
+/* With this flag rbg_asm_build_graph returns the counts, cutoffs and timings
+ * only (the result's array pointers are NULL); the edge list stays in the
+ * assembly's device buffers until the next build and is copied out with
+ * rbg_asm_copy_edges. */
+#define RBG_EDGES_ON_DEVICE 1u
 
 /* Fills `p` with the reference defaults listed above (rand_seed = 1). */
 void rbg_default_params(rbg_graph_params* p);
@@ -100,6 +107,10 @@ typedef struct rbg_graph_result {
   uint64_t gated_pairs;    /* pairs that reached the register compare */
   /* stage timings in milliseconds, measured with HIP events on the stream */
   double ms_sketch, ms_pairs, ms_select, ms_edges, ms_total;
+  /* single-kernel durations: the pair kernel (abundance gate in the fused
+   * regime, all-pairs register compare in the small regime), the register
+   * compare of the gated pairs, and the top-N selection */
+  double ms_k_pair, ms_k_compare, ms_k_select;
 } rbg_graph_result;
 
 /* Optional per-row diagnostics for parity tests (any pointer may be NULL):
@@ -132,6 +143,10 @@ rbg_status rbg_asm_create(rbg_ctx* ctx, const char* seqs, const uint64_t* offset
 void rbg_asm_free(rbg_asm* a);
 rbg_status rbg_asm_build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* params,
                                rbg_graph_result* out, rbg_graph_diag* diag);
+/* Copies the last build's n_edges edges into caller buffers (any may be NULL;
+ * score is only written when the assembly has depth samples). */
+rbg_status rbg_asm_copy_edges(rbg_ctx* ctx, rbg_asm* a, uint32_t* from, uint32_t* to,
+                              float* scomp, float* score);
 
 /* ---------------------------------------------------------------------------
  * Stage (d) alone (src/rabbitbin.cpp:5482-5509): edge weights for a given

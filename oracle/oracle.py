@@ -191,6 +191,23 @@ def edge_weights(frm, to, scomp, unit, raw, colnorm, S, mode, min_edge_weight=0.
     return w, w64
 
 
+def pair_pass_rows(win16, unit, anynz, S, b0, row0, row1, K=200, threads=0):
+    """Timed fused pair pass of rows [row0, row1) against all partners."""
+    L = orc()
+    L.orc_pair_pass_rows.restype = C.c_double
+    L.orc_pair_pass_rows.argtypes = [u16p, f32p, u8p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                                     C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, u64p, u64p]
+    w = np.ascontiguousarray(win16, np.uint16)
+    n, m = w.shape
+    u = np.ascontiguousarray(unit, np.float32) if unit is not None else None
+    nz = np.ascontiguousarray(anynz, np.uint8) if anynz is not None else None
+    pairs = C.c_uint64()
+    gated = C.c_uint64()
+    secs = L.orc_pair_pass_rows(_p(w, u16p), _p(u, f32p), _p(nz, u8p), n, m, S, b0, K, row0, row1, threads,
+                                C.byref(pairs), C.byref(gated))
+    return {"seconds": secs, "pairs": pairs.value, "gated": gated.value}
+
+
 def full_pipeline(seqs, offsets, depth, rand_seed=1, m=500, K=200, force_regime=0, threads=0):
     """Oracle of the whole path: sketches, b0, graph, edge weights (reference arithmetic)."""
     w32 = sketch_batch(seqs, offsets, m=m, threads=threads)

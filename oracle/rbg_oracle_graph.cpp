@@ -530,3 +530,68 @@ double orc_calibrate(const uint32_t* rowmax, uint64_t n, uint32_t m, double b0, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Timed CPU restatement of the fused pair pass for a row sample (the CPU
+// baseline of bench.py): rows [row0, row1) against every partner, the gate
+// and 16-bit register compare in AVX-512 (src/rabbitbin.cpp:754-776,
+// 1261-1293) and a top-K heap per row.  OpenMP over rows.
+// ---------------------------------------------------------------------------
+__attribute__((target("avx512f,avx512bw"))) static uint32_t match16_avx512(const uint16_t* a,
+                                                                           const uint16_t* b,
+                                                                           uint32_t m) {
+  uint32_t cnt = 0, k = 0;
+  const __m512i vz = _mm512_setzero_si512();
+  for (; k + 32 <= m; k += 32) {
+    const __m512i va = _mm512_loadu_si512((const void*)(a + k));
+    const __m512i vb = _mm512_loadu_si512((const void*)(b + k));
+    cnt += (uint32_t)__builtin_popcount(
+        (unsigned)(_mm512_cmpeq_epi16_mask(va, vb) & _mm512_cmpneq_epi16_mask(va, vz)));
+  }
+  for (; k < m; ++k) cnt += (a[k] != 0 && a[k] == b[k]) ? 1u : 0u;
+  return cnt;
+}
+
+extern "C" double orc_pair_pass_rows(const uint16_t* win16, const float* unit, const uint8_t* anynz,
+                                     uint64_t n, uint32_t m, uint32_t S, double b0, uint32_t K,
+                                     uint64_t row0, uint64_t row1, int threads, uint64_t* pairs,
+                                     uint64_t* gated_out) {
+  if (threads <= 0) threads = omp_get_max_threads();
+  const bool avx = __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw");
+  std::vector<float> svf(m + 1);
+  for (uint32_t c = 0; c <= m; ++c) svf[c] = (float)orc_sv_from_count(c, m, b0);
+  const bool gate = unit && S > 2;  // w_comp = 0 for S >= 3
+  const float thr = (float)(0.7 - 1e-4);
+  uint64_t gated = 0;
+  const double t0 = omp_get_wtime();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : gated)
+  for (uint64_t i = row0; i < row1; ++i) {
+    std::priority_queue<Cand, std::vector<Cand>, Weaker> h;
+    const uint16_t* wi = win16 + i * m;
+    for (uint64_t j = 0; j < n; ++j) {
+      if (j == i || !is_nz(anynz, i, j)) continue;
+      if (gate) {
+        float c;
+        if (avx)
+          c = unit_dot_avx512(unit + i * S, unit + j * S, S);
+        else
+          c = orc_unit_dot_f(unit + i * S, unit + j * S, S);
+        if (c < thr) continue;
+      }
+      ++gated;
+      const uint32_t cnt = avx ? match16_avx512(wi, win16 + j * m, m)
+                               : orc_match_count16(wi, win16 + j * m, m);
+      const Cand cd{(uint32_t)j, svf[cnt]};
+      if (h.size() < K) {
+        h.push(cd);
+      } else if (Weaker()(cd, h.top())) {
+        h.pop();
+        h.push(cd);
+      }
+    }
+  }
+  const double secs = omp_get_wtime() - t0;
+  *pairs = (row1 - row0) * (n - 1);
+  *gated_out = gated;
+  return secs;
+}

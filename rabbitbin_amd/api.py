@@ -37,7 +37,11 @@ class GraphParams(C.Structure):
         ("pmh_seed", C.c_uint64),
         ("rand_seed", C.c_uint32),
         ("regime", C.c_int),
+        ("flags", C.c_uint32),
     ]
+
+
+EDGES_ON_DEVICE = 1
 
 
 class GraphResult(C.Structure):
@@ -56,6 +60,9 @@ class GraphResult(C.Structure):
         ("ms_select", C.c_double),
         ("ms_edges", C.c_double),
         ("ms_total", C.c_double),
+        ("ms_k_pair", C.c_double),
+        ("ms_k_compare", C.c_double),
+        ("ms_k_select", C.c_double),
     ]
 
 
@@ -93,6 +100,9 @@ def lib():
                                   C.POINTER(GraphParams), C.POINTER(GraphResult), C.POINTER(GraphDiag)]
     L.rbg_asm_create.argtypes = [vp, C.c_char_p, u64p, C.c_uint32, f32p, C.c_uint32, C.c_double, C.POINTER(vp)]
     L.rbg_asm_free.argtypes = [vp]
+    L.rbg_asm_copy_edges.argtypes = [vp, vp, u32p, u32p, f32p, f32p]
+    L.rbg_sketch_tables.argtypes = [C.c_uint32, C.c_uint64, f64p, u64p, u8p]
+    L.rbg_calibrate.argtypes = [u32p, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_uint32, C.c_int, f64p]
     L.rbg_asm_build_graph.argtypes = [vp, vp, C.POINTER(GraphParams), C.POINTER(GraphResult), C.POINTER(GraphDiag)]
     L.rbg_graph_result_free.argtypes = [C.POINTER(GraphResult)]
     L.rbg_edge_weights.argtypes = [vp, u32p, u32p, f32p, C.c_uint64, f32p, f32p, f64p, C.c_uint32,
@@ -166,15 +176,22 @@ class Context:
 
     def _result(self, res, n, K, diag_arrays):
         ne = res.n_edges
+        host = bool(res.from_)
+
+        def arr(ptr, dt):
+            return np.ctypeslib.as_array(ptr, (ne,)).copy() if (ne and ptr) else np.zeros(0, dt)
+
         out = {
-            "from": np.ctypeslib.as_array(res.from_, (ne,)).copy() if ne else np.zeros(0, np.uint32),
-            "to": np.ctypeslib.as_array(res.to, (ne,)).copy() if ne else np.zeros(0, np.uint32),
-            "scomp": np.ctypeslib.as_array(res.scomp, (ne,)).copy() if ne else np.zeros(0, np.float32),
+            "n_edges": ne,
+            "from": arr(res.from_, np.uint32) if host else None,
+            "to": arr(res.to, np.uint32) if host else None,
+            "scomp": arr(res.scomp, np.float32) if host else None,
             "score": (np.ctypeslib.as_array(res.score, (ne,)).copy() if (ne and res.score) else None),
             "sim_cutoff": res.sim_cutoff, "b0": res.b0, "regime": res.regime,
             "gated_pairs": res.gated_pairs,
             "ms": {"sketch": res.ms_sketch, "pairs": res.ms_pairs, "select": res.ms_select,
-                   "edges": res.ms_edges, "total": res.ms_total},
+                   "edges": res.ms_edges, "total": res.ms_total, "k_pair": res.ms_k_pair,
+                   "k_compare": res.ms_k_compare, "k_select": res.ms_k_select},
         }
         if diag_arrays is not None:
             out.update(diag_arrays)
@@ -237,13 +254,26 @@ class Assembly:
         ctx._check(ctx._L.rbg_asm_create(ctx._h, buf, _p(off, u64p), self.n, _p(dep, f32p), self.S, min_cv,
                                          C.byref(self._h)), "rbg_asm_create")
 
-    def build_graph(self, params=None, diag=False):
+    def build_graph(self, params=None, diag=False, on_device=False):
+        """on_device=True leaves the edge list in HBM (fetch with copy_edges)."""
         p = params or default_params()
+        if on_device:
+            p = GraphParams.from_buffer_copy(p)
+            p.flags |= EDGES_ON_DEVICE
         d, arrs = self.ctx._diag(self.n, p.max_edges, p.m, diag)
         res = GraphResult()
         self.ctx._check(self.ctx._L.rbg_asm_build_graph(self.ctx._h, self._h, C.byref(p), C.byref(res),
                                                         C.byref(d) if d else None), "rbg_asm_build_graph")
         return self.ctx._result(res, self.n, p.max_edges, arrs)
+
+    def copy_edges(self, ne):
+        frm = np.zeros(ne, np.uint32)
+        to = np.zeros(ne, np.uint32)
+        sc = np.zeros(ne, np.float32)
+        score = np.zeros(ne, np.float32) if self.S else None
+        self.ctx._check(self.ctx._L.rbg_asm_copy_edges(self.ctx._h, self._h, _p(frm, u32p), _p(to, u32p),
+                                                       _p(sc, f32p), _p(score, f32p)), "rbg_asm_copy_edges")
+        return frm, to, sc, score
 
     def free(self):
         if self._h:

@@ -56,7 +56,7 @@ struct rbg_ctx {
     DevBuf T, cls, codes;
   };
   std::vector<std::unique_ptr<Tables>> tables;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[12] = {};
   rbg_status fail(rbg_status st, const std::string& msg) {
     err = msg;
     return st;
@@ -126,14 +126,17 @@ struct rbg_asm {
   uint64_t seq_bytes = 0;
   DevBuf seqs, offs;
   // depth features (device) and their host copies
-  DevBuf unit_pad, unit, raw, colnorm, anynz;
-  uint32_t NC = 0;
+  DevBuf unit_pad, unit, raw, colnorm, anynz, q8;
+  uint32_t NC = 1;
   bool has_anynz = false;
   std::vector<uint8_t> anynz_h;
+  GateQuant gq;  // bound statistics of q8 (the int8 rows live on the device)
   // pipeline intermediates
-  DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, surv, nsurv, scnt, deg, rowmax, off, cursor,
+  DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, cand, surv, nsurv, scnt, deg, rowmax, off, cursor,
       list, topk_id, topk_cnt, topk_len, row_edges, row_off, efrom, eto, ecnt, escomp, escore,
-      scan_tmp;
+      scan_tmp, tiles;
+  uint32_t ntiles = 0;
+  uint64_t last_edges = 0;  // edge count of the last build (device buffers)
 };
 
 namespace {
@@ -178,7 +181,31 @@ rbg_status asm_upload(rbg_ctx* ctx, rbg_asm* a, const char* seqs, const uint64_t
                                     hipMemcpyHostToDevice, ctx->stream));
     RBG_TRY_HIP(ctx, hipMemcpyAsync(a->anynz.p, a->anynz_h.data(), n, hipMemcpyHostToDevice,
                                     ctx->stream));
+    RBG_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (S > 1 && S <= 64) {
+      gate_quantize(unit.data(), n, S, a->gq);
+      RBG_TRY_HIP(ctx, a->q8.ensure(a->gq.q.size()));
+      RBG_TRY_HIP(ctx, hipMemcpy(a->q8.p, a->gq.q.data(), a->gq.q.size(), hipMemcpyHostToDevice));
+      std::vector<int8_t>().swap(a->gq.q);
+    }
   }
+  if (!(S > 1 && S <= 64)) {
+    // no abundance rows: an all-zero image (the gate is then off)
+    a->NC = 1;
+    a->gq = GateQuant{};
+    a->gq.kpad = 32;
+    RBG_TRY_HIP(ctx, a->q8.ensure((size_t)n * 32));
+    RBG_TRY_HIP(ctx, hipMemsetAsync(a->q8.p, 0, (size_t)n * 32, ctx->stream));
+    RBG_TRY_HIP(ctx, a->unit_pad.ensure((size_t)n * 16 * sizeof(float)));
+    RBG_TRY_HIP(ctx, hipMemsetAsync(a->unit_pad.p, 0, (size_t)n * 16 * sizeof(float), ctx->stream));
+  }
+  // work items of the fused pair pass (256-row stripes x up to 16 tiles)
+  std::vector<uint32_t> tl;
+  gate_work_items(n, tl);
+  a->ntiles = (uint32_t)(tl.size() / 2);
+  RBG_TRY_HIP(ctx, a->tiles.ensure(tl.size() * sizeof(uint32_t)));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(a->tiles.p, tl.data(), tl.size() * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, ctx->stream));
   RBG_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RBG_OK;
 }
@@ -262,67 +289,80 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   if (regime == 1) {
     RBG_TRY_HIP(ctx, a->mat.ensure((size_t)n * n * sizeof(uint16_t)));
     RBG_TRY_HIP(ctx, launch_dense_counts((const uint32_t*)a->planes.p, W, n, (uint16_t*)a->mat.p, st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[6], st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[7], st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[8], st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[3], st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[9], st));
     RBG_TRY_HIP(ctx, launch_select_dense((const uint16_t*)a->mat.p, anynz, n,
                                          (const uint16_t*)a->level.p, nlevels, K,
                                          (uint32_t*)a->topk_id.p, (uint16_t*)a->topk_cnt.p,
                                          (uint32_t*)a->topk_len.p, (uint32_t*)a->rowmax.p, st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[10], st));
   } else {
     // abundance-first gate (src/impl/rb_graph.cpp:693-706)
     const bool gate = S > 1 && w_comp < 1.0 && w_comp < P->min_edge_weight;
     if (gate && S > 64) return ctx->fail(RBG_ERR_UNSUPPORTED, "abundance gate needs S <= 64");
     const float thr = gate ? (float)((P->min_edge_weight - w_comp) / (1.0 - w_comp) - 1e-4)
                            : -__builtin_huge_valf();
-    const uint32_t NC = gate ? a->NC : 1;
+    const uint32_t NC = a->NC;
     const uint32_t acc1_end = gate ? 2 * (S / 32) : 0;
-    const float* unit = gate ? (const float*)a->unit_pad.p : nullptr;
-    DevBuf zero_unit;
-    if (!gate) {
-      RBG_TRY_HIP(ctx, zero_unit.ensure((size_t)n * 16 * sizeof(float)));
-      RBG_TRY_HIP(ctx, hipMemsetAsync(zero_unit.p, 0, (size_t)n * 16 * sizeof(float), st));
-      unit = (const float*)zero_unit.p;
-    }
+    // prefilter cut: with the gate off every pair passes (cut = INT32_MIN, thr = -inf)
+    const int32_t qcut = gate ? gate_qcut(a->gq, thr) : INT32_MIN;
     const uint64_t total_pairs = (uint64_t)n * (n - 1) / 2;
     uint64_t cap = std::min<uint64_t>(total_pairs, std::max<uint64_t>(1ull << 26, 2048ull * n));
-    RBG_TRY_HIP(ctx, a->nsurv.ensure(sizeof(unsigned long long)));
-    uint64_t ns = 0;
+    RBG_TRY_HIP(ctx, a->nsurv.ensure(2 * sizeof(unsigned long long)));
+    unsigned long long* counters = (unsigned long long*)a->nsurv.p;  // [0] candidates, [1] survivors
+    uint64_t nc = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
-      RBG_TRY_HIP(ctx, a->surv.ensure(std::max<uint64_t>(cap, 1) * sizeof(uint64_t)));
-      RBG_TRY_HIP(ctx, hipMemsetAsync(a->nsurv.p, 0, sizeof(unsigned long long), st));
-      RBG_TRY_HIP(ctx, launch_gate(unit, NC, anynz, n, thr, acc1_end, (uint64_t*)a->surv.p,
-                                   (unsigned long long*)a->nsurv.p, cap, st));
+      RBG_TRY_HIP(ctx, a->cand.ensure(std::max<uint64_t>(cap, 1) * sizeof(uint64_t)));
+      RBG_TRY_HIP(ctx, hipMemsetAsync(counters, 0, 2 * sizeof(unsigned long long), st));
+      RBG_TRY_HIP(ctx, hipEventRecord(ev[2], st));
+      RBG_TRY_HIP(ctx, launch_gate_mfma((const int8_t*)a->q8.p, a->gq.kpad, n, qcut,
+                                        (const uint32_t*)a->tiles.p, a->ntiles,
+                                        (uint64_t*)a->cand.p, counters, cap, st));
+      RBG_TRY_HIP(ctx, hipEventRecord(ev[6], st));
       unsigned long long got = 0;
-      RBG_TRY_HIP(ctx, hipMemcpyAsync(&got, a->nsurv.p, sizeof(got), hipMemcpyDeviceToHost, st));
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(&got, counters, sizeof(got), hipMemcpyDeviceToHost, st));
       RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
-      ns = got;
-      if (ns <= cap) break;
-      cap = ns;  // rerun with room for every survivor
+      nc = got;
+      if (nc <= cap) break;
+      cap = nc;  // rerun with room for every candidate
     }
+    // exact FP32 gate of the candidates (survivors <= candidates: no overflow)
+    RBG_TRY_HIP(ctx, a->surv.ensure(std::max<uint64_t>(nc, 1) * sizeof(uint64_t)));
+    RBG_TRY_HIP(ctx, launch_exact_gate((const uint64_t*)a->cand.p, nc, (const float*)a->unit_pad.p,
+                                       NC, anynz, thr, acc1_end, (uint64_t*)a->surv.p,
+                                       counters + 1, nc, st));
+    unsigned long long got_s = 0;
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(&got_s, counters + 1, sizeof(got_s), hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+    const uint64_t ns = got_s;
     out->gated_pairs = ns;
-    RBG_TRY_HIP(ctx, a->scnt.ensure(std::max<uint64_t>(ns, 1) * sizeof(uint16_t)));
     RBG_TRY_HIP(ctx, a->deg.ensure((size_t)n * sizeof(uint32_t)));
     RBG_TRY_HIP(ctx, a->cursor.ensure((size_t)n * sizeof(uint32_t)));
     RBG_TRY_HIP(ctx, a->off.ensure((size_t)(n + 1) * sizeof(uint64_t)));
     RBG_TRY_HIP(ctx, hipMemsetAsync(a->deg.p, 0, (size_t)n * 4, st));
     RBG_TRY_HIP(ctx, hipMemsetAsync(a->cursor.p, 0, (size_t)n * 4, st));
-    RBG_TRY_HIP(ctx, hipMemsetAsync(a->rowmax.p, 0, (size_t)n * 4, st));
-    RBG_TRY_HIP(ctx, launch_survivor_counts((const uint32_t*)a->planes.p, W, (const uint64_t*)a->surv.p,
-                                            ns, (uint16_t*)a->scnt.p, (uint32_t*)a->deg.p,
-                                            (uint32_t*)a->rowmax.p, st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[7], st));
+    RBG_TRY_HIP(ctx, launch_pair_compare((const uint32_t*)a->planes.p, W, (uint64_t*)a->surv.p, ns,
+                                         (uint32_t*)a->deg.p, st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[8], st));
     size_t tmp_bytes = 0;
     RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64(nullptr, nullptr, n, nullptr, tmp_bytes, st));
     RBG_TRY_HIP(ctx, a->scan_tmp.ensure(tmp_bytes));
     RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)a->deg.p, (uint64_t*)a->off.p, n,
                                                a->scan_tmp.p, tmp_bytes, st));
     RBG_TRY_HIP(ctx, a->list.ensure(std::max<uint64_t>(2 * ns, 1) * sizeof(uint64_t)));
-    RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)a->surv.p, (const uint16_t*)a->scnt.p, ns,
-                                    (const uint64_t*)a->off.p, (uint32_t*)a->cursor.p,
-                                    (uint64_t*)a->list.p, st));
+    RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)a->surv.p, ns, (const uint64_t*)a->off.p,
+                                    (uint32_t*)a->cursor.p, (uint64_t*)a->list.p, st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[3], st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[9], st));
     RBG_TRY_HIP(ctx, launch_select_list((const uint64_t*)a->list.p, (const uint64_t*)a->off.p, n,
                                         (const uint16_t*)a->level.p, nlevels, K,
                                         (uint32_t*)a->topk_id.p, (uint16_t*)a->topk_cnt.p,
                                         (uint32_t*)a->topk_len.p, (uint32_t*)a->rowmax.p, st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[10], st));
   }
   // calibration on the host (glibc rand stream)
   std::vector<uint32_t> rowmax(n);
@@ -373,20 +413,22 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
                                          (float*)a->escore.p, nullptr, st));
   }
   RBG_TRY_HIP(ctx, hipEventRecord(ev[5], st));
-  // results to the host
   out->n_edges = ne;
-  const size_t nb = std::max<uint64_t>(ne, 1);
-  out->from = (uint32_t*)std::malloc(nb * 4);
-  out->to = (uint32_t*)std::malloc(nb * 4);
-  out->scomp = (float*)std::malloc(nb * 4);
-  out->score = S > 0 ? (float*)std::malloc(nb * 4) : nullptr;
-  if (!out->from || !out->to || !out->scomp || (S > 0 && !out->score))
-    return ctx->fail(RBG_ERR_NOMEM, "host allocation of the edge list");
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(out->from, a->efrom.p, ne * 4, hipMemcpyDeviceToHost, st));
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(out->to, a->eto.p, ne * 4, hipMemcpyDeviceToHost, st));
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(out->scomp, a->escomp.p, ne * 4, hipMemcpyDeviceToHost, st));
-  if (S > 0)
-    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->score, a->escore.p, ne * 4, hipMemcpyDeviceToHost, st));
+  a->last_edges = ne;
+  if (!(P->flags & RBG_EDGES_ON_DEVICE)) {  // results to the host
+    const size_t nb = std::max<uint64_t>(ne, 1);
+    out->from = (uint32_t*)std::malloc(nb * 4);
+    out->to = (uint32_t*)std::malloc(nb * 4);
+    out->scomp = (float*)std::malloc(nb * 4);
+    out->score = S > 0 ? (float*)std::malloc(nb * 4) : nullptr;
+    if (!out->from || !out->to || !out->scomp || (S > 0 && !out->score))
+      return ctx->fail(RBG_ERR_NOMEM, "host allocation of the edge list");
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->from, a->efrom.p, ne * 4, hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->to, a->eto.p, ne * 4, hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->scomp, a->escomp.p, ne * 4, hipMemcpyDeviceToHost, st));
+    if (S > 0)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(out->score, a->escore.p, ne * 4, hipMemcpyDeviceToHost, st));
+  }
   if (diag) {
     if (diag->topk_id)
       RBG_TRY_HIP(ctx, hipMemcpyAsync(diag->topk_id, a->topk_id.p, (size_t)n * K * 4, hipMemcpyDeviceToHost, st));
@@ -413,6 +455,9 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   out->ms_select = elapsed_ms(ev[3], ev[4]);
   out->ms_edges = elapsed_ms(ev[4], ev[5]);
   out->ms_total = elapsed_ms(ev[0], ev[5]);
+  out->ms_k_pair = elapsed_ms(ev[2], ev[6]);
+  out->ms_k_compare = elapsed_ms(ev[7], ev[8]);
+  out->ms_k_select = elapsed_ms(ev[9], ev[10]);
   return RBG_OK;
 }
 
@@ -462,6 +507,7 @@ void rbg_default_params(rbg_graph_params* p) {
   p->pmh_seed = 42;
   p->rand_seed = 1;
   p->regime = 0;
+  p->flags = 0;
 }
 
 rbg_status rbg_sketch_winners(rbg_ctx* ctx, const char* seqs, const uint64_t* offsets, uint32_t n,
@@ -516,6 +562,22 @@ rbg_status rbg_asm_build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params*
   const rbg_status s = build_graph(ctx, a, params, out, diag);
   if (s != RBG_OK) rbg_graph_result_free(out);
   return s;
+}
+
+rbg_status rbg_asm_copy_edges(rbg_ctx* ctx, rbg_asm* a, uint32_t* from, uint32_t* to,
+                              float* scomp, float* score) {
+  if (!ctx || !a) return RBG_ERR_ARG;
+  const uint64_t ne = a->last_edges;
+  if (ne == 0) return RBG_OK;
+  RBG_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (from) RBG_TRY_HIP(ctx, hipMemcpyAsync(from, a->efrom.p, ne * 4, hipMemcpyDeviceToHost, st));
+  if (to) RBG_TRY_HIP(ctx, hipMemcpyAsync(to, a->eto.p, ne * 4, hipMemcpyDeviceToHost, st));
+  if (scomp) RBG_TRY_HIP(ctx, hipMemcpyAsync(scomp, a->escomp.p, ne * 4, hipMemcpyDeviceToHost, st));
+  if (score && a->S > 0)
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(score, a->escore.p, ne * 4, hipMemcpyDeviceToHost, st));
+  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+  return RBG_OK;
 }
 
 rbg_status rbg_build_graph(rbg_ctx* ctx, const char* seqs, const uint64_t* offsets, uint32_t n,

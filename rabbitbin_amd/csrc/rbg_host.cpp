@@ -296,6 +296,49 @@ void depth_features(const float* depth, size_t n, size_t S, double min_cv, float
     }
 }
 
+void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g) {
+  g.S = S;
+  g.kpad = 32 * ((S + 31) / 32);
+  g.q.assign(n * g.kpad, 0);
+  double amax = 0;
+  for (size_t t = 0; t < n * S; ++t) amax = std::max(amax, (double)std::fabs(unit[t]));
+  g.scale = amax > 0 ? 127.0 / amax : 0.0;
+  g.max_qnorm = g.max_err = g.max_unorm = 0;
+  for (size_t r = 0; r < n; ++r) {
+    double qq = 0, ee = 0, uu = 0;
+    for (uint32_t k = 0; k < S; ++k) {
+      const double u = unit[r * S + k];
+      long v = g.scale > 0 ? std::lround(u * g.scale) : 0;
+      v = std::max(-127L, std::min(127L, v));
+      g.q[r * g.kpad + k] = (int8_t)v;
+      const double x = g.scale > 0 ? (double)v / g.scale : 0.0;
+      qq += x * x;
+      ee += (u - x) * (u - x);
+      uu += u * u;
+    }
+    g.max_qnorm = std::max(g.max_qnorm, std::sqrt(qq));
+    g.max_err = std::max(g.max_err, std::sqrt(ee));
+    g.max_unorm = std::max(g.max_unorm, std::sqrt(uu));
+  }
+}
+
+int32_t gate_qcut(const GateQuant& g, float thr) {
+  if (g.scale <= 0) return INT32_MIN;
+  // FP32 dot: each product passes through at most ceil(S/16) lane FMAs, the
+  // acc0 + acc1 add and four reduction levels.
+  const double u = std::ldexp(1.0, -24);
+  const double depth = (double)((g.S + 15) / 16 + 6);
+  const double gamma = depth * u / (1.0 - depth * u);
+  // |dot(u_i,u_j) - dot(q_i,q_j)/s^2| <= 2 X E + E^2 (Cauchy-Schwarz), plus
+  // a margin for the double arithmetic of this bound.
+  const double slack = 2.0 * g.max_qnorm * g.max_err + g.max_err * g.max_err +
+                       gamma * g.max_unorm * g.max_unorm + 1e-6;
+  const double cut = g.scale * g.scale * ((double)thr - slack);
+  if (cut <= (double)INT32_MIN + 1) return INT32_MIN;
+  if (cut > (double)INT32_MAX) return INT32_MAX;
+  return (int32_t)std::ceil(cut);
+}
+
 RandStream::RandStream(unsigned seed) {
   std::memset(&rd_, 0, sizeof(rd_));
   initstate_r(seed, state_, sizeof(state_), &rd_);

@@ -42,6 +42,25 @@ void sv_tables(uint32_t m, double b0, std::vector<double>& svd, std::vector<floa
 void depth_features(const float* depth, size_t n, size_t S, double min_cv, float* ranked,
                     float* unit, float* raw, double* colnorm, uint8_t* anynz);
 
+// int8 image of the unit rows for the MFMA prefilter of the abundance gate.
+// q = round(u * scale) with scale = 127 / max|u| (zero padded to kpad).  The
+// prefilter keeps a pair iff its integer dot >= gate_qcut(thr); gate_qcut
+// charges the quantisation error (Cauchy-Schwarz over the per-row error
+// norms) and the forward error of the FP32 dot's rounding chain, so every
+// pair whose exact FP32 gate value reaches thr is kept.
+struct GateQuant {
+  std::vector<int8_t> q;  // n x kpad
+  uint32_t kpad = 0;
+  double scale = 0;       // 0 when every unit entry is zero
+  double max_qnorm = 0;   // max_i ||q_i / scale||
+  double max_err = 0;     // max_i ||u_i - q_i / scale||
+  double max_unorm = 0;   // max_i ||u_i||
+  uint32_t S = 0;
+};
+void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g);
+// Smallest integer dot a kept pair can have; INT32_MIN disables the prefilter.
+int32_t gate_qcut(const GateQuant& g, float thr);
+
 // glibc rand() stream with private state (srand(seed) then rand()).
 class RandStream {
  public:

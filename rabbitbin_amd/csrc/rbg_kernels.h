@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace rbg {
 
 hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
@@ -18,11 +20,18 @@ hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, uint32_t n, u
 hipError_t launch_gate(const float* unit, uint32_t NC, const uint8_t* anynz, uint32_t n, float thr,
                        uint32_t acc1_end, uint64_t* surv, unsigned long long* nsurv, uint64_t cap,
                        hipStream_t st);
-hipError_t launch_survivor_counts(const uint32_t* planes, uint32_t W, const uint64_t* surv,
-                                  uint64_t ns, uint16_t* cnt, uint32_t* deg, uint32_t* rowmax,
-                                  hipStream_t st);
-hipError_t launch_scatter(const uint64_t* surv, const uint16_t* cnt, uint64_t ns,
-                          const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st);
+// Work items of the fused pair pass: (bi << 16 | bj0, tile count) per item.
+void gate_work_items(uint32_t n, std::vector<uint32_t>& items);
+hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, uint32_t n, int32_t qcut,
+                            const uint32_t* items, uint32_t nitems, uint64_t* cand,
+                            unsigned long long* ncand, uint64_t cap, hipStream_t st);
+hipError_t launch_exact_gate(const uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,
+                             const uint8_t* anynz, float thr, uint32_t acc1_end, uint64_t* surv,
+                             unsigned long long* nsurv, uint64_t cap, hipStream_t st);
+hipError_t launch_pair_compare(const uint32_t* planes, uint32_t W, uint64_t* surv, uint64_t ns,
+                               uint32_t* deg, hipStream_t st);
+hipError_t launch_scatter(const uint64_t* surv, uint64_t ns, const uint64_t* off, uint32_t* cursor,
+                          uint64_t* list, hipStream_t st);
 hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32_t n,
                                const uint16_t* level, uint32_t nlevels, uint32_t K,
                                uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,

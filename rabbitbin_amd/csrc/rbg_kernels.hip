@@ -337,35 +337,302 @@ __global__ __launch_bounds__(256) void gate_kernel(const float* __restrict__ uni
   }
 }
 
-// Register compare of the gated pairs; per-row degree and maximum count.
-__global__ __launch_bounds__(256) void survivor_counts_kernel(
-    const uint32_t* __restrict__ planes, uint32_t W, const uint64_t* __restrict__ surv,
-    uint64_t ns, uint16_t* __restrict__ cnt, uint32_t* __restrict__ deg,
-    uint32_t* __restrict__ rowmax) {
+// Exact FP32 gate value of one pair, both rows in VGPRs (same order as
+// gate_kernel; rows are 16*NC floats, zero padded).
+template <int NC>
+__device__ __forceinline__ float exact_gate_dot(const float* __restrict__ ua,
+                                                const float* __restrict__ ub, uint32_t acc1_end) {
+  float acc0[16], acc1[16];
+#pragma unroll
+  for (int l = 0; l < 16; ++l) {
+    acc0[l] = 0.0f;
+    acc1[l] = 0.0f;
+  }
+  bool used1 = false;
+  // one 16-wide chunk at a time keeps only 32 operand registers live
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    float a[16], b[16];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float4 x = reinterpret_cast<const float4*>(ua)[4 * c + t];
+      const float4 y = reinterpret_cast<const float4*>(ub)[4 * c + t];
+      a[4 * t] = x.x; a[4 * t + 1] = x.y; a[4 * t + 2] = x.z; a[4 * t + 3] = x.w;
+      b[4 * t] = y.x; b[4 * t + 1] = y.y; b[4 * t + 2] = y.z; b[4 * t + 3] = y.w;
+    }
+    if ((c & 1) && (uint32_t)c < acc1_end) {
+      used1 = true;
+#pragma unroll
+      for (int l = 0; l < 16; ++l) acc1[l] = __builtin_fmaf(a[l], b[l], acc1[l]);
+    } else {
+#pragma unroll
+      for (int l = 0; l < 16; ++l) acc0[l] = __builtin_fmaf(a[l], b[l], acc0[l]);
+    }
+  }
+  if (used1) {
+#pragma unroll
+    for (int l = 0; l < 16; ++l) acc0[l] = acc0[l] + acc1[l];
+  }
+  float t8[8], t4[4];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) t8[l] = acc0[l + 8] + acc0[l];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) t4[l] = t8[l + 4] + t8[l];
+  return (t4[0] + t4[2]) + (t4[1] + t4[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused regime, abundance gate in three kernels:
+//  1. gate_mfma_kernel: int8 MFMA prefilter over the upper triangle, emits
+//     candidates (i < j, qdot >= qcut);
+//  2. exact_gate_kernel: is_nz and the exact FP32 gate (exact_gate_dot) per
+//     candidate, compacting the survivors;
+//  3. pair_compare_kernel: register compare of the survivors.
+// qcut is a rigorous lower bound (gate_qcut), so the survivors are exactly
+// the pairs the reference's gate keeps.
+//
+// gate_mfma_kernel: work item = a 256-row stripe bi and up to 16 consecutive
+// 256-column tiles bj0.. (bi <= bj).  Wave w owns rows bi*256+64w..+64 as four
+// 16-row operands held in registers for the whole item; the tiles' column
+// operands are double-buffered in LDS (the next tile's loads are in flight
+// while the current tile computes).  v_mfma_i32_16x16x32_i8 gives the integer
+// dot of the quantised rows for 16 x 16 pairs (lane l, register v -> row
+// 4*(l/16)+v, column l%16).  Candidates are staged per wave in LDS as local
+// coordinates and appended to `cand` with one global atomic per flush.
+// ---------------------------------------------------------------------------
+typedef int v4i32 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kGateBuf = 512;  // per-wave candidate buffer (a 16-row block adds <= 256)
+constexpr uint32_t kGateTilesPerItem = 16;
+
+template <int KC>
+__device__ __forceinline__ void load_q8(const int8_t* __restrict__ q, uint32_t row, uint32_t n,
+                                        uint32_t kpad, uint32_t ln, long* out) {
+#pragma unroll
+  for (int c = 0; c < KC; ++c)
+    out[c] = row < n ? *reinterpret_cast<const long*>(q + (size_t)row * kpad + 32 * c + 8 * (ln >> 4)) : 0L;
+}
+
+// This thread's 4 entries of a tile's column operands: entry e = tid + 256k
+// is (step e/64, lane e%64).
+template <int KC>
+__device__ __forceinline__ void load_tile_ops(const int8_t* __restrict__ q, uint32_t jb, uint32_t n,
+                                              long (*dst)[KC]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t e = threadIdx.x + 256u * k;
+    load_q8<KC>(q, jb + 16u * (e >> 6) + (e & 15u), n, 32 * KC, e & 63u, dst[k]);
+  }
+}
+
+template <int KC>
+__global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict__ q, uint32_t n,
+                                                        int32_t qcut, const uint2* __restrict__ items,
+                                                        uint64_t* __restrict__ cand,
+                                                        unsigned long long* __restrict__ ncand,
+                                                        uint64_t cap) {
+  __shared__ uint32_t buf[4][kGateBuf];
+  __shared__ long bops[2][16][KC][64];
+  const uint32_t wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
+  const uint2 item = items[blockIdx.x];
+  const uint32_t ib = (item.x >> 16) * 256u, jbase = (item.x & 0xFFFFu) * 256u;
+  const uint32_t ntile = item.y;
+  const uint32_t row0 = ib + 64 * wv;
+
+  // Row operands go through LDS (buffer 1 is free until tile 1), so inside
+  // the loop the only outstanding vector-memory loads are the prefetch of
+  // the next tile.
+  {
+    long ra[4][KC];
+    load_tile_ops<KC>(q, ib, n, ra);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t e = threadIdx.x + 256u * k;
+#pragma unroll
+      for (int c = 0; c < KC; ++c) bops[1][e >> 6][c][e & 63u] = ra[k][c];
+    }
+  }
+  long nb[4][KC];
+  load_tile_ops<KC>(q, jbase, n, nb);
+  __syncthreads();
+  long A[4][KC];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < KC; ++c) A[r][c] = bops[1][4 * wv + r][c][ln];
+  __syncthreads();
+  uint32_t bn = 0;  // wave-uniform count of staged candidates
+
+  auto flush = [&]() {
+    __builtin_amdgcn_wave_barrier();  // entries written by other lanes
+    unsigned long long base = 0;
+    if (ln == 0) base = atomicAdd(ncand, (unsigned long long)bn);
+    base = __shfl(base, 0);
+    for (uint32_t t = ln; t < bn; t += 64) {
+      const uint32_t e = buf[wv][t];
+      const uint64_t i = ib + (e >> 12), j = jbase + (e & 0xFFFu);
+      if (base + t < cap) cand[base + t] = (i << 40) | (j << 16);
+    }
+    bn = 0;
+  };
+
+  const uint32_t total = ntile * 16;
+#pragma unroll 1
+  for (uint32_t u = 0; u < total; ++u) {
+    const uint32_t t = u >> 4, s = u & 15u;
+    if (s == 0) {
+      // tile t's operands (prefetched) go to LDS; then prefetch tile t+1
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t e = threadIdx.x + 256u * k;
+#pragma unroll
+        for (int c = 0; c < KC; ++c) bops[t & 1][e >> 6][c][e & 63u] = nb[k][c];
+      }
+      __syncthreads();
+      if (t + 1 < ntile) load_tile_ops<KC>(q, jbase + 256u * (t + 1), n, nb);
+    }
+    const uint32_t jb = jbase + 256u * t;
+    const uint32_t j0 = jb + 16u * s;
+    const bool diag = ib == jb;
+    if (j0 < n && !(diag && j0 + 15 <= row0)) {  // skip blocks with every j <= i
+      long B[KC];
+#pragma unroll
+      for (int c = 0; c < KC; ++c) B[c] = bops[t & 1][s][c][ln];
+      v4i32 acc[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        acc[r] = v4i32{0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+          acc[r] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A[r][c], B[c], acc[r], 0, 0, 0);
+      }
+      int32_t mr[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mr[r] = max(max(acc[r][0], acc[r][1]), max(acc[r][2], acc[r][3]));
+      const int32_t mx = max(max(mr[0], mr[1]), max(mr[2], mr[3]));
+      if (__ballot(mx >= qcut) != 0) {  // some pair of these 1024 is near the gate
+        const uint32_t j = j0 + (ln & 15u);
+        const uint32_t jl = j - jbase;
+        const bool full = !diag && jb + 256 <= n;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (__ballot(mr[r] >= qcut) == 0) continue;  // only the 16-row block with candidates
+          if (bn > kGateBuf - 256) flush();
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const uint32_t i = row0 + 16 * r + 4 * (ln >> 4) + v;
+            const bool c = full ? acc[r][v] >= qcut : (acc[r][v] >= qcut && i < j && j < n);
+            const uint64_t mask = __ballot(c);
+            if (mask) {
+              if (c) buf[wv][bn + lanes_below(mask)] = ((i - ib) << 12) | jl;
+              bn += (uint32_t)__popcll(mask);
+            }
+          }
+        }
+      }
+    }
+  }
+  if (bn) flush();
+}
+
+// Exact gate of the int8 candidates: is_nz(i, j) and !(unit_dot_f < thr);
+// survivors are compacted per wave (one global atomic per wave).
+template <int NC>
+__global__ __launch_bounds__(256) void exact_gate_kernel(const uint64_t* __restrict__ cand,
+                                                         uint64_t nc, const float* __restrict__ unit,
+                                                         const uint8_t* __restrict__ anynz, float thr,
+                                                         uint32_t acc1_end,
+                                                         uint64_t* __restrict__ surv,
+                                                         unsigned long long* __restrict__ nsurv,
+                                                         uint64_t cap) {
   const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= ns) return;
-  const uint32_t i = (uint32_t)(surv[t] >> 32), j = (uint32_t)surv[t];
-  const uint32_t c =
-      pair_count_rows(planes + (size_t)i * 8u * W, planes + (size_t)j * 8u * W, W);
-  cnt[t] = (uint16_t)c;
-  atomicAdd(&deg[i], 1u);
-  atomicAdd(&deg[j], 1u);
-  atomicMax(&rowmax[i], c);
-  atomicMax(&rowmax[j], c);
+  bool pass = false;
+  uint64_t e = 0;
+  if (t < nc) {
+    e = cand[t];
+    const uint32_t i = (uint32_t)(e >> 40), j = (uint32_t)(e >> 16) & 0xFFFFFFu;
+    const bool nz = !anynz || anynz[i] || anynz[j];
+    pass = nz && !(exact_gate_dot<NC>(unit + (size_t)i * 16 * NC, unit + (size_t)j * 16 * NC,
+                                      acc1_end) < thr);
+  }
+  const uint64_t mask = __ballot(pass);
+  if (mask == 0) return;
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(nsurv, (unsigned long long)__popcll(mask));
+  base = __shfl(base, 0);
+  if (pass) {
+    const uint64_t slot = base + lanes_below(mask);
+    if (slot < cap) surv[slot] = e;
+  }
+}
+
+// Gated pairs are packed as (i << 40) | (j << 16) | count.
+__device__ __forceinline__ void unpack_pair(uint64_t e, uint32_t& i, uint32_t& j, uint32_t& c) {
+  i = (uint32_t)(e >> 40);
+  j = (uint32_t)(e >> 16) & 0xFFFFFFu;
+  c = (uint32_t)(e & 0xFFFFu);
+}
+
+// Register compare of the gated pairs (count written into the low 16 bits of
+// each packed pair) and the per-row candidate counts of both endpoints.
+// LPP lanes share one pair: lane q of the group takes word groups
+// [q*W/LPP, (q+1)*W/LPP) of both rows (8 planes x 4 bytes each, contiguous),
+// loads them with 16-byte loads in one batch, and the partial mismatch counts
+// meet in a shuffle reduction.
+template <int W>
+__global__ __launch_bounds__(256) void pair_compare_kernel(const uint32_t* __restrict__ planes,
+                                                           uint64_t* __restrict__ surv, uint64_t ns,
+                                                           uint32_t* __restrict__ deg) {
+  constexpr int LPP = W < 4 ? W : 4;  // lanes per pair
+  constexpr int WPL = W / LPP;        // word groups per lane
+  const uint64_t gt = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t t = gt / LPP;
+  const uint32_t part = (uint32_t)(gt % LPP);
+  uint32_t i = 0, j = 0, c = 0;
+  uint64_t e = 0;
+  uint32_t mism = 0;
+  if (t < ns) {
+    e = surv[t];
+    unpack_pair(e, i, j, c);
+    const uint4* ra = reinterpret_cast<const uint4*>(planes + (size_t)i * 8u * W + 8u * WPL * part);
+    const uint4* rb = reinterpret_cast<const uint4*>(planes + (size_t)j * 8u * W + 8u * WPL * part);
+    uint4 a[2 * WPL], b[2 * WPL];
+#pragma unroll
+    for (int x = 0; x < 2 * WPL; ++x) {
+      a[x] = ra[x];
+      b[x] = rb[x];
+    }
+#pragma unroll
+    for (int w = 0; w < WPL; ++w) {
+      const uint32_t av[8] = {a[2 * w].x, a[2 * w].y, a[2 * w].z, a[2 * w].w,
+                              a[2 * w + 1].x, a[2 * w + 1].y, a[2 * w + 1].z, a[2 * w + 1].w};
+      const uint32_t bs[8] = {b[2 * w].x, b[2 * w].y, b[2 * w].z, b[2 * w].w,
+                              b[2 * w + 1].x, b[2 * w + 1].y, b[2 * w + 1].z, b[2 * w + 1].w};
+      uint32_t bv[8];
+      column_word(bs, bv);
+      mism += __popc(word_mismatch(av, bv));
+    }
+  }
+#pragma unroll
+  for (int d = 1; d < LPP; d <<= 1) mism += __shfl_xor(mism, d);
+  if (t < ns && part == 0) {
+    surv[t] = (e & ~0xFFFFull) | (uint64_t)(32u * W - mism);
+    atomicAdd(&deg[i], 1u);
+    atomicAdd(&deg[j], 1u);
+  }
 }
 
 // Scatter each gated pair into both rows' candidate lists: entry = (cnt<<32)|id.
-__global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict__ surv,
-                                                      const uint16_t* __restrict__ cnt, uint64_t ns,
+__global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict__ surv, uint64_t ns,
                                                       const uint64_t* __restrict__ off,
                                                       uint32_t* __restrict__ cursor,
                                                       uint64_t* __restrict__ list) {
   const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= ns) return;
-  const uint32_t i = (uint32_t)(surv[t] >> 32), j = (uint32_t)surv[t];
-  const uint64_t c = (uint64_t)cnt[t] << 32;
-  list[off[i] + atomicAdd(&cursor[i], 1u)] = c | j;
-  list[off[j] + atomicAdd(&cursor[j], 1u)] = c | i;
+  uint32_t i, j, c;
+  unpack_pair(surv[t], i, j, c);
+  const uint64_t hi = (uint64_t)c << 32;
+  list[off[i] + atomicAdd(&cursor[i], 1u)] = hi | j;
+  list[off[j] + atomicAdd(&cursor[j], 1u)] = hi | i;
 }
 
 // ---------------------------------------------------------------------------
@@ -821,20 +1088,70 @@ hipError_t launch_gate(const float* unit, uint32_t NC, const uint8_t* anynz, uin
   return hipGetLastError();
 }
 
-hipError_t launch_survivor_counts(const uint32_t* planes, uint32_t W, const uint64_t* surv,
-                                  uint64_t ns, uint16_t* cnt, uint32_t* deg, uint32_t* rowmax,
-                                  hipStream_t st) {
-  if (ns == 0) return hipSuccess;
-  hipLaunchKernelGGL(survivor_counts_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
-                     planes, W, surv, ns, cnt, deg, rowmax);
+void gate_work_items(uint32_t n, std::vector<uint32_t>& items) {
+  // (bi << 16 | bj0, tile count) pairs; full items first
+  const uint32_t T = (n + 255) / 256;
+  std::vector<uint32_t> full, part;
+  for (uint32_t bi = 0; bi < T; ++bi)
+    for (uint32_t bj = bi; bj < T; bj += kGateTilesPerItem) {
+      const uint32_t cnt = std::min(kGateTilesPerItem, T - bj);
+      auto& dst = cnt == kGateTilesPerItem ? full : part;
+      dst.push_back((bi << 16) | bj);
+      dst.push_back(cnt);
+    }
+  items = std::move(full);
+  items.insert(items.end(), part.begin(), part.end());
+}
+
+hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, uint32_t n, int32_t qcut,
+                            const uint32_t* items, uint32_t nitems, uint64_t* cand,
+                            unsigned long long* ncand, uint64_t cap, hipStream_t st) {
+  if (nitems == 0) return hipSuccess;
+  const uint2* it = reinterpret_cast<const uint2*>(items);
+  switch (kpad / 32) {
+    case 1: hipLaunchKernelGGL(gate_mfma_kernel<1>, dim3(nitems), dim3(256), 0, st, q, n, qcut, it, cand, ncand, cap); break;
+    case 2: hipLaunchKernelGGL(gate_mfma_kernel<2>, dim3(nitems), dim3(256), 0, st, q, n, qcut, it, cand, ncand, cap); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_scatter(const uint64_t* surv, const uint16_t* cnt, uint64_t ns,
-                          const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st) {
+hipError_t launch_exact_gate(const uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,
+                             const uint8_t* anynz, float thr, uint32_t acc1_end, uint64_t* surv,
+                             unsigned long long* nsurv, uint64_t cap, hipStream_t st) {
+  if (nc == 0) return hipSuccess;
+  const dim3 grid((unsigned)((nc + 255) / 256));
+  switch (NC) {
+    case 1: hipLaunchKernelGGL(exact_gate_kernel<1>, grid, dim3(256), 0, st, cand, nc, unit, anynz, thr, acc1_end, surv, nsurv, cap); break;
+    case 2: hipLaunchKernelGGL(exact_gate_kernel<2>, grid, dim3(256), 0, st, cand, nc, unit, anynz, thr, acc1_end, surv, nsurv, cap); break;
+    case 3: hipLaunchKernelGGL(exact_gate_kernel<3>, grid, dim3(256), 0, st, cand, nc, unit, anynz, thr, acc1_end, surv, nsurv, cap); break;
+    case 4: hipLaunchKernelGGL(exact_gate_kernel<4>, grid, dim3(256), 0, st, cand, nc, unit, anynz, thr, acc1_end, surv, nsurv, cap); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_compare(const uint32_t* planes, uint32_t W, uint64_t* surv, uint64_t ns,
+                               uint32_t* deg, hipStream_t st) {
   if (ns == 0) return hipSuccess;
-  hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, surv, cnt,
-                     ns, off, cursor, list);
+  const uint32_t lpp = W < 4 ? W : 4;
+  const dim3 grid((unsigned)((ns * lpp + 255) / 256));
+  switch (W) {
+    case 16: hipLaunchKernelGGL(pair_compare_kernel<16>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
+    case 8: hipLaunchKernelGGL(pair_compare_kernel<8>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
+    case 4: hipLaunchKernelGGL(pair_compare_kernel<4>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
+    case 2: hipLaunchKernelGGL(pair_compare_kernel<2>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
+    case 1: hipLaunchKernelGGL(pair_compare_kernel<1>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(const uint64_t* surv, uint64_t ns, const uint64_t* off, uint32_t* cursor,
+                          uint64_t* list, hipStream_t st) {
+  if (ns == 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, surv, ns,
+                     off, cursor, list);
   return hipGetLastError();
 }
 

@@ -49,8 +49,12 @@ def test_graph_small_regime_2k(ctx):
     _check_graph(g, ref)
 
 
-def test_graph_fused_regime_forced_3k(ctx):
-    seqs, off, depth, _ = synth.make_assembly(3000, 30, np.log(8000), 0.8, 10, seed=99)
+@pytest.mark.parametrize("S", [2, 3, 10, 30, 40, 64])
+def test_graph_fused_regime_forced_3k(ctx, S):
+    # S selects the gate variant: 2 -> no gate (composition weight 1), 3..16
+    # one 16-lane chunk, 30 two chunks in acc0, 40 a 32-wide chunk (acc1),
+    # 64 two 32-wide chunks.
+    seqs, off, depth, _ = synth.make_assembly(3000, 30, np.log(8000), 0.8, S, seed=99 + S)
     p = api.default_params(rand_seed=3, regime=2)
     g = ctx.build_graph(seqs, off, depth, p, diag=True)
     ref = orc.full_pipeline(seqs, off, depth, rand_seed=3, force_regime=2)
