@@ -228,3 +228,144 @@ def full_pipeline(seqs, offsets, depth, rand_seed=1, m=500, K=200, force_regime=
         g["score_ref"], _ = edge_weights(g["from"], g["to"], g["sv"], unit, raw,
                                          colnorm if S > 1 else None, S, mode=0)
     return g
+
+
+# ---- primitives used by the pinning tests ---------------------------------
+
+def _rng_lib(which):
+    return orc() if which == "orc" else ref()
+
+
+def wyrand_seq(state, n, which="orc"):
+    st = np.array([state], np.uint64)
+    out = np.zeros(n, np.uint64)
+    getattr(_rng_lib(which), f"{which}_wyrand_seq")(_p(st, u64p), n, _p(out, u64p))
+    return out, int(st[0])
+
+
+def wy_uniform_int_seq(state, bound, n, which="orc"):
+    st = np.array([state], np.uint64)
+    out = np.zeros(n, np.uint32)
+    getattr(_rng_lib(which), f"{which}_wy_uniform_int_seq")(_p(st, u64p), bound, n, _p(out, u32p))
+    return out, int(st[0])
+
+
+def zig_seq(state, n, which="orc"):
+    st = np.array([state], np.uint64)
+    out = np.zeros(n, np.float64)
+    getattr(_rng_lib(which), f"{which}_zig_seq")(_p(st, u64p), n, _p(out, f64p))
+    return out, int(st[0])
+
+
+def ted_seq(c1, c2, c3, state, n, which="orc"):
+    st = np.array([state], np.uint64)
+    out = np.zeros(n, np.float64)
+    getattr(_rng_lib(which), f"{which}_ted_seq")(c1, c2, c3, _p(st, u64p), n, _p(out, f64p))
+    return out, int(st[0])
+
+
+def ted_params(m):
+    """TED table rows (boundary, gap, c1, c2, c3) for i < m-1, and 1/firstBoundary."""
+    out = np.zeros((m - 1, 5), np.float64)
+    fbi = C.c_double()
+    orc().orc_ted_params(m, _p(out, f64p), C.byref(fbi))
+    return out, fbi.value
+
+
+def fmix64(x, seed, which="orc"):
+    return int(getattr(_rng_lib(which), f"{which}_fmix64")(x, seed))
+
+
+def pmh_stream(hashes, weights, m, seed=42, which="orc"):
+    """Insert (hash, weight) pairs in order into a fresh streaming sketch;
+    return (winners u64[m], registers f64[m])."""
+    L = _rng_lib(which)
+    h = L.orc_pmh_new(m, seed) if which == "orc" else L.ref_pmh_new(m, 4, seed)
+    try:
+        for x, w in zip(hashes, weights):
+            getattr(L, f"{which}_pmh_add_hash")(h, int(x), float(w))
+        win = np.zeros(m, np.uint64)
+        reg = np.zeros(m, np.float64)
+        getattr(L, f"{which}_pmh_get")(h, _p(win, u64p), _p(reg, f64p))
+    finally:
+        getattr(L, f"{which}_pmh_free")(h)
+    return win, reg
+
+
+def zig_tables_ref():
+    x = np.zeros(257, np.float64)
+    y = np.zeros(257, np.float64)
+    ref().ref_zig_tables(_p(x, f64p), _p(y, f64p))
+    return x, y
+
+
+def kmer4_hist(seq):
+    cnt = np.zeros(256, np.uint32)
+    base = np.zeros(4, np.uint64)
+    tot = C.c_uint64()
+    orc().orc_kmer4_hist(seq, len(seq), _p(cnt, u32p), _p(base, u64p), C.byref(tot))
+    return cnt, base, tot.value
+
+
+def kmer4_weights(cnt, base, total, cap=20.0):
+    w = np.zeros(256, np.float64)
+    c = np.ascontiguousarray(cnt, np.uint32)
+    b = np.ascontiguousarray(base, np.uint64)
+    orc().orc_kmer4_weights(_p(c, u32p), _p(b, u64p), total, cap, _p(w, f64p))
+    return w
+
+
+def rand_seq(seed, n):
+    out = np.zeros(n, np.int32)
+    orc().orc_rand_seq(seed, n, out.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out
+
+
+def mt19937_64_nth(k):
+    L = orc()
+    L.orc_mt19937_64_nth.restype = C.c_uint64
+    L.orc_mt19937_64_nth.argtypes = [C.c_uint64]
+    return int(L.orc_mt19937_64_nth(k))
+
+
+def baseline_draws(n):
+    """The 30000 (i, j) draws of estimate_pmh_baseline for n rows."""
+    L = orc()
+    L.orc_baseline_draws.restype = C.c_uint32
+    L.orc_baseline_draws.argtypes = [C.c_uint64, u64p, u64p]
+    i = np.zeros(30000, np.uint64)
+    j = np.zeros(30000, np.uint64)
+    L.orc_baseline_draws(n, _p(i, u64p), _p(j, u64p))
+    return i, j
+
+
+def unit_dot_f(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return np.float32(orc().orc_unit_dot_f(_p(a, f32p), _p(b, f32p), len(a)))
+
+
+def unit_dot_f_avx512(a, b):
+    """The same dot product through AVX-512 intrinsics compiled by the host
+    compiler (None when the host lacks AVX-512F)."""
+    L = orc()
+    L.orc_unit_dot_f_avx512.restype = C.c_int
+    L.orc_unit_dot_f_avx512.argtypes = [f32p, f32p, C.c_uint32, f32p]
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = C.c_float()
+    if L.orc_unit_dot_f_avx512(_p(a, f32p), _p(b, f32p), len(a), C.byref(out)) != 0:
+        return None
+    return np.float32(out.value)
+
+
+def sv_from_count(cnt, m, b0):
+    return orc().orc_sv_from_count(cnt, m, b0)
+
+
+def calibrate(rowmax, m, b0, regime, coverage=0.95, rand_seed=1):
+    L = orc()
+    L.orc_calibrate.restype = C.c_double
+    L.orc_calibrate.argtypes = [u32p, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint, C.c_int]
+    rm = np.ascontiguousarray(rowmax, np.uint32)
+    return L.orc_calibrate(_p(rm, u32p), len(rm), m, b0, coverage, rand_seed, regime)

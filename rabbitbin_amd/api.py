@@ -295,3 +295,30 @@ def depth_features(depth, min_cv=1.0):
     if st != 0:
         raise RbgError(f"rbg_depth_features: status {st}")
     return ranked, unit, raw, colnorm, anynz
+
+
+N_CANON4 = 136  # canonical 4-mers (min of a code and its reverse complement)
+
+
+def sketch_tables(m=500, seed=42):
+    """rbg_sketch_tables: (T [136, m] f64, W [136, m] u64, cls [m, 136] u8).
+
+    Element x is the x-th canonical 4-mer in ascending code order; T/W are its
+    proposal values at weight 1 and its winner identities (src/probmh.cpp:634-686)."""
+    T = np.zeros((N_CANON4, m), np.float64)
+    W = np.zeros((N_CANON4, m), np.uint64)
+    cls = np.zeros((m, N_CANON4), np.uint8)
+    st = lib().rbg_sketch_tables(m, seed, _p(T, f64p), _p(W, u64p), _p(cls, u8p))
+    if st != 0:
+        raise RbgError(f"rbg_sketch_tables(m={m}): status {st}")
+    return T, W, cls
+
+
+def calibrate(rowmax_cnt, m, b0, regime, coverage=0.95, rand_seed=1):
+    """rbg_calibrate: simCutoff (x1000) from per-row maximum match counts."""
+    rm = np.ascontiguousarray(rowmax_cnt, np.uint32)
+    out = C.c_double()
+    st = lib().rbg_calibrate(_p(rm, u32p), len(rm), m, b0, coverage, rand_seed, regime, C.byref(out))
+    if st != 0:
+        raise RbgError(f"rbg_calibrate(n={len(rm)}, regime={regime}): status {st}")
+    return out.value

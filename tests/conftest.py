@@ -10,6 +10,13 @@ if REPO not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    # Build the product library and the oracle if a fresh checkout lacks them
+    # (make is incremental; normally __graft_entry__.build() has done this).
+    import subprocess
+    for sub, lib in (("rabbitbin_amd/csrc", "rabbitbin_amd/librbg.so"), ("oracle", "oracle/liboracle.so")):
+        if not os.path.exists(os.path.join(REPO, lib)):
+            subprocess.run(["make", "-C", os.path.join(REPO, sub), "-j8"], check=True,
+                           stdout=subprocess.DEVNULL)
 
 
 @pytest.fixture(scope="session")
