@@ -53,7 +53,7 @@ struct rbg_ctx {
   struct Tables {
     uint32_t m;
     uint64_t seed;
-    DevBuf T, cls, codes;
+    DevBuf T, cls, codes, sT, sX;
   };
   std::vector<std::unique_ptr<Tables>> tables;
   hipEvent_t ev[12] = {};
@@ -103,6 +103,10 @@ rbg_status get_tables(rbg_ctx* ctx, uint32_t m, uint64_t seed, rbg_ctx::Tables**
   RBG_TRY_HIP(ctx, hipMemcpy(t->T.p, pt.T.data(), pt.T.size() * sizeof(double), hipMemcpyHostToDevice));
   RBG_TRY_HIP(ctx, hipMemcpy(t->cls.p, pt.cls.data(), pt.cls.size(), hipMemcpyHostToDevice));
   RBG_TRY_HIP(ctx, hipMemcpy(t->codes.p, canonical_codes(), kElems, hipMemcpyHostToDevice));
+  RBG_TRY_HIP(ctx, t->sT.ensure(pt.sT.size() * sizeof(double)));
+  RBG_TRY_HIP(ctx, t->sX.ensure(pt.sX.size()));
+  RBG_TRY_HIP(ctx, hipMemcpy(t->sT.p, pt.sT.data(), pt.sT.size() * sizeof(double), hipMemcpyHostToDevice));
+  RBG_TRY_HIP(ctx, hipMemcpy(t->sX.p, pt.sX.data(), pt.sX.size(), hipMemcpyHostToDevice));
   *out = t.get();
   ctx->tables.push_back(std::move(t));
   return RBG_OK;
@@ -148,7 +152,9 @@ rbg_status asm_upload(rbg_ctx* ctx, rbg_asm* a, const char* seqs, const uint64_t
   a->seq_bytes = offsets[n] - offsets[0];
   std::vector<uint64_t> rel(n + 1);
   for (uint32_t r = 0; r <= n; ++r) rel[r] = offsets[r] - offsets[0];
-  RBG_TRY_HIP(ctx, a->seqs.ensure(a->seq_bytes));
+  // The sketch kernel reads whole 16-byte chunks: keep 64 readable bytes
+  // past the last contig (their contents are never counted).
+  RBG_TRY_HIP(ctx, a->seqs.ensure(a->seq_bytes + 64));
   RBG_TRY_HIP(ctx, a->offs.ensure((n + 1) * sizeof(uint64_t)));
   RBG_TRY_HIP(ctx, hipMemcpyAsync(a->seqs.p, seqs + offsets[0], a->seq_bytes, hipMemcpyHostToDevice,
                                   ctx->stream));
@@ -218,7 +224,7 @@ rbg_status run_sketch(rbg_ctx* ctx, rbg_asm* a, uint32_t m, uint64_t seed, bool 
   const uint32_t mpad = 32 * W;
   if (want_wx) RBG_TRY_HIP(ctx, a->wx.ensure((size_t)a->n * mpad));
   RBG_TRY_HIP(ctx, launch_sketch((const uint8_t*)a->seqs.p, (const uint64_t*)a->offs.p, a->n, m, W,
-                                 (const double*)t->T.p, (const uint8_t*)t->cls.p,
+                                 (const double*)t->sT.p, (const uint8_t*)t->sX.p, (const uint8_t*)t->cls.p,
                                  (const uint8_t*)t->codes.p, want_wx ? (uint8_t*)a->wx.p : nullptr,
                                  mpad, (uint32_t*)a->planes.p, ctx->stream));
   return RBG_OK;

@@ -205,6 +205,19 @@ const ProposalTables& proposal_tables(uint32_t m, uint64_t seed) {
           }
       t.cls[(size_t)k * kElems + x] = c;
     }
+  t.sT.resize((size_t)kElems * m);
+  t.sX.resize((size_t)kElems * m);
+  std::vector<int> ord(kElems);
+  for (uint32_t k = 0; k < m; ++k) {
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+      return t.T[(size_t)a * m + k] < t.T[(size_t)b * m + k];
+    });
+    for (int r = 0; r < kElems; ++r) {
+      t.sT[(size_t)r * m + k] = t.T[(size_t)ord[r] * m + k];
+      t.sX[(size_t)r * m + k] = (uint8_t)ord[r];
+    }
+  }
   return cache.emplace(key, std::move(t)).first->second;
 }
 

@@ -31,6 +31,11 @@ struct ProposalTables {
   std::vector<uint16_t> W16; // [136][m] fold_win16
   std::vector<uint8_t> cls;  // [m][136] class byte: smallest x' with equal
                              // fold16 at this register; 0xFF if fold16 == 0
+  // Per register k, the elements in ascending order of T[x][k] (ties: smaller
+  // x first), laid out [rank][k] so consecutive registers are contiguous:
+  // sT[r*m + k] = T[sX[r*m + k]][k].
+  std::vector<double> sT;
+  std::vector<uint8_t> sX;
 };
 const ProposalTables& proposal_tables(uint32_t m, uint64_t seed);
 

@@ -9,7 +9,8 @@
 namespace rbg {
 
 hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
-                         uint32_t W, const double* T, const uint8_t* cls, const uint8_t* codes,
+                         uint32_t W, const double* sT, const uint8_t* sX, const uint8_t* cls,
+                         const uint8_t* codes,
                          uint8_t* wx, uint32_t mpad, uint32_t* planes, hipStream_t st);
 hipError_t launch_list_counts(const uint32_t* planes, uint32_t W, const uint32_t* pi,
                               const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st);

@@ -72,103 +72,207 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Stage (a): sketch.  One workgroup sketches CPB contigs.
-// Phase 1 (per contig): canonical 4-mer histogram in LDS; a non-ACGT byte
-// breaks every 4-mer that covers it (src/rabbitbin.cpp:1467-1484).  Then the
+// Stage (a): sketch.  One workgroup of four waves sketches four contigs.
+//
+// Phase 1 (wave w, contig c0 + w): canonical 4-mer histogram in LDS
+// (src/rabbitbin.cpp:1459-1484).  Each lane takes 16-byte aligned chunks of
+// the sequence buffer plus the dword before the chunk (the 3-base overlap of
+// 4-mers that end in the chunk) and decodes them four bytes at a time:
+//   u = byte & 0xDF folds a/c/g/t onto A/C/G/T; the 2-bit code is
+//   ((u >> 1) ^ (u >> 2)) & 3 (A 0, C 1, G 2, T 3); a byte is a base when u
+//   is one of the four letters (any other byte breaks every 4-mer covering
+//   it, as the reference's window reset does); bytes outside the contig are
+//   masked off.
+// A 4-mer ending at byte r is valid when bytes r-3..r all are; its window
+// holds codes r-3..r in bits 0..7 (earliest base lowest), so the forward code
+// is the window with its 2-bit groups reversed and the reverse complement is
+// ~window & 0xFF.  The histogram bin is the smaller of the two.  Then the
 // GC-normalised weights in FP64, in the reference's operation order
 // (src/rabbitbin.cpp:1491-1516), kept as 1/w per element.
+//
 // Phase 2: thread t owns registers k = t, t+256, ...; for each register the
 // winner is argmin_x fl((1/w_x) * T[x][k]) over present elements, ties to the
 // smaller x (the sketch's strict-less update, src/probmh.cpp:417-431).
-// The proposal table T is read once per workgroup for all CPB contigs.
+// Elements are visited in ascending T[x][k] (sT/sX, shared by the four
+// contigs).  With wmin the smallest 1/w of a contig, every element at or
+// after rank r has fl((1/w_x) * T_x) >= fl(wmin * T_r) (both factors are no
+// smaller, and rounding is monotone), so once fl(wmin * T_r) > best no later
+// element can win or tie and the walk stops: typically after a few ranks.
+//
+// The sequence buffer must be readable 16 bytes past its last contig (the
+// last chunk of a contig is read whole).
 // ---------------------------------------------------------------------------
+constexpr int kSketchWaves = 4;
+
+// Bit 7 of each byte set iff that byte of x is non-zero.
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
+  return (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+}
+// Bit b (b < 4) set iff byte b of w is A, C, G or T in either case.
+__device__ __forceinline__ uint32_t base_bits4(uint32_t w) {
+  const uint32_t u = w & 0xDFDFDFDFu;
+  const uint32_t other = nonzero_bytes(u ^ 0x41414141u) & nonzero_bytes(u ^ 0x43434343u) &
+                         nonzero_bytes(u ^ 0x47474747u) & nonzero_bytes(u ^ 0x54545454u);
+  const uint32_t bb = ~other & 0x80808080u;
+  return ((bb >> 7) | (bb >> 14) | (bb >> 21) | (bb >> 28)) & 0xFu;
+}
+// 2-bit codes of the four bytes of w, byte b in bits 2b..2b+1.
+__device__ __forceinline__ uint32_t codes8(uint32_t w) {
+  const uint32_t u = w & 0xDFDFDFDFu;
+  uint32_t t = ((u >> 1) ^ (u >> 2)) & 0x03030303u;
+  t = (t | (t >> 6)) & 0x000F000Fu;
+  return (t | (t >> 12)) & 0xFFu;
+}
+// Bit i of a 16-bit mask moved to bit 2i.
+__device__ __forceinline__ uint32_t spread16(uint32_t x) {
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  return (x | (x << 1)) & 0x55555555u;
+}
+
+struct KmerTally {
+  uint32_t base[4] = {0, 0, 0, 0};
+  uint32_t kmers = 0;
+};
+
+// 4-mers ending in the 16 bytes [16j, 16j+16); prev is the dword at 16j-4
+// and `first` the buffer position 16j-4, [beg, end) the contig.
+__device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_t prev, uint4 cur,
+                                            int64_t first, int64_t beg, int64_t end,
+                                            KmerTally& tl) {
+  uint32_t vm = base_bits4(prev) | (base_bits4(cur.x) << 4) | (base_bits4(cur.y) << 8) |
+                (base_bits4(cur.z) << 12) | (base_bits4(cur.w) << 16);
+  const int64_t lo = beg - first, hi = end - first;
+  const uint32_t lo_c = (uint32_t)(lo < 0 ? 0 : (lo > 20 ? 20 : lo));
+  const uint32_t hi_c = (uint32_t)(hi < 0 ? 0 : (hi > 20 ? 20 : hi));
+  vm &= ((1u << hi_c) - 1u) & ~((1u << lo_c) - 1u);
+  const uint64_t c40 = (uint64_t)codes8(prev) | ((uint64_t)codes8(cur.x) << 8) |
+                       ((uint64_t)codes8(cur.y) << 16) | ((uint64_t)codes8(cur.z) << 24) |
+                       ((uint64_t)codes8(cur.w) << 32);
+  // bit i: the 4-mer ending at chunk byte i (buffer bytes 16j+i-3 .. 16j+i) is valid
+  const uint32_t km = ((vm & (vm >> 1) & (vm >> 2) & (vm >> 3)) >> 1) & 0xFFFFu;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if ((km >> i) & 1u) {
+      const uint32_t win = (uint32_t)(c40 >> (2 * (i + 1))) & 0xFFu;
+      const uint32_t fwd = ((win & 0x03u) << 6) | ((win & 0x0Cu) << 2) | ((win & 0x30u) >> 2) |
+                           ((win & 0xC0u) >> 6);
+      const uint32_t rev = ~win & 0xFFu;
+      atomicAdd(&hist[fwd <= rev ? fwd : rev], 1u);
+    }
+  }
+  tl.kmers += __popc(km);
+  const uint32_t c32 = (uint32_t)(c40 >> 8);  // codes of the 16 chunk bytes
+  const uint32_t valid = spread16((vm >> 4) & 0xFFFFu);
+#pragma unroll
+  for (uint32_t b = 0; b < 4; ++b) {
+    const uint32_t eq = ~(c32 ^ (0x55555555u * b));
+    tl.base[b] += __popc(eq & (eq >> 1) & valid);
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 template <int CPB>
 __global__ __launch_bounds__(256) void sketch_kernel(
     const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
-    uint32_t W, const double* __restrict__ T, const uint8_t* __restrict__ cls,
-    const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx, uint32_t mpad,
-    uint32_t* __restrict__ planes) {
-  __shared__ uint32_t hist[256];
-  __shared__ uint32_t bases[4];
-  __shared__ uint32_t total;
+    uint32_t W, const double* __restrict__ sT, const uint8_t* __restrict__ sX,
+    const uint8_t* __restrict__ cls, const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx,
+    uint32_t mpad, uint32_t* __restrict__ planes) {
+  static_assert(CPB == kSketchWaves, "one contig per wave");
+  __shared__ uint32_t hist[CPB][256];
+  __shared__ uint32_t bases[CPB][4];
+  __shared__ uint32_t total[CPB];
   __shared__ double winv[CPB][kElemCount];
+  __shared__ unsigned long long wmin_bits[CPB];  // positive doubles order as integers
   const uint32_t tid = threadIdx.x;
+  const uint32_t wv = tid >> 6, ln = tid & 63u;
   const uint32_t c0 = blockIdx.x * CPB;
 
-  for (int q = 0; q < CPB; ++q) {
-    const uint32_t c = c0 + q;
-    for (uint32_t v = tid; v < 256; v += 256) hist[v] = 0;
-    if (tid < 4) bases[tid] = 0;
-    if (tid == 0) total = 0;
-    __syncthreads();
+  for (uint32_t v = tid; v < CPB * 256; v += 256) (&hist[0][0])[v] = 0;
+  if (tid < CPB) wmin_bits[tid] = 0x7FF0000000000000ull;  // +inf: no element present
+  __syncthreads();
+  {
+    const uint32_t c = c0 + wv;
+    KmerTally tl;
     if (c < n) {
-      const uint64_t beg = offs[c];
-      const int64_t len = (int64_t)(offs[c + 1] - beg);
-      const uint8_t* s = seqs + beg;
-      uint32_t lb0 = 0, lb1 = 0, lb2 = 0, lb3 = 0, kmers = 0;
-      for (int64_t p0 = (int64_t)tid * 4; p0 < len; p0 += 1024) {
-        uint32_t e[7];
-#pragma unroll
-        for (int t = 0; t < 7; ++t) {
-          const int64_t pos = p0 - 3 + t;
-          e[t] = (pos >= 0 && pos < len) ? base_code(s[pos]) : 255u;
+      const int64_t beg = (int64_t)offs[c], end = (int64_t)offs[c + 1];
+      if (end > beg) {
+        const int64_t j1 = (end - 1) >> 4;
+        int64_t j = (beg >> 4) + ln;
+        uint4 cur = make_uint4(0, 0, 0, 0);
+        uint32_t prev = 0;
+        if (j <= j1) {
+          cur = *reinterpret_cast<const uint4*>(seqs + 16 * j);
+          prev = j > 0 ? *reinterpret_cast<const uint32_t*>(seqs + 16 * j - 4) : 0u;
         }
-#pragma unroll
-        for (int q2 = 0; q2 < 4; ++q2) {
-          const uint32_t ep = e[3 + q2];
-          if (p0 + q2 >= len || ep == 255u) continue;
-          lb0 += ep == 0u;
-          lb1 += ep == 1u;
-          lb2 += ep == 2u;
-          lb3 += ep == 3u;
-          if (e[q2] != 255u && e[q2 + 1] != 255u && e[q2 + 2] != 255u) {
-            const uint32_t fwd = (e[q2] << 6) | (e[q2 + 1] << 4) | (e[q2 + 2] << 2) | ep;
-            const uint32_t rev =
-                ((3u - ep) << 6) | ((3u - e[q2 + 2]) << 4) | ((3u - e[q2 + 1]) << 2) | (3u - e[q2]);
-            atomicAdd(&hist[fwd <= rev ? fwd : rev], 1u);
-            ++kmers;
+        while (j <= j1) {
+          const int64_t jn = j + 64;
+          uint4 nxt = make_uint4(0, 0, 0, 0);
+          uint32_t nprev = 0;
+          if (jn <= j1) {  // next chunk in flight while this one is counted
+            nxt = *reinterpret_cast<const uint4*>(seqs + 16 * jn);
+            nprev = *reinterpret_cast<const uint32_t*>(seqs + 16 * jn - 4);
           }
+          chunk_kmers(hist[wv], prev, cur, 16 * j - 4, beg, end, tl);
+          cur = nxt;
+          prev = nprev;
+          j = jn;
         }
       }
-      if (lb0) atomicAdd(&bases[0], lb0);
-      if (lb1) atomicAdd(&bases[1], lb1);
-      if (lb2) atomicAdd(&bases[2], lb2);
-      if (lb3) atomicAdd(&bases[3], lb3);
-      if (kmers) atomicAdd(&total, kmers);
     }
-    __syncthreads();
-    if (tid < kElemCount) {
-      double wi = 0.0;
-      const uint32_t v = codes[tid];
-      const uint32_t cnt = hist[v];
-      if (c < n && cnt != 0 && total != 0) {
-        const uint64_t vb = (uint64_t)bases[0] + bases[1] + bases[2] + bases[3];
-        double p[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) p[b] = vb > 0 ? (double)bases[b] / (double)vb : 0.25;
-        const double inv = 1.0 / (double)total;
-        double pf = 1.0, pr = 1.0;
-        uint32_t code = v;
+    for (int b = 0; b < 4; ++b) tl.base[b] = wave_sum(tl.base[b]);
+    tl.kmers = wave_sum(tl.kmers);
+    if (ln == 0) {
 #pragma unroll
-        for (int pos = 0; pos < 4; ++pos) {
-          const uint32_t b = code & 3u;
-          pf *= p[b];
-          pr *= p[3u - b];
-          code >>= 2;
-        }
-        const double obs = (double)cnt * inv;
-        const double ex = pf + pr;
-        double w = 0.0;
-        if (ex > 1e-14) {
-          w = obs / ex;
-          if (w > 20.0) w = 20.0;
-        }
-        if (w > 0.0) wi = 1.0 / w;
-      }
-      winv[q][tid] = wi;
+      for (int b = 0; b < 4; ++b) bases[wv][b] = tl.base[b];
+      total[wv] = tl.kmers;
     }
-    __syncthreads();
   }
+  __syncthreads();
+  for (uint32_t e = tid; e < CPB * kElemCount; e += 256) {
+    const uint32_t q = e / kElemCount, x = e % kElemCount;
+    const uint32_t c = c0 + q;
+    double wi = 0.0;
+    const uint32_t cnt = hist[q][codes[x]];
+    const uint32_t tot = total[q];
+    if (c < n && cnt != 0 && tot != 0) {
+      const uint64_t vb = (uint64_t)bases[q][0] + bases[q][1] + bases[q][2] + bases[q][3];
+      double p[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) p[b] = vb > 0 ? (double)bases[q][b] / (double)vb : 0.25;
+      const double inv = 1.0 / (double)tot;
+      double pf = 1.0, pr = 1.0;
+      uint32_t code = codes[x];
+#pragma unroll
+      for (int pos = 0; pos < 4; ++pos) {
+        const uint32_t b = code & 3u;
+        pf *= p[b];
+        pr *= p[3u - b];
+        code >>= 2;
+      }
+      const double obs = (double)cnt * inv;
+      const double ex = pf + pr;
+      double w = 0.0;
+      if (ex > 1e-14) {
+        w = obs / ex;
+        if (w > 20.0) w = 20.0;
+      }
+      if (w > 0.0) wi = 1.0 / w;
+    }
+    winv[q][x] = wi;
+    if (wi > 0.0) atomicMin(&wmin_bits[q], (unsigned long long)__double_as_longlong(wi));
+  }
+  __syncthreads();
+  double wmin[CPB];
+#pragma unroll
+  for (int q = 0; q < CPB; ++q) wmin[q] = __longlong_as_double((long long)wmin_bits[q]);
 
   const uint32_t kspan = ((32u * W + 255u) / 256u) * 256u;
   for (uint32_t k = tid; k < kspan; k += 256) {
@@ -180,19 +284,33 @@ __global__ __launch_bounds__(256) void sketch_kernel(
       bx[q] = -1;
     }
     if (k < m) {
-      for (int x = 0; x < kElemCount; ++x) {
-        const double t = T[(size_t)x * m + k];
+      double t = sT[k];
+      uint32_t x = sX[k];
+      for (int r = 0; r < kElemCount; ++r) {
+        bool done = true;
+#pragma unroll
+        for (int q = 0; q < CPB; ++q)  // wmin = +inf: contig without elements
+          done = done && (wmin[q] == __builtin_huge_val() || wmin[q] * t > best[q]);
+        if (done) break;
+        double tn = 0.0;
+        uint32_t xn = 0;
+        if (r + 1 < kElemCount) {  // next rank in flight while this one is applied
+          tn = sT[(size_t)(r + 1) * m + k];
+          xn = sX[(size_t)(r + 1) * m + k];
+        }
 #pragma unroll
         for (int q = 0; q < CPB; ++q) {
           const double wv = winv[q][x];
           if (wv != 0.0) {
             const double hv = wv * t;
-            if (hv < best[q]) {
+            if (hv < best[q] || (hv == best[q] && (int)x < bx[q])) {
               best[q] = hv;
-              bx[q] = x;
+              bx[q] = (int)x;
             }
           }
         }
+        t = tn;
+        x = xn;
       }
     }
     const uint32_t w0 = k / 32u;  // lanes 0..31 -> w0, lanes 32..63 -> w0+1
@@ -206,7 +324,6 @@ __global__ __launch_bounds__(256) void sketch_kernel(
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         const uint64_t bits = __ballot((code >> p) & 1u);
-        const uint32_t ln = lane_id();
         if (c < n && (ln == 0 || ln == 32) && w0 < W)
           planes[(size_t)c * 8u * W + 8u * w0 + p] = ln == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
       }
@@ -1032,13 +1149,14 @@ __global__ __launch_bounds__(256) void scan_final_kernel(const uint32_t* __restr
 // Launchers.
 // ---------------------------------------------------------------------------
 hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
-                         uint32_t W, const double* T, const uint8_t* cls, const uint8_t* codes,
-                         uint8_t* wx, uint32_t mpad, uint32_t* planes, hipStream_t st) {
+                         uint32_t W, const double* sT, const uint8_t* sX, const uint8_t* cls,
+                         const uint8_t* codes, uint8_t* wx, uint32_t mpad, uint32_t* planes,
+                         hipStream_t st) {
   constexpr int CPB = 4;
   const uint32_t blocks = (n + CPB - 1) / CPB;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(sketch_kernel<CPB>, dim3(blocks), dim3(256), 0, st, seqs, offs, n, m, W, T,
-                     cls, codes, wx, mpad, planes);
+  hipLaunchKernelGGL(sketch_kernel<CPB>, dim3(blocks), dim3(256), 0, st, seqs, offs, n, m, W, sT,
+                     sX, cls, codes, wx, mpad, planes);
   return hipGetLastError();
 }
 
