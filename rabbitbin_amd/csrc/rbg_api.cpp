@@ -130,7 +130,7 @@ struct rbg_asm {
   uint64_t seq_bytes = 0;
   DevBuf seqs, offs;
   // depth features (device) and their host copies
-  DevBuf unit_pad, unit, raw, colnorm, anynz, q8;
+  DevBuf unit_pad, raw_pad, unit, raw, colnorm, anynz, q8;
   uint32_t NC = 1;
   bool has_anynz = false;
   std::vector<uint8_t> anynz_h;
@@ -169,10 +169,15 @@ rbg_status asm_upload(rbg_ctx* ctx, rbg_asm* a, const char* seqs, const uint64_t
                    a->anynz_h.data());
     a->has_anynz = true;
     a->NC = (S + 15) / 16;
-    std::vector<float> pad((size_t)n * 16 * a->NC, 0.0f);
-    for (uint32_t r = 0; r < n; ++r)
+    std::vector<float> pad((size_t)n * 16 * a->NC, 0.0f), rpad(pad.size(), 0.0f);
+    for (uint32_t r = 0; r < n; ++r) {
       std::memcpy(&pad[(size_t)r * 16 * a->NC], &unit[(size_t)r * S], S * sizeof(float));
+      std::memcpy(&rpad[(size_t)r * 16 * a->NC], &raw[(size_t)r * S], S * sizeof(float));
+    }
     RBG_TRY_HIP(ctx, a->unit_pad.ensure(pad.size() * sizeof(float)));
+    RBG_TRY_HIP(ctx, a->raw_pad.ensure(rpad.size() * sizeof(float)));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(a->raw_pad.p, rpad.data(), rpad.size() * sizeof(float),
+                                    hipMemcpyHostToDevice, ctx->stream));
     RBG_TRY_HIP(ctx, a->unit.ensure(unit.size() * sizeof(float)));
     RBG_TRY_HIP(ctx, a->raw.ensure(raw.size() * sizeof(float)));
     RBG_TRY_HIP(ctx, a->colnorm.ensure(S * sizeof(double)));
@@ -411,12 +416,15 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   RBG_TRY_HIP(ctx, hipEventRecord(ev[4], st));
   if (S > 0) {
     RBG_TRY_HIP(ctx, a->escore.ensure(std::max<uint64_t>(ne, 1) * 4));
-    RBG_TRY_HIP(ctx, launch_edge_weights((const uint32_t*)a->efrom.p, (const uint32_t*)a->eto.p,
-                                         (const float*)a->escomp.p, ne, (const float*)a->unit.p,
-                                         (const float*)a->raw.p,
-                                         S > 1 ? (const double*)a->colnorm.p : nullptr, S, w_comp,
-                                         P->min_edge_weight, P->neg_depth_thr,
-                                         (float*)a->escore.p, nullptr, st));
+    const bool padded = S > 1 && S <= 64;  // unit_pad / raw_pad hold the rows
+    RBG_TRY_HIP(ctx, launch_edge_weights(
+                         (const uint32_t*)a->efrom.p, (const uint32_t*)a->eto.p,
+                         (const float*)a->escomp.p, ne,
+                         (const float*)(padded ? a->unit_pad.p : a->unit.p),
+                         (const float*)(padded ? a->raw_pad.p : a->raw.p),
+                         S > 1 ? (const double*)a->colnorm.p : nullptr, S, w_comp,
+                         P->min_edge_weight, P->neg_depth_thr, (float*)a->escore.p, nullptr, st,
+                         padded));
   }
   RBG_TRY_HIP(ctx, hipEventRecord(ev[5], st));
   out->n_edges = ne;

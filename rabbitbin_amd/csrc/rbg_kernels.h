@@ -48,7 +48,9 @@ hipError_t launch_emit(const uint32_t* topk_id, const uint16_t* topk_cnt, const 
 hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const float* scomp,
                                uint64_t ne, const float* unit, const float* raw,
                                const double* colnorm, uint32_t S, double w_comp, double min_edge,
-                               double neg_thr, float* score, double* score64, hipStream_t st);
+                               double neg_thr, float* score, double* score64, hipStream_t st,
+                               bool padded = false);
+// padded: unit and raw rows are zero-padded to 16*ceil(S/16) floats (S <= 64).
 hipError_t launch_cnt_to_sv(const uint16_t* c, uint64_t ne, const float* svf, float* out,
                             hipStream_t st);
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n, void* tmp,

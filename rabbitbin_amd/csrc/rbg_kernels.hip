@@ -1070,6 +1070,74 @@ __global__ __launch_bounds__(256) void edge_weight_kernel(
   if (score64) score64[e] = w;
 }
 
+// The same weight for 1 < S <= 16*NC with unit and raw rows zero-padded to
+// 16*NC floats (128-byte aligned rows read as float4).  Per edge the FP64
+// sums run in the same order as above, so the results are identical.
+template <int NC>
+__global__ __launch_bounds__(256) void edge_weight_pad_kernel(
+    const uint32_t* __restrict__ efrom, const uint32_t* __restrict__ eto,
+    const float* __restrict__ scomp, uint64_t ne, const float* __restrict__ unitp,
+    const float* __restrict__ rawp, const double* __restrict__ colnorm, uint32_t S,
+    double w_comp, double min_edge, double neg_thr, float* __restrict__ score,
+    double* __restrict__ score64) {
+  constexpr int V = 4 * NC;  // float4 per row
+  const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  const uint32_t i = efrom[e], j = eto[e];
+  const float4* ua = reinterpret_cast<const float4*>(unitp + (size_t)i * 16 * NC);
+  const float4* ub = reinterpret_cast<const float4*>(unitp + (size_t)j * 16 * NC);
+  float a[4 * V], b[4 * V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const float4 x = ua[v], y = ub[v];
+    a[4 * v] = x.x, a[4 * v + 1] = x.y, a[4 * v + 2] = x.z, a[4 * v + 3] = x.w;
+    b[4 * v] = y.x, b[4 * v + 1] = y.y, b[4 * v + 2] = y.z, b[4 * v + 3] = y.w;
+  }
+  double corr = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4 * V; ++k)
+    if ((uint32_t)k < S) corr += (double)a[k] * (double)b[k];
+  double w = 0.0;
+  const bool finite = corr == corr && corr != __builtin_huge_val() && corr != -__builtin_huge_val();
+  if (finite && !(neg_thr > -0.99 && corr < neg_thr)) {
+    const double cp = corr < 0.0 ? 0.0 : corr;
+    const float4* ra = reinterpret_cast<const float4*>(rawp + (size_t)i * 16 * NC);
+    const float4* rb = reinterpret_cast<const float4*>(rawp + (size_t)j * 16 * NC);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const float4 x = ra[v], y = rb[v];
+      a[4 * v] = x.x, a[4 * v + 1] = x.y, a[4 * v + 2] = x.z, a[4 * v + 3] = x.w;
+      b[4 * v] = y.x, b[4 * v + 1] = y.y, b[4 * v + 2] = y.z, b[4 * v + 3] = y.w;
+    }
+    double mn = 0.0, mx = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4 * V; ++k) {
+      if ((uint32_t)k < S) {
+        double x = a[k], y = b[k];
+        if (colnorm) {
+          x *= colnorm[k];
+          y *= colnorm[k];
+        }
+        if (x < y) {
+          mn += x;
+          mx += y;
+        } else {
+          mn += y;
+          mx += x;
+        }
+      }
+    }
+    double wj = mx <= 0.0 ? 0.0 : mn / mx;
+    if (!(wj == wj)) wj = 0.0;
+    wj = wj < 0.0 ? 0.0 : (wj > 1.0 ? 1.0 : wj);
+    const double d = cp < wj ? cp : wj;
+    w = w_comp * (double)scomp[e] + (1.0 - w_comp) * d;
+    if (!(w == w) || w < min_edge) w = 0.0;
+  }
+  score[e] = (float)w;
+  if (score64) score64[e] = w;
+}
+
 __global__ void u16_to_sv_kernel(const uint16_t* __restrict__ c, uint64_t ne,
                                  const float* __restrict__ svf, float* __restrict__ out) {
   const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1316,10 +1384,26 @@ hipError_t launch_emit(const uint32_t* topk_id, const uint16_t* topk_cnt, const 
 hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const float* scomp,
                                uint64_t ne, const float* unit, const float* raw,
                                const double* colnorm, uint32_t S, double w_comp, double min_edge,
-                               double neg_thr, float* score, double* score64, hipStream_t st) {
+                               double neg_thr, float* score, double* score64, hipStream_t st,
+                               bool padded) {
   if (ne == 0) return hipSuccess;
-  hipLaunchKernelGGL(edge_weight_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, efrom,
-                     eto, scomp, ne, unit, raw, colnorm, S, w_comp, min_edge, neg_thr, score, score64);
+  const dim3 grid((unsigned)((ne + 255) / 256));
+  const uint32_t NC = (S + 15) / 16;
+  if (padded && S > 1 && NC <= 4) {
+#define RBG_EW(NCV)                                                                               \
+  hipLaunchKernelGGL(edge_weight_pad_kernel<NCV>, grid, dim3(256), 0, st, efrom, eto, scomp, ne, \
+                     unit, raw, colnorm, S, w_comp, min_edge, neg_thr, score, score64)
+    switch (NC) {
+      case 1: RBG_EW(1); break;
+      case 2: RBG_EW(2); break;
+      case 3: RBG_EW(3); break;
+      default: RBG_EW(4); break;
+    }
+#undef RBG_EW
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(edge_weight_kernel, grid, dim3(256), 0, st, efrom, eto, scomp, ne, unit, raw,
+                     colnorm, S, w_comp, min_edge, neg_thr, score, score64);
   return hipGetLastError();
 }
 
