@@ -136,7 +136,7 @@ struct rbg_asm {
   std::vector<uint8_t> anynz_h;
   GateQuant gq;  // bound statistics of q8 (the int8 rows live on the device)
   // pipeline intermediates
-  DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, cand, surv, nsurv, scnt, deg, rowmax, off, cursor,
+  DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, cand, nsurv, scnt, deg, rowmax, off, cursor,
       list, topk_id, topk_cnt, topk_len, row_edges, row_off, efrom, eto, ecnt, escomp, escore,
       scan_tmp, tiles;
   uint32_t ntiles = 0;
@@ -297,6 +297,7 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   const uint8_t* anynz = (S > 0 && a->has_anynz) ? (const uint8_t*)a->anynz.p : nullptr;
 
   RBG_TRY_HIP(ctx, hipEventRecord(ev[2], st));
+  uint64_t off_total = 0;  // fused regime: 2 x gated pairs
   if (regime == 1) {
     RBG_TRY_HIP(ctx, a->mat.ensure((size_t)n * n * sizeof(uint16_t)));
     RBG_TRY_HIP(ctx, launch_dense_counts((const uint32_t*)a->planes.p, W, n, (uint16_t*)a->mat.p, st));
@@ -323,7 +324,7 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
     const uint64_t total_pairs = (uint64_t)n * (n - 1) / 2;
     uint64_t cap = std::min<uint64_t>(total_pairs, std::max<uint64_t>(1ull << 26, 2048ull * n));
     RBG_TRY_HIP(ctx, a->nsurv.ensure(2 * sizeof(unsigned long long)));
-    unsigned long long* counters = (unsigned long long*)a->nsurv.p;  // [0] candidates, [1] survivors
+    unsigned long long* counters = (unsigned long long*)a->nsurv.p;  // [0] candidates
     uint64_t nc = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
       RBG_TRY_HIP(ctx, a->cand.ensure(std::max<uint64_t>(cap, 1) * sizeof(uint64_t)));
@@ -340,23 +341,15 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
       if (nc <= cap) break;
       cap = nc;  // rerun with room for every candidate
     }
-    // exact FP32 gate of the candidates (survivors <= candidates: no overflow)
-    RBG_TRY_HIP(ctx, a->surv.ensure(std::max<uint64_t>(nc, 1) * sizeof(uint64_t)));
-    RBG_TRY_HIP(ctx, launch_exact_gate((const uint64_t*)a->cand.p, nc, (const float*)a->unit_pad.p,
-                                       NC, anynz, thr, acc1_end, (uint64_t*)a->surv.p,
-                                       counters + 1, nc, st));
-    unsigned long long got_s = 0;
-    RBG_TRY_HIP(ctx, hipMemcpyAsync(&got_s, counters + 1, sizeof(got_s), hipMemcpyDeviceToHost, st));
-    RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
-    const uint64_t ns = got_s;
-    out->gated_pairs = ns;
+    // exact FP32 gate and register compare of the candidates, in place
     RBG_TRY_HIP(ctx, a->deg.ensure((size_t)n * sizeof(uint32_t)));
     RBG_TRY_HIP(ctx, a->cursor.ensure((size_t)n * sizeof(uint32_t)));
     RBG_TRY_HIP(ctx, a->off.ensure((size_t)(n + 1) * sizeof(uint64_t)));
     RBG_TRY_HIP(ctx, hipMemsetAsync(a->deg.p, 0, (size_t)n * 4, st));
     RBG_TRY_HIP(ctx, hipMemsetAsync(a->cursor.p, 0, (size_t)n * 4, st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[7], st));
-    RBG_TRY_HIP(ctx, launch_pair_compare((const uint32_t*)a->planes.p, W, (uint64_t*)a->surv.p, ns,
+    RBG_TRY_HIP(ctx, launch_gate_compare((uint64_t*)a->cand.p, nc, (const float*)a->unit_pad.p, NC,
+                                         anynz, thr, acc1_end, (const uint32_t*)a->planes.p, W,
                                          (uint32_t*)a->deg.p, st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[8], st));
     size_t tmp_bytes = 0;
@@ -364,9 +357,12 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
     RBG_TRY_HIP(ctx, a->scan_tmp.ensure(tmp_bytes));
     RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)a->deg.p, (uint64_t*)a->off.p, n,
                                                a->scan_tmp.p, tmp_bytes, st));
-    RBG_TRY_HIP(ctx, a->list.ensure(std::max<uint64_t>(2 * ns, 1) * sizeof(uint64_t)));
-    RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)a->surv.p, ns, (const uint64_t*)a->off.p,
+    // off[n] = 2 x survivors <= 2 x candidates
+    RBG_TRY_HIP(ctx, a->list.ensure(std::max<uint64_t>(2 * nc, 1) * sizeof(uint64_t)));
+    RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)a->cand.p, nc, (const uint64_t*)a->off.p,
                                     (uint32_t*)a->cursor.p, (uint64_t*)a->list.p, st));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(&off_total, (const uint64_t*)a->off.p + n, sizeof(uint64_t),
+                                    hipMemcpyDeviceToHost, st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[3], st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[9], st));
     RBG_TRY_HIP(ctx, launch_select_list((const uint64_t*)a->list.p, (const uint64_t*)a->off.p, n,
@@ -379,6 +375,7 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   std::vector<uint32_t> rowmax(n);
   RBG_TRY_HIP(ctx, hipMemcpyAsync(rowmax.data(), a->rowmax.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
   RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if (regime == 2) out->gated_pairs = off_total / 2;
   const double sim_cutoff = regime == 1 ? calibrate_small(rowmax, svd, P->coverage, P->rand_seed)
                                         : calibrate_fused(rowmax, svf, P->coverage, P->rand_seed);
   out->sim_cutoff = sim_cutoff;

@@ -18,19 +18,16 @@ hipError_t launch_win16_counts(const uint16_t* win16, uint32_t m, const uint32_t
                                const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st);
 hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, uint32_t n, uint16_t* out,
                                hipStream_t st);
-hipError_t launch_gate(const float* unit, uint32_t NC, const uint8_t* anynz, uint32_t n, float thr,
-                       uint32_t acc1_end, uint64_t* surv, unsigned long long* nsurv, uint64_t cap,
-                       hipStream_t st);
 // Work items of the fused pair pass: (bi << 16 | bj0, tile count) per item.
 void gate_work_items(uint32_t n, std::vector<uint32_t>& items);
 hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, uint32_t n, int32_t qcut,
                             const uint32_t* items, uint32_t nitems, uint64_t* cand,
                             unsigned long long* ncand, uint64_t cap, hipStream_t st);
-hipError_t launch_exact_gate(const uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,
-                             const uint8_t* anynz, float thr, uint32_t acc1_end, uint64_t* surv,
-                             unsigned long long* nsurv, uint64_t cap, hipStream_t st);
-hipError_t launch_pair_compare(const uint32_t* planes, uint32_t W, uint64_t* surv, uint64_t ns,
-                               uint32_t* deg, hipStream_t st);
+// Exact gate + register compare of the candidates, in place (rejected ->
+// all ones, survivors -> count in the low 16 bits; deg counts survivors per row).
+hipError_t launch_gate_compare(uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,
+                               const uint8_t* anynz, float thr, uint32_t acc1_end,
+                               const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st);
 hipError_t launch_scatter(const uint64_t* surv, uint64_t ns, const uint64_t* off, uint32_t* cursor,
                           uint64_t* list, hipStream_t st);
 hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32_t n,

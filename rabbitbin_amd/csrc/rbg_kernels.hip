@@ -381,132 +381,13 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
 }
 
 // ---------------------------------------------------------------------------
-// Fused regime, abundance gate (src/impl/rb_graph.cpp:1108-1116): keep pair
-// (i < j) iff is_nz(i, j) and unit_dot_f(u_i, u_j) >= thr, where unit_dot_f
-// is evaluated in the AVX-512 order of the reference build
-// (src/rabbitbin.cpp:754-776): 16 lanes; chunk c (16 elements, zero padded)
-// accumulates into acc1 when c is odd and lies inside a full 32-wide chunk
-// (c < acc1_end), else into acc0; each lane's chain is one FMA per chunk in
-// chunk order; then acc0+acc1 and GCC's _mm512_reduce_add_ps tree.
-// Padding lanes add fma(0,0,acc) = acc, matching the masked tail.
-// Survivors (i, j) are appended to `surv`.
-// ---------------------------------------------------------------------------
-template <int NC>
-__global__ __launch_bounds__(256) void gate_kernel(const float* __restrict__ unit,
-                                                   const uint8_t* __restrict__ anynz, uint32_t n,
-                                                   float thr, uint32_t acc1_end,
-                                                   uint32_t rows_per_block,
-                                                   uint64_t* __restrict__ surv,
-                                                   unsigned long long* __restrict__ nsurv,
-                                                   uint64_t cap) {
-  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t r0 = blockIdx.y * rows_per_block;
-  const uint32_t jmax = min(n, blockIdx.x * 256 + 256);
-  if (r0 + 1 >= jmax) return;  // block lies on or below the diagonal
-  const uint32_t r1 = min(jmax - 1, r0 + rows_per_block);
-  const uint32_t jj = j < n ? j : n - 1;
-  float b[16 * NC];
-#pragma unroll
-  for (int t = 0; t < 16 * NC; ++t) b[t] = unit[(size_t)jj * 16 * NC + t];
-  const bool nz_j = anynz ? anynz[jj] != 0 : true;
-  for (uint32_t i = r0; i < r1; ++i) {
-    const float* __restrict__ a = unit + (size_t)i * 16 * NC;
-    float acc0[16], acc1[16];
-#pragma unroll
-    for (int l = 0; l < 16; ++l) {
-      acc0[l] = 0.0f;
-      acc1[l] = 0.0f;
-    }
-    bool used1 = false;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      if ((c & 1) && (uint32_t)c < acc1_end) {
-        used1 = true;
-#pragma unroll
-        for (int l = 0; l < 16; ++l) acc1[l] = __builtin_fmaf(a[16 * c + l], b[16 * c + l], acc1[l]);
-      } else {
-#pragma unroll
-        for (int l = 0; l < 16; ++l) acc0[l] = __builtin_fmaf(a[16 * c + l], b[16 * c + l], acc0[l]);
-      }
-    }
-    if (used1) {
-#pragma unroll
-      for (int l = 0; l < 16; ++l) acc0[l] = acc0[l] + acc1[l];
-    }
-    float t8[8], t4[4];
-#pragma unroll
-    for (int l = 0; l < 8; ++l) t8[l] = acc0[l + 8] + acc0[l];
-#pragma unroll
-    for (int l = 0; l < 4; ++l) t4[l] = t8[l + 4] + t8[l];
-    const float dot = (t4[0] + t4[2]) + (t4[1] + t4[3]);
-    const bool nz = nz_j || (anynz ? anynz[i] != 0 : true);
-    const bool pass = j < n && i < j && nz && !(dot < thr);
-    const uint64_t mask = __ballot(pass);
-    if (mask) {
-      unsigned long long base = 0;
-      if (lane_id() == 0) base = atomicAdd(nsurv, (unsigned long long)__popcll(mask));
-      base = __shfl(base, 0);
-      if (pass) {
-        const uint64_t slot = base + lanes_below(mask);
-        if (slot < cap) surv[slot] = ((uint64_t)i << 32) | j;
-      }
-    }
-  }
-}
-
-// Exact FP32 gate value of one pair, both rows in VGPRs (same order as
-// gate_kernel; rows are 16*NC floats, zero padded).
-template <int NC>
-__device__ __forceinline__ float exact_gate_dot(const float* __restrict__ ua,
-                                                const float* __restrict__ ub, uint32_t acc1_end) {
-  float acc0[16], acc1[16];
-#pragma unroll
-  for (int l = 0; l < 16; ++l) {
-    acc0[l] = 0.0f;
-    acc1[l] = 0.0f;
-  }
-  bool used1 = false;
-  // one 16-wide chunk at a time keeps only 32 operand registers live
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    float a[16], b[16];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const float4 x = reinterpret_cast<const float4*>(ua)[4 * c + t];
-      const float4 y = reinterpret_cast<const float4*>(ub)[4 * c + t];
-      a[4 * t] = x.x; a[4 * t + 1] = x.y; a[4 * t + 2] = x.z; a[4 * t + 3] = x.w;
-      b[4 * t] = y.x; b[4 * t + 1] = y.y; b[4 * t + 2] = y.z; b[4 * t + 3] = y.w;
-    }
-    if ((c & 1) && (uint32_t)c < acc1_end) {
-      used1 = true;
-#pragma unroll
-      for (int l = 0; l < 16; ++l) acc1[l] = __builtin_fmaf(a[l], b[l], acc1[l]);
-    } else {
-#pragma unroll
-      for (int l = 0; l < 16; ++l) acc0[l] = __builtin_fmaf(a[l], b[l], acc0[l]);
-    }
-  }
-  if (used1) {
-#pragma unroll
-    for (int l = 0; l < 16; ++l) acc0[l] = acc0[l] + acc1[l];
-  }
-  float t8[8], t4[4];
-#pragma unroll
-  for (int l = 0; l < 8; ++l) t8[l] = acc0[l + 8] + acc0[l];
-#pragma unroll
-  for (int l = 0; l < 4; ++l) t4[l] = t8[l + 4] + t8[l];
-  return (t4[0] + t4[2]) + (t4[1] + t4[3]);
-}
-
-// ---------------------------------------------------------------------------
-// Fused regime, abundance gate in three kernels:
-//  1. gate_mfma_kernel: int8 MFMA prefilter over the upper triangle, emits
-//     candidates (i < j, qdot >= qcut);
-//  2. exact_gate_kernel: is_nz and the exact FP32 gate (exact_gate_dot) per
-//     candidate, compacting the survivors;
-//  3. pair_compare_kernel: register compare of the survivors.
+// Fused regime, pair pass in two kernels:
+//  1. gate_mfma_kernel: int8 MFMA prefilter of the abundance gate over the
+//     upper triangle, emits candidates (i < j, qdot >= qcut);
+//  2. gate_compare_kernel: is_nz and the exact FP32 gate per candidate, and
+//     the register compare of the survivors, in place.
 // qcut is a rigorous lower bound (gate_qcut), so the survivors are exactly
-// the pairs the reference's gate keeps.
+// the pairs the reference's gate keeps (src/impl/rb_graph.cpp:1108-1116).
 //
 // gate_mfma_kernel: work item = a 256-row stripe bi and up to 16 consecutive
 // 256-column tiles bj0.. (bi <= bj).  Wave w owns rows bi*256+64w..+64 as four
@@ -651,90 +532,123 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
   if (bn) flush();
 }
 
-// Exact gate of the int8 candidates: is_nz(i, j) and !(unit_dot_f < thr);
-// survivors are compacted per wave (one global atomic per wave).
-template <int NC>
-__global__ __launch_bounds__(256) void exact_gate_kernel(const uint64_t* __restrict__ cand,
-                                                         uint64_t nc, const float* __restrict__ unit,
-                                                         const uint8_t* __restrict__ anynz, float thr,
-                                                         uint32_t acc1_end,
-                                                         uint64_t* __restrict__ surv,
-                                                         unsigned long long* __restrict__ nsurv,
-                                                         uint64_t cap) {
-  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  bool pass = false;
-  uint64_t e = 0;
-  if (t < nc) {
-    e = cand[t];
-    const uint32_t i = (uint32_t)(e >> 40), j = (uint32_t)(e >> 16) & 0xFFFFFFu;
-    const bool nz = !anynz || anynz[i] || anynz[j];
-    pass = nz && !(exact_gate_dot<NC>(unit + (size_t)i * 16 * NC, unit + (size_t)j * 16 * NC,
-                                      acc1_end) < thr);
-  }
-  const uint64_t mask = __ballot(pass);
-  if (mask == 0) return;
-  unsigned long long base = 0;
-  if (lane_id() == 0) base = atomicAdd(nsurv, (unsigned long long)__popcll(mask));
-  base = __shfl(base, 0);
-  if (pass) {
-    const uint64_t slot = base + lanes_below(mask);
-    if (slot < cap) surv[slot] = e;
-  }
-}
+// Gated pairs are packed as (i << 40) | (j << 16) | count; a candidate that
+// the exact gate rejects is overwritten with kDeadPair.
+constexpr uint64_t kDeadPair = ~0ull;
 
-// Gated pairs are packed as (i << 40) | (j << 16) | count.
 __device__ __forceinline__ void unpack_pair(uint64_t e, uint32_t& i, uint32_t& j, uint32_t& c) {
   i = (uint32_t)(e >> 40);
   j = (uint32_t)(e >> 16) & 0xFFFFFFu;
   c = (uint32_t)(e & 0xFFFFu);
 }
 
-// Register compare of the gated pairs (count written into the low 16 bits of
-// each packed pair) and the per-row candidate counts of both endpoints.
-// LPP lanes share one pair: lane q of the group takes word groups
-// [q*W/LPP, (q+1)*W/LPP) of both rows (8 planes x 4 bytes each, contiguous),
-// loads them with 16-byte loads in one batch, and the partial mismatch counts
-// meet in a shuffle reduction.
-template <int W>
-__global__ __launch_bounds__(256) void pair_compare_kernel(const uint32_t* __restrict__ planes,
-                                                           uint64_t* __restrict__ surv, uint64_t ns,
-                                                           uint32_t* __restrict__ deg) {
-  constexpr int LPP = W < 4 ? W : 4;  // lanes per pair
-  constexpr int WPL = W / LPP;        // word groups per lane
+// Exact gate and register compare of the int8 candidates, in place.
+//
+// Gate: keep (i < j) iff is_nz(i, j) and !(unit_dot_f(u_i, u_j) < thr), with
+// unit_dot_f in the AVX-512 order of the reference build
+// (src/rabbitbin.cpp:754-776): 16 accumulator lanes; chunk c (16 elements,
+// zero padded) accumulates into acc1 when c is odd and lies inside a full
+// 32-wide chunk (c < acc1_end), else into acc0; one FMA per chunk per lane
+// in chunk order (padding adds fma(0,0,acc) = acc, as the masked tail does);
+// then acc0+acc1 and GCC's _mm512_reduce_add_ps tree
+// (l, l+8) -> (l, l+4) -> (0,2),(1,3) -> final.
+//
+// Four GPU lanes take one candidate.  Lane q of the group holds accumulator
+// lanes 4q..4q+3 (float4 q of every 16-float chunk of both rows); the first
+// reduction level pairs lanes q and q^2, the second q and q^1, the last runs
+// in lane 0 of the group (IEEE addition is commutative, so which lane holds
+// which operand does not change a sum).  Lane q also holds word groups
+// [q*W/4, (q+1)*W/4) of both rows' planes (mismatch counts meet in the same
+// shuffles).  All loads are issued before the gate is known: nearly every
+// candidate survives.  Survivors add 1 to deg[i] and deg[j].
+template <int NC, int W>
+__global__ __launch_bounds__(256) void gate_compare_kernel(
+    uint64_t* __restrict__ cand, uint64_t nc, const float* __restrict__ unit,
+    const uint8_t* __restrict__ anynz, float thr, uint32_t acc1_end,
+    const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
+  constexpr int WPL = W >= 4 ? W / 4 : 1;  // word groups per lane
   const uint64_t gt = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  const uint64_t t = gt / LPP;
-  const uint32_t part = (uint32_t)(gt % LPP);
-  uint32_t i = 0, j = 0, c = 0;
-  uint64_t e = 0;
-  uint32_t mism = 0;
-  if (t < ns) {
-    e = surv[t];
-    unpack_pair(e, i, j, c);
-    const uint4* ra = reinterpret_cast<const uint4*>(planes + (size_t)i * 8u * W + 8u * WPL * part);
-    const uint4* rb = reinterpret_cast<const uint4*>(planes + (size_t)j * 8u * W + 8u * WPL * part);
-    uint4 a[2 * WPL], b[2 * WPL];
+  const uint64_t t = gt >> 2;
+  const uint32_t q = (uint32_t)(gt & 3u);
+  const bool live = t < nc;
+  const uint64_t e = live ? cand[t] : 0;
+  const uint32_t i = (uint32_t)(e >> 40), j = (uint32_t)(e >> 16) & 0xFFFFFFu;
+  const bool has_words = W >= 4 || q < (uint32_t)W;
+  float4 ua[NC], ub[NC];
+  uint4 pa[2 * WPL], pb[2 * WPL];
+  if (live) {
+    const float4* ra = reinterpret_cast<const float4*>(unit + (size_t)i * 16 * NC);
+    const float4* rb = reinterpret_cast<const float4*>(unit + (size_t)j * 16 * NC);
 #pragma unroll
-    for (int x = 0; x < 2 * WPL; ++x) {
-      a[x] = ra[x];
-      b[x] = rb[x];
+    for (int c = 0; c < NC; ++c) {
+      ua[c] = ra[4 * c + q];
+      ub[c] = rb[4 * c + q];
     }
+    if (has_words) {
+      const uint4* wa = reinterpret_cast<const uint4*>(planes + (size_t)i * 8u * W + 8u * WPL * q);
+      const uint4* wb = reinterpret_cast<const uint4*>(planes + (size_t)j * 8u * W + 8u * WPL * q);
+#pragma unroll
+      for (int x = 0; x < 2 * WPL; ++x) {
+        pa[x] = wa[x];
+        pb[x] = wb[x];
+      }
+    }
+  }
+  // gate value
+  float acc0[4] = {0.f, 0.f, 0.f, 0.f}, acc1[4] = {0.f, 0.f, 0.f, 0.f};
+  bool used1 = false;
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const float a4[4] = {ua[c].x, ua[c].y, ua[c].z, ua[c].w};
+      const float b4[4] = {ub[c].x, ub[c].y, ub[c].z, ub[c].w};
+      if ((c & 1) && (uint32_t)c < acc1_end) {
+        used1 = true;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) acc1[l] = __builtin_fmaf(a4[l], b4[l], acc1[l]);
+      } else {
+#pragma unroll
+        for (int l = 0; l < 4; ++l) acc0[l] = __builtin_fmaf(a4[l], b4[l], acc0[l]);
+      }
+    }
+  }
+  if (used1) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) acc0[l] = acc0[l] + acc1[l];
+  }
+  float t8[4], t4[4];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) t8[l] = __shfl_xor(acc0[l], 2) + acc0[l];  // lanes 0,1: l+8 with l
+#pragma unroll
+  for (int l = 0; l < 4; ++l) t4[l] = __shfl_xor(t8[l], 1) + t8[l];  // lane 0: l+4 with l
+  const float dot = (t4[0] + t4[2]) + (t4[1] + t4[3]);
+  const bool nz = !anynz || (live && (anynz[i] || anynz[j]));
+  bool pass = live && nz && !(dot < thr);
+  pass = __shfl(pass, (int)(lane_id() & ~3u));  // the group's lane 0 decides
+  // register compare
+  uint32_t mism = 0;
+  if (pass && has_words) {
 #pragma unroll
     for (int w = 0; w < WPL; ++w) {
-      const uint32_t av[8] = {a[2 * w].x, a[2 * w].y, a[2 * w].z, a[2 * w].w,
-                              a[2 * w + 1].x, a[2 * w + 1].y, a[2 * w + 1].z, a[2 * w + 1].w};
-      const uint32_t bs[8] = {b[2 * w].x, b[2 * w].y, b[2 * w].z, b[2 * w].w,
-                              b[2 * w + 1].x, b[2 * w + 1].y, b[2 * w + 1].z, b[2 * w + 1].w};
+      const uint32_t av[8] = {pa[2 * w].x, pa[2 * w].y, pa[2 * w].z, pa[2 * w].w,
+                              pa[2 * w + 1].x, pa[2 * w + 1].y, pa[2 * w + 1].z, pa[2 * w + 1].w};
+      const uint32_t bs[8] = {pb[2 * w].x, pb[2 * w].y, pb[2 * w].z, pb[2 * w].w,
+                              pb[2 * w + 1].x, pb[2 * w + 1].y, pb[2 * w + 1].z, pb[2 * w + 1].w};
       uint32_t bv[8];
       column_word(bs, bv);
       mism += __popc(word_mismatch(av, bv));
     }
   }
-#pragma unroll
-  for (int d = 1; d < LPP; d <<= 1) mism += __shfl_xor(mism, d);
-  if (t < ns && part == 0) {
-    surv[t] = (e & ~0xFFFFull) | (uint64_t)(32u * W - mism);
-    atomicAdd(&deg[i], 1u);
-    atomicAdd(&deg[j], 1u);
+  mism += __shfl_xor(mism, 1);
+  mism += __shfl_xor(mism, 2);
+  if (live && q == 0) {
+    if (pass) {
+      cand[t] = (e & ~0xFFFFull) | (uint64_t)(32u * W - mism);
+      atomicAdd(&deg[i], 1u);
+      atomicAdd(&deg[j], 1u);
+    } else {
+      cand[t] = kDeadPair;
+    }
   }
 }
 
@@ -745,8 +659,10 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict
                                                       uint64_t* __restrict__ list) {
   const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= ns) return;
+  const uint64_t e = surv[t];
+  if (e == kDeadPair) return;
   uint32_t i, j, c;
-  unpack_pair(surv[t], i, j, c);
+  unpack_pair(e, i, j, c);
   const uint64_t hi = (uint64_t)c << 32;
   list[off[i] + atomicAdd(&cursor[i], 1u)] = hi | j;
   list[off[j] + atomicAdd(&cursor[j], 1u)] = hi | i;
@@ -1259,21 +1175,6 @@ hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, uint32_t n, u
   return hipGetLastError();
 }
 
-hipError_t launch_gate(const float* unit, uint32_t NC, const uint8_t* anynz, uint32_t n, float thr,
-                       uint32_t acc1_end, uint64_t* surv, unsigned long long* nsurv, uint64_t cap,
-                       hipStream_t st) {
-  const uint32_t rows = 256;
-  dim3 grid((n + 255) / 256, (n + rows - 1) / rows);
-  switch (NC) {
-    case 1: hipLaunchKernelGGL(gate_kernel<1>, grid, dim3(256), 0, st, unit, anynz, n, thr, acc1_end, rows, surv, nsurv, cap); break;
-    case 2: hipLaunchKernelGGL(gate_kernel<2>, grid, dim3(256), 0, st, unit, anynz, n, thr, acc1_end, rows, surv, nsurv, cap); break;
-    case 3: hipLaunchKernelGGL(gate_kernel<3>, grid, dim3(256), 0, st, unit, anynz, n, thr, acc1_end, rows, surv, nsurv, cap); break;
-    case 4: hipLaunchKernelGGL(gate_kernel<4>, grid, dim3(256), 0, st, unit, anynz, n, thr, acc1_end, rows, surv, nsurv, cap); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
 void gate_work_items(uint32_t n, std::vector<uint32_t>& items) {
   // (bi << 16 | bj0, tile count) pairs; full items first
   const uint32_t T = (n + 255) / 256;
@@ -1302,34 +1203,32 @@ hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, uint32_t n, int32_t 
   return hipGetLastError();
 }
 
-hipError_t launch_exact_gate(const uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,
-                             const uint8_t* anynz, float thr, uint32_t acc1_end, uint64_t* surv,
-                             unsigned long long* nsurv, uint64_t cap, hipStream_t st) {
+hipError_t launch_gate_compare(uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,
+                               const uint8_t* anynz, float thr, uint32_t acc1_end,
+                               const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st) {
   if (nc == 0) return hipSuccess;
-  const dim3 grid((unsigned)((nc + 255) / 256));
+  const dim3 grid((unsigned)((nc * 4 + 255) / 256));
+#define RBG_GC(NCV, WV)                                                                          \
+  hipLaunchKernelGGL((gate_compare_kernel<NCV, WV>), grid, dim3(256), 0, st, cand, nc, unit, anynz, \
+                     thr, acc1_end, planes, deg)
+#define RBG_GC_W(NCV)                   \
+  switch (W) {                          \
+    case 16: RBG_GC(NCV, 16); break;    \
+    case 8: RBG_GC(NCV, 8); break;      \
+    case 4: RBG_GC(NCV, 4); break;      \
+    case 2: RBG_GC(NCV, 2); break;      \
+    case 1: RBG_GC(NCV, 1); break;      \
+    default: return hipErrorInvalidValue; \
+  }
   switch (NC) {
-    case 1: hipLaunchKernelGGL(exact_gate_kernel<1>, grid, dim3(256), 0, st, cand, nc, unit, anynz, thr, acc1_end, surv, nsurv, cap); break;
-    case 2: hipLaunchKernelGGL(exact_gate_kernel<2>, grid, dim3(256), 0, st, cand, nc, unit, anynz, thr, acc1_end, surv, nsurv, cap); break;
-    case 3: hipLaunchKernelGGL(exact_gate_kernel<3>, grid, dim3(256), 0, st, cand, nc, unit, anynz, thr, acc1_end, surv, nsurv, cap); break;
-    case 4: hipLaunchKernelGGL(exact_gate_kernel<4>, grid, dim3(256), 0, st, cand, nc, unit, anynz, thr, acc1_end, surv, nsurv, cap); break;
+    case 1: RBG_GC_W(1); break;
+    case 2: RBG_GC_W(2); break;
+    case 3: RBG_GC_W(3); break;
+    case 4: RBG_GC_W(4); break;
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
-}
-
-hipError_t launch_pair_compare(const uint32_t* planes, uint32_t W, uint64_t* surv, uint64_t ns,
-                               uint32_t* deg, hipStream_t st) {
-  if (ns == 0) return hipSuccess;
-  const uint32_t lpp = W < 4 ? W : 4;
-  const dim3 grid((unsigned)((ns * lpp + 255) / 256));
-  switch (W) {
-    case 16: hipLaunchKernelGGL(pair_compare_kernel<16>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
-    case 8: hipLaunchKernelGGL(pair_compare_kernel<8>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
-    case 4: hipLaunchKernelGGL(pair_compare_kernel<4>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
-    case 2: hipLaunchKernelGGL(pair_compare_kernel<2>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
-    case 1: hipLaunchKernelGGL(pair_compare_kernel<1>, grid, dim3(256), 0, st, planes, surv, ns, deg); break;
-    default: return hipErrorInvalidValue;
-  }
+#undef RBG_GC_W
+#undef RBG_GC
   return hipGetLastError();
 }
 
