@@ -369,3 +369,25 @@ def calibrate(rowmax, m, b0, regime, coverage=0.95, rand_seed=1):
     L.orc_calibrate.argtypes = [u32p, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint, C.c_int]
     rm = np.ascontiguousarray(rowmax, np.uint32)
     return L.orc_calibrate(_p(rm, u32p), len(rm), m, b0, coverage, rand_seed, regime)
+
+
+def rows_topk(win16, unit, anynz, S, b0, rows, K=200, thr=None, threads=0):
+    """Fused-regime top-K (ids sorted), row maxima and gated-partner counts of
+    the given rows against all partners (thr None: the default gate value)."""
+    L = orc()
+    L.orc_rows_topk.argtypes = [u16p, f32p, u8p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                                C.c_uint32, C.c_float, u32p, C.c_uint32, u32p, u32p, u32p, u64p, C.c_int]
+    w = np.ascontiguousarray(win16, np.uint16)
+    n, m = w.shape
+    u = np.ascontiguousarray(unit, np.float32) if unit is not None else None
+    nz = np.ascontiguousarray(anynz, np.uint8) if anynz is not None else None
+    r = np.ascontiguousarray(rows, np.uint32)
+    if thr is None:
+        thr = np.float32(0.7 - 1e-4) if (u is not None and S > 2) else -np.inf
+    ids = np.zeros((len(r), K), np.uint32)
+    ln = np.zeros(len(r), np.uint32)
+    rm = np.zeros(len(r), np.uint32)
+    gt = np.zeros(len(r), np.uint64)
+    L.orc_rows_topk(_p(w, u16p), _p(u, f32p), _p(nz, u8p), n, m, S, b0, K, float(thr), _p(r, u32p), len(r),
+                    _p(ids, u32p), _p(ln, u32p), _p(rm, u32p), _p(gt, u64p), threads)
+    return ids, ln, rm, gt

@@ -595,3 +595,52 @@ extern "C" double orc_pair_pass_rows(const uint16_t* win16, const float* unit, c
   *gated_out = gated;
   return secs;
 }
+
+// Fused-regime top-K of selected rows against all partners
+// (src/impl/rb_graph.cpp:931-975 heap order, gate of :1108-1116): for each
+// row r in rows[], the gated partners j != r (is_nz, !(unit_dot_f < thr))
+// ranked by (sv desc, id asc), the first K ids written to out_ids[t*K ..]
+// sorted by id, their number to out_len[t], the row's maximum count over
+// its gated partners to out_rowmax[t] and the number of gated partners to
+// out_gated[t].  thr <= -inf disables the gate.  A size-independent check of
+// the selection for sampled rows of a large assembly.
+extern "C" void orc_rows_topk(const uint16_t* win16, const float* unit, const uint8_t* anynz,
+                              uint64_t n, uint32_t m, uint32_t S, double b0, uint32_t K, float thr,
+                              const uint32_t* rows, uint32_t nrows, uint32_t* out_ids,
+                              uint32_t* out_len, uint32_t* out_rowmax, uint64_t* out_gated,
+                              int threads) {
+  if (threads <= 0) threads = omp_get_max_threads();
+  std::vector<float> svf(m + 1);
+  for (uint32_t c = 0; c <= m; ++c) svf[c] = (float)orc_sv_from_count(c, m, b0);
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (uint32_t t = 0; t < nrows; ++t) {
+    const size_t i = rows[t];
+    std::priority_queue<Cand, std::vector<Cand>, Weaker> h;
+    uint32_t rmax = 0;
+    uint64_t gated = 0;
+    for (size_t j = 0; j < n; ++j) {
+      if (j == i || !is_nz(anynz, i, j)) continue;
+      if (unit && orc_unit_dot_f(unit + i * S, unit + j * S, S) < thr) continue;
+      ++gated;
+      const uint32_t c = orc_match_count16(win16 + i * m, win16 + j * m, m);
+      if (c > rmax) rmax = c;
+      const Cand cd{(uint32_t)j, svf[c]};
+      if (h.size() < K) {
+        h.push(cd);
+      } else if (Weaker()(cd, h.top())) {
+        h.pop();
+        h.push(cd);
+      }
+    }
+    std::vector<uint32_t> ids;
+    while (!h.empty()) {
+      ids.push_back(h.top().id);
+      h.pop();
+    }
+    std::sort(ids.begin(), ids.end());
+    std::copy(ids.begin(), ids.end(), out_ids + (size_t)t * K);
+    out_len[t] = (uint32_t)ids.size();
+    out_rowmax[t] = rmax;
+    out_gated[t] = gated;
+  }
+}

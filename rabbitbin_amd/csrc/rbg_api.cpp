@@ -136,7 +136,7 @@ struct rbg_asm {
   std::vector<uint8_t> anynz_h;
   GateQuant gq;  // bound statistics of q8 (the int8 rows live on the device)
   // pipeline intermediates
-  DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, cand, nsurv, scnt, deg, rowmax, off, cursor,
+  DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, cand, nsurv, keep, scnt, deg, rowmax, off, cursor,
       list, topk_id, topk_cnt, topk_len, row_edges, row_off, efrom, eto, ecnt, escomp, escore,
       scan_tmp, tiles;
   uint32_t ntiles = 0;
@@ -389,9 +389,10 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   RBG_TRY_HIP(ctx, a->row_edges.ensure((size_t)n * 4));
   RBG_TRY_HIP(ctx, a->row_off.ensure((size_t)(n + 1) * 8));
   const int mutual = regime == 2 ? 1 : 0;
-  RBG_TRY_HIP(ctx, launch_emit((const uint32_t*)a->topk_id.p, (const uint16_t*)a->topk_cnt.p,
-                               (const uint32_t*)a->topk_len.p, n, K, cmin, mutual, 0,
-                               (uint32_t*)a->row_edges.p, nullptr, nullptr, nullptr, nullptr, st));
+  RBG_TRY_HIP(ctx, a->keep.ensure((size_t)n * ((K + 63) / 64) * sizeof(uint64_t)));
+  RBG_TRY_HIP(ctx, launch_emit_mark((const uint32_t*)a->topk_id.p, (const uint16_t*)a->topk_cnt.p,
+                                    (const uint32_t*)a->topk_len.p, n, K, cmin, mutual,
+                                    (uint64_t*)a->keep.p, (uint32_t*)a->row_edges.p, st));
   size_t tmp_bytes = 0;
   RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64(nullptr, nullptr, n, nullptr, tmp_bytes, st));
   RBG_TRY_HIP(ctx, a->scan_tmp.ensure(tmp_bytes));
@@ -404,10 +405,11 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   RBG_TRY_HIP(ctx, a->eto.ensure(std::max<uint64_t>(ne, 1) * 4));
   RBG_TRY_HIP(ctx, a->ecnt.ensure(std::max<uint64_t>(ne, 1) * 2));
   RBG_TRY_HIP(ctx, a->escomp.ensure(std::max<uint64_t>(ne, 1) * 4));
-  RBG_TRY_HIP(ctx, launch_emit((const uint32_t*)a->topk_id.p, (const uint16_t*)a->topk_cnt.p,
-                               (const uint32_t*)a->topk_len.p, n, K, cmin, mutual, 1, nullptr,
-                               (const uint64_t*)a->row_off.p, (uint32_t*)a->efrom.p,
-                               (uint32_t*)a->eto.p, (uint16_t*)a->ecnt.p, st));
+  RBG_TRY_HIP(ctx, launch_emit_write((const uint32_t*)a->topk_id.p, (const uint16_t*)a->topk_cnt.p,
+                                     (const uint32_t*)a->topk_len.p, n, K,
+                                     (const uint64_t*)a->keep.p, (const uint64_t*)a->row_off.p,
+                                     (uint32_t*)a->efrom.p, (uint32_t*)a->eto.p,
+                                     (uint16_t*)a->ecnt.p, st));
   RBG_TRY_HIP(ctx, launch_cnt_to_sv((const uint16_t*)a->ecnt.p, ne, (const float*)a->svf.p,
                                     (float*)a->escomp.p, st));
   RBG_TRY_HIP(ctx, hipEventRecord(ev[4], st));

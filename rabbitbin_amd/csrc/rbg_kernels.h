@@ -38,10 +38,15 @@ hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_
                               const uint16_t* level, uint32_t nlevels, uint32_t K,
                               uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
                               uint32_t* rowmax, hipStream_t st);
-hipError_t launch_emit(const uint32_t* topk_id, const uint16_t* topk_cnt, const uint32_t* topk_len,
-                       uint32_t n, uint32_t K, uint32_t cmin, int mutual, int pass,
-                       uint32_t* row_edges, const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
-                       uint16_t* ecnt, hipStream_t st);
+// Edge emission: pass 0 marks kept top-K entries (keep: n x ceil(K/64) u64)
+// and counts them per row; pass 1 writes them at the rows' offsets.
+hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
+                            const uint32_t* topk_len, uint32_t n, uint32_t K, uint32_t cmin,
+                            int mutual, uint64_t* keep, uint32_t* row_edges, hipStream_t st);
+hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
+                             const uint32_t* topk_len, uint32_t n, uint32_t K, const uint64_t* keep,
+                             const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
+                             uint16_t* ecnt, hipStream_t st);
 hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const float* scomp,
                                uint64_t ne, const float* unit, const float* raw,
                                const double* colnorm, uint32_t S, double w_comp, double min_edge,

@@ -748,12 +748,124 @@ __device__ void find_reaching(const uint32_t* h, uint32_t nb, bool rev, uint32_t
   __syncthreads();
 }
 
+// Rows of a candidate list with at most kSelectStage entries: the row is
+// staged once in LDS (ids, level << 16 | count); the threshold level comes
+// from a level histogram; entries above it are kept, and of the entries at
+// it the smallest ids fill the remaining slots (rank by id among those
+// entries only).  The kept entries (at most K) are compacted and each one's
+// output slot is the number of kept ids below it (ids within a row are
+// distinct), counted four ids per 128-bit LDS read, so the list comes out
+// sorted by id without a sorting network.  `lds` holds 4096 words: staged ids
+// [0, 1024), level|count [1024, 2048), level histogram [2048, 3072), tie ids
+// [3072, 4096); kept ids and counts go to sel_id / sel_cnt (>= K + 3 slots).
+constexpr uint32_t kSelectStage = 1024;
+
+__device__ __forceinline__ uint32_t count_below(const uint32_t* __restrict__ v, uint32_t nv4,
+                                                uint32_t key) {
+  uint32_t r = 0;
+  const uint4* v4 = reinterpret_cast<const uint4*>(v);
+  uint32_t q = 0;
+  for (; q + 4 <= nv4; q += 4) {
+    const uint4 a = v4[q], b = v4[q + 1], c = v4[q + 2], d = v4[q + 3];
+    r += (a.x < key) + (a.y < key) + (a.z < key) + (a.w < key);
+    r += (b.x < key) + (b.y < key) + (b.z < key) + (b.w < key);
+    r += (c.x < key) + (c.y < key) + (c.z < key) + (c.w < key);
+    r += (d.x < key) + (d.y < key) + (d.z < key) + (d.w < key);
+  }
+  for (; q < nv4; ++q) {
+    const uint4 a = v4[q];
+    r += (a.x < key) + (a.y < key) + (a.z < key) + (a.w < key);
+  }
+  return r;
+}
+
+__device__ void select_staged_row(const RowSource& src, uint32_t i, uint32_t len,
+                                  const uint16_t* __restrict__ level, uint32_t nlevels, uint32_t K,
+                                  uint32_t* lds, uint32_t* sel_id, uint16_t* sel_cnt,
+                                  uint32_t* wave_tot, uint32_t* tmp_bin, uint32_t* tmp_below,
+                                  uint32_t* rmax, uint32_t* nkept, uint32_t* ntie,
+                                  uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
+                                  uint32_t* __restrict__ topk_len,
+                                  uint32_t* __restrict__ rowmax_out) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t* eid = lds;
+  uint32_t* elc = lds + 1024;
+  uint32_t* lhist = lds + 2048;
+  uint32_t* tie = lds + 3072;
+  for (uint32_t b = tid; b < 1024; b += 256) lhist[b] = 0;
+  if (tid == 0) {
+    *rmax = 0;
+    *nkept = 0;
+    *ntie = 0;
+  }
+  __syncthreads();
+  uint32_t my_max = 0;
+  for (uint32_t t = tid; t < len; t += 256) {
+    const uint64_t e = src.list[src.off[i] + t];
+    const uint32_t c = (uint32_t)(e >> 32), lv = level[c];
+    eid[t] = (uint32_t)e;
+    elc[t] = (lv << 16) | c;
+    atomicAdd(&lhist[lv], 1u);
+    my_max = c > my_max ? c : my_max;
+  }
+  if (my_max) atomicMax(rmax, my_max);
+  __syncthreads();
+  uint32_t Ls = 0, need_here = 0xFFFFFFFFu;  // len <= K: every entry is kept
+  if (len > K) {
+    find_reaching(lhist, nlevels, true, K, wave_tot, tmp_bin, tmp_below);
+    Ls = *tmp_bin;
+    need_here = K - *tmp_below;
+  }
+  const bool tie_cut = len > K && need_here < lhist[Ls];
+  for (uint32_t t = tid; t < len; t += 256) {
+    const uint32_t lv = elc[t] >> 16;
+    if (lv > Ls || (lv == Ls && !tie_cut)) {
+      const uint32_t s = atomicAdd(nkept, 1u);
+      sel_id[s] = eid[t];
+      sel_cnt[s] = (uint16_t)(elc[t] & 0xFFFFu);
+    } else if (lv == Ls) {
+      tie[atomicAdd(ntie, 1u)] = t;
+    }
+  }
+  __syncthreads();
+  if (tie_cut) {
+    // the smallest need_here ids among the entries at the threshold level
+    // (their counts may differ: several counts can share one level)
+    const uint32_t nt = *ntie;
+    uint32_t* tie_id = lhist;  // the level histogram is no longer read
+    for (uint32_t u = tid; u < ((nt + 3) & ~3u); u += 256) tie_id[u] = u < nt ? eid[tie[u]] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t u = tid; u < nt; u += 256) {
+      const uint32_t me = tie_id[u];
+      if (count_below(tie_id, (nt + 3) / 4, me) < need_here) {
+        const uint32_t s = atomicAdd(nkept, 1u);
+        sel_id[s] = me;
+        sel_cnt[s] = (uint16_t)(elc[tie[u]] & 0xFFFFu);
+      }
+    }
+    __syncthreads();
+  }
+  const uint32_t ns = *nkept;  // = min(len, K)
+  for (uint32_t u = ns + tid; u < ((ns + 3) & ~3u); u += 256) sel_id[u] = 0xFFFFFFFFu;
+  __syncthreads();
+  for (uint32_t t = tid; t < ns; t += 256) {
+    const uint32_t me = sel_id[t];
+    const uint32_t pos = count_below(sel_id, (ns + 3) / 4, me);
+    topk_id[(size_t)i * K + pos] = me;
+    topk_cnt[(size_t)i * K + pos] = sel_cnt[t];
+  }
+  if (tid == 0) {
+    topk_len[i] = ns;
+    rowmax_out[i] = *rmax;
+  }
+}
+
 __global__ __launch_bounds__(256) void select_topk_kernel(
     RowSource src, uint32_t n, const uint16_t* __restrict__ level, uint32_t nlevels, uint32_t K,
     uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
     uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out) {
-  __shared__ uint32_t hist[4096];
-  __shared__ uint32_t sel_id[1024];
+  __shared__ __align__(16) uint32_t hist[4096];
+  __shared__ __align__(16) uint32_t sel_id[1024];
   __shared__ uint16_t sel_cnt[1024];
   __shared__ uint32_t nsel, tot, tmp_bin, tmp_below, rmax;
   __shared__ uint32_t wave_tot[4];
@@ -761,6 +873,11 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
   const uint32_t tid = threadIdx.x;
   const uint32_t len = row_len(src, i);
 
+  if (!src.dense && len <= kSelectStage && nlevels <= 1024) {
+    select_staged_row(src, i, len, level, nlevels, K, hist, sel_id, sel_cnt, wave_tot, &tmp_bin,
+                      &tmp_below, &rmax, &nsel, &tot, topk_id, topk_cnt, topk_len, rowmax_out);
+    return;
+  }
   for (uint32_t b = tid; b < 4096; b += 256) hist[b] = 0;
   if (tid == 0) {
     nsel = 0;
@@ -881,53 +998,87 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
 
 // Edge emission.  Small regime: (i, j) for i < j, j in top(i), count >= cmin.
 // Fused regime additionally requires i in top(j) (mutual, default on,
-// src/impl/rb_graph.cpp:1283-1348).  Pass 0 counts per row, pass 1 writes at
-// the row's offset, so the edges come out sorted by (i, j).
-__global__ __launch_bounds__(256) void emit_kernel(const uint32_t* __restrict__ topk_id,
-                                                   const uint16_t* __restrict__ topk_cnt,
-                                                   const uint32_t* __restrict__ topk_len, uint32_t n,
-                                                   uint32_t K, uint32_t cmin, int mutual, int pass,
-                                                   uint32_t* __restrict__ row_edges,
-                                                   const uint64_t* __restrict__ row_off,
-                                                   uint32_t* __restrict__ efrom,
-                                                   uint32_t* __restrict__ eto,
-                                                   uint16_t* __restrict__ ecnt) {
+// src/impl/rb_graph.cpp:1283-1348).  Top-K lists are sorted by id.
+// Pass 0 (emit_mark_kernel): one wave per row; lane l takes list slots
+// l, l+64, l+128, l+192 together, and the membership test of i in top(j) is a
+// fixed-step binary search (steps topstep, topstep/2, ..., 1) whose four
+// searches per lane are interleaved, so a row costs about log2(K) dependent
+// loads.  The keep decisions are stored as one 64-bit mask per slot group
+// and the per-row edge count.  Pass 1 (emit_write_kernel) writes the kept
+// entries at the row's offset, so the edges come out sorted by (i, j).
+__global__ __launch_bounds__(256) void emit_mark_kernel(
+    const uint32_t* __restrict__ topk_id, const uint16_t* __restrict__ topk_cnt,
+    const uint32_t* __restrict__ topk_len, uint32_t n, uint32_t K, uint32_t topstep,
+    uint32_t cmin, int mutual, uint32_t KW, uint64_t* __restrict__ keep,
+    uint32_t* __restrict__ row_edges) {
   const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t ln = threadIdx.x & 63u;
   if (i >= n) return;
   const uint32_t len = topk_len[i];
-  uint32_t written = 0;
-  for (uint32_t t0 = 0; t0 < len; t0 += 64) {
-    const uint32_t t = t0 + ln;
-    bool keep = false;
-    uint32_t j = 0, c = 0;
-    if (t < len) {
-      j = topk_id[(size_t)i * K + t];
-      c = topk_cnt[(size_t)i * K + t];
-      keep = j > i && c >= cmin;
-      if (keep && mutual) {
-        const uint32_t* lst = topk_id + (size_t)j * K;
-        uint32_t lo = 0, hi = topk_len[j];
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (lst[mid] < i)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        keep = lo < topk_len[j] && lst[lo] == i;
+  uint32_t total = 0;
+  for (uint32_t s0 = 0; s0 * 64 < len; s0 += 4) {
+    uint32_t jv[4], pos[4], L[4];
+    bool k[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t t = (s0 + u) * 64 + ln;
+      k[u] = false;
+      jv[u] = 0;
+      if (t < len) {
+        jv[u] = topk_id[(size_t)i * K + t];
+        k[u] = jv[u] > i && topk_cnt[(size_t)i * K + t] >= cmin;
       }
     }
-    const uint64_t mask = __ballot(keep);
-    if (pass == 1 && keep) {
-      const uint64_t pos = row_off[i] + written + lanes_below(mask);
-      efrom[pos] = i;
-      eto[pos] = j;
-      ecnt[pos] = (uint16_t)c;
+    if (mutual) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        L[u] = k[u] ? topk_len[jv[u]] : 0u;
+        pos[u] = 0;
+      }
+      for (uint32_t step = topstep; step > 0; step >>= 1) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)  // pos = number of list entries < i
+          if (pos[u] + step <= L[u] && topk_id[(size_t)jv[u] * K + pos[u] + step - 1] < i)
+            pos[u] += step;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k[u]) k[u] = pos[u] < L[u] && topk_id[(size_t)jv[u] * K + pos[u]] == i;
     }
-    written += (uint32_t)__popcll(mask);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if ((s0 + u) * 64 < len) {
+        const uint64_t m = __ballot(k[u]);
+        if (ln == 0) keep[(size_t)i * KW + s0 + u] = m;
+        total += (uint32_t)__popcll(m);
+      }
+    }
   }
-  if (pass == 0 && ln == 0) row_edges[i] = written;
+  if (ln == 0) row_edges[i] = total;
+}
+
+__global__ __launch_bounds__(256) void emit_write_kernel(
+    const uint32_t* __restrict__ topk_id, const uint16_t* __restrict__ topk_cnt,
+    const uint32_t* __restrict__ topk_len, uint32_t n, uint32_t K, uint32_t KW,
+    const uint64_t* __restrict__ keep, const uint64_t* __restrict__ row_off,
+    uint32_t* __restrict__ efrom, uint32_t* __restrict__ eto, uint16_t* __restrict__ ecnt) {
+  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t ln = threadIdx.x & 63u;
+  if (i >= n) return;
+  const uint32_t len = topk_len[i];
+  const uint64_t off = row_off[i];
+  uint64_t written = 0;
+  for (uint32_t s = 0; s * 64 < len; ++s) {
+    const uint64_t m = keep[(size_t)i * KW + s];
+    if ((m >> ln) & 1ull) {
+      const uint32_t t = s * 64 + ln;
+      const uint64_t pos = off + written + lanes_below(m);
+      efrom[pos] = i;
+      eto[pos] = topk_id[(size_t)i * K + t];
+      ecnt[pos] = topk_cnt[(size_t)i * K + t];
+    }
+    written += (uint32_t)__popcll(m);
+  }
 }
 
 // Stage (d): per-edge weight in FP64 (the reference's depth_edge_term with
@@ -1270,13 +1421,24 @@ hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_emit(const uint32_t* topk_id, const uint16_t* topk_cnt, const uint32_t* topk_len,
-                       uint32_t n, uint32_t K, uint32_t cmin, int mutual, int pass,
-                       uint32_t* row_edges, const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
-                       uint16_t* ecnt, hipStream_t st) {
+hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
+                            const uint32_t* topk_len, uint32_t n, uint32_t K, uint32_t cmin,
+                            int mutual, uint64_t* keep, uint32_t* row_edges, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(emit_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt, topk_len, n,
-                     K, cmin, mutual, pass, row_edges, row_off, efrom, eto, ecnt);
+  uint32_t topstep = 1;
+  while (topstep * 2 <= K) topstep *= 2;
+  hipLaunchKernelGGL(emit_mark_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt,
+                     topk_len, n, K, topstep, cmin, mutual, (K + 63) / 64, keep, row_edges);
+  return hipGetLastError();
+}
+
+hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
+                             const uint32_t* topk_len, uint32_t n, uint32_t K, const uint64_t* keep,
+                             const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
+                             uint16_t* ecnt, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(emit_write_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt,
+                     topk_len, n, K, (K + 63) / 64, keep, row_off, efrom, eto, ecnt);
   return hipGetLastError();
 }
 

@@ -61,3 +61,121 @@ def test_graph_fused_regime_forced_3k(ctx, S):
     assert g["regime"] == 2
     assert g["gated_pairs"] == ref["gated_pairs"]
     _check_graph(g, ref)
+
+
+def _concat(contigs):
+    off = np.zeros(len(contigs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(c) for c in contigs])
+    return b"".join(contigs), off
+
+
+EDGE_CONTIGS = [b"", b"A", b"ACG", b"ACGT", b"NNNN", b"acgtacgtAC", b"ACGTNACGTNNACGT", b"TTTTTTTTTTTTTTTT",
+                b"ACGT" * 100, b"GGGGCCCC" * 50, b"AC-GT*RYac" * 20, bytes(range(256)) * 3, b"N" * 5000,
+                b"ACGTTGCA" * 1000 + b"N" * 7 + b"ttgcaacg" * 500]
+
+
+@pytest.mark.parametrize("m", [2, 16, 100, 512])
+def test_sketch_edge_cases(ctx, m):
+    """Empty, shorter than 4, all-N, lowercase, junk bytes, N runs, ragged
+    lengths at every alignment of the sequence buffer."""
+    rng = np.random.default_rng(m)
+    contigs = list(EDGE_CONTIGS)
+    for L in rng.integers(0, 3000, 40):
+        s = np.array(list("ACGTacgtN"))[rng.choice(9, int(L), p=[.22, .22, .22, .22, .02, .02, .02, .02, .04])]
+        contigs.append("".join(s).encode())
+    seqs, off = _concat(contigs)
+    w32, w16 = ctx.build_pmh_winners(seqs, off, m=m, seed=42)
+    ref32 = orc.sketch_batch(seqs, off, m=m, seed=42)
+    assert np.array_equal(w32, ref32)
+    assert np.array_equal(w16, orc.fold16(ref32))
+    assert not w32[:3].any() and not w32[4].any()  # no 4-mer: every register unset
+
+
+def test_graph_natural_fused_regime_26k(ctx):
+    """n = 26,000 > 25,000 selects the fused regime by itself (S = 10: one
+    masked 16-lane chunk in the gate); the resident assembly with the edge
+    list kept in HBM gives the same graph."""
+    seqs, off, depth, _ = synth.workload("asm50k", n=26000)
+    ref = orc.full_pipeline(seqs, off, depth, rand_seed=1)
+    assert ref["regime"] == 2
+    g = ctx.build_graph(seqs, off, depth, api.default_params(rand_seed=1), diag=True)
+    assert g["regime"] == 2 and g["gated_pairs"] == ref["gated_pairs"]
+    _check_graph(g, ref)
+    asm = api.Assembly(ctx, seqs, off, depth)
+    try:
+        r = asm.build_graph(api.default_params(rand_seed=1), on_device=True)
+        assert r["from"] is None and r["n_edges"] == len(g["from"])
+        frm, to, sc, score = asm.copy_edges(r["n_edges"])
+        assert np.array_equal(frm, g["from"]) and np.array_equal(to, g["to"])
+        assert np.array_equal(sc, g["scomp"]) and np.array_equal(score, g["score"])
+    finally:
+        asm.free()
+
+
+def test_edge_weights_entry_point(ctx):
+    """rbg_edge_weights on an arbitrary edge list == the FP64 oracle, bit for
+    bit in FP64, for S on both sides of the 16-wide chunking."""
+    rng = np.random.default_rng(5)
+    for S in (2, 3, 16, 30, 70):
+        n = 500
+        d = rng.poisson(rng.lognormal(1, 1, (n, 1)) * rng.lognormal(0, 1, (1, S)) * 4).astype(np.float32)
+        _, unit, raw, colnorm, _ = orc.depth_features(d)
+        ne = 5000
+        fr = rng.integers(0, n, ne).astype(np.uint32)
+        to = rng.integers(0, n, ne).astype(np.uint32)
+        sc = rng.random(ne).astype(np.float32)
+        w, w64 = ctx.edge_weights(fr, to, sc, unit, raw, colnorm, S, orc.w_comp_for(S))
+        rw, rw64 = orc.edge_weights(fr, to, sc, unit, raw, colnorm, S, mode=1)
+        assert np.array_equal(w64, rw64) and np.array_equal(w, rw)
+
+
+def test_full_size_assembly_properties(ctx):
+    """asm300k (300,000 contigs, S = 30): size-independent checks of the whole
+    build -- sampled sketches, b0 and calibration recomputed from the GPU's
+    own outputs, exact top-K / row maxima of sampled rows against all
+    partners, and for sampled edges the count, the gate and the weight."""
+    seqs, off, depth, _ = synth.workload("asm300k")
+    n = len(off) - 1
+    asm = api.Assembly(ctx, seqs, off, depth)
+    try:
+        g = asm.build_graph(api.default_params(rand_seed=1), diag=True)
+    finally:
+        asm.free()
+    frm, to, sc, score = g["from"], g["to"], g["scomp"], g["score"]
+    assert g["regime"] == 2 and len(frm) > 1_000_000
+    # edges: sorted, i < j, unique, above the cutoff
+    assert (frm < to).all()
+    key = frm.astype(np.uint64) << np.uint64(32) | to
+    assert (np.diff(key.astype(np.int64)) > 0).all()
+    assert (sc.astype(np.float64) > g["sim_cutoff"] / 1000.0).all()
+    # sketches of sampled contigs
+    rng = np.random.default_rng(300)
+    sample = np.sort(rng.choice(n, 400, replace=False))
+    buf = np.asarray(seqs, np.uint8) if not isinstance(seqs, bytes) else np.frombuffer(seqs, np.uint8)
+    sub = [bytes(buf[off[c]:off[c + 1]]) for c in sample]
+    s_seqs, s_off = _concat(sub)
+    assert np.array_equal(g["win16"][sample], orc.fold16(orc.sketch_batch(s_seqs, s_off)))
+    # b0 and calibration from the GPU's own winners / row maxima
+    assert orc.pmh_baseline(g["win16"]) == g["b0"]
+    assert orc.calibrate(g["rowmax_cnt"], 500, g["b0"], 2, 0.95, 1) == g["sim_cutoff"]
+    # exact top-K, row maxima and gated-partner counts of sampled rows
+    _, unit, raw, colnorm, anynz = orc.depth_features(depth)
+    rows = np.sort(rng.choice(n, 48, replace=False)).astype(np.uint32)
+    ids, ln, rm, _ = orc.rows_topk(g["win16"], unit, anynz, 30, g["b0"], rows)
+    for t, r in enumerate(rows):
+        assert g["topk_len"][r] == ln[t] and g["rowmax_cnt"][r] == rm[t]
+        assert np.array_equal(np.sort(g["topk_id"][r, :ln[t]]), ids[t, :ln[t]])
+    # mutual filter, counts, gate and weights of sampled edges
+    es = rng.choice(len(frm), 3000, replace=False)
+    for e in es[:500]:
+        i, j = int(frm[e]), int(to[e])
+        assert j in g["topk_id"][i, :g["topk_len"][i]] and i in g["topk_id"][j, :g["topk_len"][j]]
+    svd = np.array([orc.sv_from_count(c, 500, g["b0"]) for c in range(501)])
+    for e in es:
+        i, j = int(frm[e]), int(to[e])
+        c = orc.orc().orc_match_count16(g["win16"][i].ctypes.data_as(orc.u16p),
+                                        g["win16"][j].ctypes.data_as(orc.u16p), 500)
+        assert np.float32(svd[c]) == sc[e]
+        assert not (orc.unit_dot_f(unit[i], unit[j]) < np.float32(0.7 - 1e-4))
+    w, w64 = orc.edge_weights(frm[es], to[es], sc[es], unit, raw, colnorm, 30, mode=1)
+    assert np.abs(score[es].astype(np.float64) - w64).max() <= 1e-6

@@ -672,6 +672,15 @@ def test_fused_regime_bruteforce():
     _, unit, _, _, _ = orc.depth_features(d)
     g = check_graph_bruteforce(w, unit, 3, 2, K=7)
     assert len(g["from"]) > 0 and 0 < g["gated_pairs"] < n * (n - 1) // 2
+    # the per-row restatement used for sampled rows of large assemblies
+    rows = np.array([0, 5, 100, 1300, n - 1], np.uint32)
+    ids, ln, rm, gated = orc.rows_topk(w, unit, np.ones(n, np.uint8), 3, orc.pmh_baseline(w), rows, K=7)
+    for t, r in enumerate(rows):
+        assert ln[t] == g["topk_len"][r] and rm[t] == g["rowmax_cnt"][r]
+        assert set(ids[t, :ln[t]].tolist()) == set(g["topk_id"][r, :ln[t]].tolist())
+    ok = ~(gate_dot_s3(unit) < np.float32(0.7 - 1e-4))
+    np.fill_diagonal(ok, False)
+    assert np.array_equal(gated, ok[rows].sum(1))
 
 
 # ---------------------------------------------------------------------------
