@@ -136,7 +136,7 @@ struct rbg_asm {
   std::vector<uint8_t> anynz_h;
   GateQuant gq;  // bound statistics of q8 (the int8 rows live on the device)
   // pipeline intermediates
-  DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, cand, nsurv, keep, scnt, deg, rowmax, off, cursor,
+  DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, cand, nsurv, keep, fence, scnt, deg, rowmax, off, cursor,
       list, topk_id, topk_cnt, topk_len, row_edges, row_off, efrom, eto, ecnt, escomp, escore,
       scan_tmp, tiles;
   uint32_t ntiles = 0;
@@ -290,7 +290,10 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   RBG_TRY_HIP(ctx, a->svf.ensure((m + 1) * sizeof(float)));
   RBG_TRY_HIP(ctx, hipMemcpyAsync(a->level.p, level.data(), (m + 1) * 2, hipMemcpyHostToDevice, st));
   RBG_TRY_HIP(ctx, hipMemcpyAsync(a->svf.p, svf.data(), (m + 1) * 4, hipMemcpyHostToDevice, st));
-  RBG_TRY_HIP(ctx, a->topk_id.ensure((size_t)n * K * sizeof(uint32_t)));
+  // +16 ids: the mutual test reads whole 16-id segments of a row's list
+  RBG_TRY_HIP(ctx, a->topk_id.ensure(((size_t)n * K + 16) * sizeof(uint32_t)));
+  const uint32_t FW = fence_words(K);
+  RBG_TRY_HIP(ctx, a->fence.ensure((size_t)n * FW * sizeof(uint32_t)));
   RBG_TRY_HIP(ctx, a->topk_cnt.ensure((size_t)n * K * sizeof(uint16_t)));
   RBG_TRY_HIP(ctx, a->topk_len.ensure((size_t)n * sizeof(uint32_t)));
   RBG_TRY_HIP(ctx, a->rowmax.ensure((size_t)n * sizeof(uint32_t)));
@@ -309,7 +312,8 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
     RBG_TRY_HIP(ctx, launch_select_dense((const uint16_t*)a->mat.p, anynz, n,
                                          (const uint16_t*)a->level.p, nlevels, K,
                                          (uint32_t*)a->topk_id.p, (uint16_t*)a->topk_cnt.p,
-                                         (uint32_t*)a->topk_len.p, (uint32_t*)a->rowmax.p, st));
+                                         (uint32_t*)a->topk_len.p, (uint32_t*)a->rowmax.p,
+                                         (uint32_t*)a->fence.p, FW, st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[10], st));
   } else {
     // abundance-first gate (src/impl/rb_graph.cpp:693-706)
@@ -368,7 +372,8 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
     RBG_TRY_HIP(ctx, launch_select_list((const uint64_t*)a->list.p, (const uint64_t*)a->off.p, n,
                                         (const uint16_t*)a->level.p, nlevels, K,
                                         (uint32_t*)a->topk_id.p, (uint16_t*)a->topk_cnt.p,
-                                        (uint32_t*)a->topk_len.p, (uint32_t*)a->rowmax.p, st));
+                                        (uint32_t*)a->topk_len.p, (uint32_t*)a->rowmax.p,
+                                        (uint32_t*)a->fence.p, FW, st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[10], st));
   }
   // calibration on the host (glibc rand stream)
@@ -391,8 +396,9 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   const int mutual = regime == 2 ? 1 : 0;
   RBG_TRY_HIP(ctx, a->keep.ensure((size_t)n * ((K + 63) / 64) * sizeof(uint64_t)));
   RBG_TRY_HIP(ctx, launch_emit_mark((const uint32_t*)a->topk_id.p, (const uint16_t*)a->topk_cnt.p,
-                                    (const uint32_t*)a->topk_len.p, n, K, cmin, mutual,
-                                    (uint64_t*)a->keep.p, (uint32_t*)a->row_edges.p, st));
+                                    (const uint32_t*)a->topk_len.p, (const uint32_t*)a->fence.p,
+                                    FW, n, K, cmin, mutual, (uint64_t*)a->keep.p,
+                                    (uint32_t*)a->row_edges.p, st));
   size_t tmp_bytes = 0;
   RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64(nullptr, nullptr, n, nullptr, tmp_bytes, st));
   RBG_TRY_HIP(ctx, a->scan_tmp.ensure(tmp_bytes));

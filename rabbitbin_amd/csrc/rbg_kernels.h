@@ -33,16 +33,20 @@ hipError_t launch_scatter(const uint64_t* surv, uint64_t ns, const uint64_t* off
 hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32_t n,
                                const uint16_t* level, uint32_t nlevels, uint32_t K,
                                uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                               uint32_t* rowmax, hipStream_t st);
+                               uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st);
 hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_t n,
                               const uint16_t* level, uint32_t nlevels, uint32_t K,
                               uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                              uint32_t* rowmax, hipStream_t st);
+                              uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st);
+// Fence rows of the selected lists: FW words per row (16-id stride, padded
+// with 0xFFFFFFFF); FW = 16 * ceil(ceil(K/16) / 16).
+inline uint32_t fence_words(uint32_t K) { return 16u * (((K + 15u) / 16u + 15u) / 16u); }
 // Edge emission: pass 0 marks kept top-K entries (keep: n x ceil(K/64) u64)
 // and counts them per row; pass 1 writes them at the rows' offsets.
 hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
-                            const uint32_t* topk_len, uint32_t n, uint32_t K, uint32_t cmin,
-                            int mutual, uint64_t* keep, uint32_t* row_edges, hipStream_t st);
+                            const uint32_t* topk_len, const uint32_t* fence, uint32_t FW,
+                            uint32_t n, uint32_t K, uint32_t cmin, int mutual, uint64_t* keep,
+                            uint32_t* row_edges, hipStream_t st);
 hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
                              const uint32_t* topk_len, uint32_t n, uint32_t K, const uint64_t* keep,
                              const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,

@@ -786,7 +786,8 @@ __device__ void select_staged_row(const RowSource& src, uint32_t i, uint32_t len
                                   uint32_t* rmax, uint32_t* nkept, uint32_t* ntie,
                                   uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
                                   uint32_t* __restrict__ topk_len,
-                                  uint32_t* __restrict__ rowmax_out) {
+                                  uint32_t* __restrict__ rowmax_out, uint32_t* __restrict__ fence,
+                                  uint32_t FW) {
   const uint32_t tid = threadIdx.x;
   uint32_t* eid = lds;
   uint32_t* elc = lds + 1024;
@@ -853,7 +854,9 @@ __device__ void select_staged_row(const RowSource& src, uint32_t i, uint32_t len
     const uint32_t pos = count_below(sel_id, (ns + 3) / 4, me);
     topk_id[(size_t)i * K + pos] = me;
     topk_cnt[(size_t)i * K + pos] = sel_cnt[t];
+    if ((pos & 15u) == 0) fence[(size_t)i * FW + pos / 16] = me;
   }
+  for (uint32_t q = (ns + 15) / 16 + tid; q < FW; q += 256) fence[(size_t)i * FW + q] = 0xFFFFFFFFu;
   if (tid == 0) {
     topk_len[i] = ns;
     rowmax_out[i] = *rmax;
@@ -863,7 +866,8 @@ __device__ void select_staged_row(const RowSource& src, uint32_t i, uint32_t len
 __global__ __launch_bounds__(256) void select_topk_kernel(
     RowSource src, uint32_t n, const uint16_t* __restrict__ level, uint32_t nlevels, uint32_t K,
     uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
-    uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out) {
+    uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out,
+    uint32_t* __restrict__ fence, uint32_t FW) {
   __shared__ __align__(16) uint32_t hist[4096];
   __shared__ __align__(16) uint32_t sel_id[1024];
   __shared__ uint16_t sel_cnt[1024];
@@ -875,7 +879,8 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
 
   if (!src.dense && len <= kSelectStage && nlevels <= 1024) {
     select_staged_row(src, i, len, level, nlevels, K, hist, sel_id, sel_cnt, wave_tot, &tmp_bin,
-                      &tmp_below, &rmax, &nsel, &tot, topk_id, topk_cnt, topk_len, rowmax_out);
+                      &tmp_below, &rmax, &nsel, &tot, topk_id, topk_cnt, topk_len, rowmax_out,
+                      fence, FW);
     return;
   }
   for (uint32_t b = tid; b < 4096; b += 256) hist[b] = 0;
@@ -990,6 +995,8 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
     topk_id[(size_t)i * K + t] = sel_id[t];
     topk_cnt[(size_t)i * K + t] = sel_cnt[t];
   }
+  for (uint32_t q = tid; q < FW; q += 256)
+    fence[(size_t)i * FW + q] = 16 * q < ns ? sel_id[16 * q] : 0xFFFFFFFFu;
   if (tid == 0) {
     topk_len[i] = ns;
     rowmax_out[i] = rmax;
@@ -1000,24 +1007,25 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
 // Fused regime additionally requires i in top(j) (mutual, default on,
 // src/impl/rb_graph.cpp:1283-1348).  Top-K lists are sorted by id.
 // Pass 0 (emit_mark_kernel): one wave per row; lane l takes list slots
-// l, l+64, l+128, l+192 together, and the membership test of i in top(j) is a
-// fixed-step binary search (steps topstep, topstep/2, ..., 1) whose four
-// searches per lane are interleaved, so a row costs about log2(K) dependent
-// loads.  The keep decisions are stored as one 64-bit mask per slot group
+// l, l+64, l+128, l+192 together.  The membership test of i in top(j) reads
+// j's fence row (the ids at list positions 0, 16, 32, ..., written by the
+// selection) to find the 16-entry segment that can hold i, then that
+// segment: two 64-byte reads per test.  The keep decisions are stored as one 64-bit mask per slot group
 // and the per-row edge count.  Pass 1 (emit_write_kernel) writes the kept
 // entries at the row's offset, so the edges come out sorted by (i, j).
 __global__ __launch_bounds__(256) void emit_mark_kernel(
     const uint32_t* __restrict__ topk_id, const uint16_t* __restrict__ topk_cnt,
-    const uint32_t* __restrict__ topk_len, uint32_t n, uint32_t K, uint32_t topstep,
-    uint32_t cmin, int mutual, uint32_t KW, uint64_t* __restrict__ keep,
+    const uint32_t* __restrict__ topk_len, const uint32_t* __restrict__ fence, uint32_t FW,
+    uint32_t n, uint32_t K, uint32_t cmin, int mutual, uint32_t KW, uint64_t* __restrict__ keep,
     uint32_t* __restrict__ row_edges) {
+  const bool aligned16 = (K & 3u) == 0;
   const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t ln = threadIdx.x & 63u;
   if (i >= n) return;
   const uint32_t len = topk_len[i];
   uint32_t total = 0;
   for (uint32_t s0 = 0; s0 * 64 < len; s0 += 4) {
-    uint32_t jv[4], pos[4], L[4];
+    uint32_t jv[4], L[4];
     bool k[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -1030,20 +1038,41 @@ __global__ __launch_bounds__(256) void emit_mark_kernel(
       }
     }
     if (mutual) {
+      // fence row of j: ids at list positions 0, 16, 32, ... (0xFFFFFFFF past
+      // the end); the number of fences <= i locates i's 16-entry segment
+      uint32_t qn[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         L[u] = k[u] ? topk_len[jv[u]] : 0u;
-        pos[u] = 0;
+        qn[u] = 0;
       }
-      for (uint32_t step = topstep; step > 0; step >>= 1) {
+      for (uint32_t f = 0; f < FW; f += 4) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u)  // pos = number of list entries < i
-          if (pos[u] + step <= L[u] && topk_id[(size_t)jv[u] * K + pos[u] + step - 1] < i)
-            pos[u] += step;
+        for (int u = 0; u < 4; ++u) {
+          if (k[u]) {
+            const uint4 fv = *reinterpret_cast<const uint4*>(fence + (size_t)jv[u] * FW + f);
+            qn[u] += (fv.x <= i) + (fv.y <= i) + (fv.z <= i) + (fv.w <= i);
+          }
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (k[u]) k[u] = pos[u] < L[u] && topk_id[(size_t)jv[u] * K + pos[u]] == i;
+      for (int u = 0; u < 4; ++u) {
+        if (k[u]) {
+          bool hit = false;
+          if (qn[u] > 0) {
+            const uint32_t p0 = 16 * (qn[u] - 1);
+            const uint32_t* seg = topk_id + (size_t)jv[u] * K + p0;
+#pragma unroll
+            for (int x = 0; x < 16; x += 4) {
+              const uint4 v = aligned16 ? *reinterpret_cast<const uint4*>(seg + x)
+                                        : make_uint4(seg[x], seg[x + 1], seg[x + 2], seg[x + 3]);
+              hit |= (v.x == i && p0 + x < L[u]) | (v.y == i && p0 + x + 1 < L[u]) |
+                     (v.z == i && p0 + x + 2 < L[u]) | (v.w == i && p0 + x + 3 < L[u]);
+            }
+          }
+          k[u] = hit;
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -1394,7 +1423,7 @@ hipError_t launch_scatter(const uint64_t* surv, uint64_t ns, const uint64_t* off
 hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32_t n,
                                const uint16_t* level, uint32_t nlevels, uint32_t K,
                                uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                               uint32_t* rowmax, hipStream_t st) {
+                               uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st) {
   RowSource s{};
   s.mat = mat;
   s.anynz = anynz;
@@ -1402,14 +1431,14 @@ hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32
   s.dense = true;
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(select_topk_kernel, dim3(n), dim3(256), 0, st, s, n, level, nlevels, K, topk_id,
-                     topk_cnt, topk_len, rowmax);
+                     topk_cnt, topk_len, rowmax, fence, FW);
   return hipGetLastError();
 }
 
 hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_t n,
                               const uint16_t* level, uint32_t nlevels, uint32_t K,
                               uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                              uint32_t* rowmax, hipStream_t st) {
+                              uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st) {
   RowSource s{};
   s.list = list;
   s.off = off;
@@ -1417,18 +1446,17 @@ hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_
   s.dense = false;
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(select_topk_kernel, dim3(n), dim3(256), 0, st, s, n, level, nlevels, K, topk_id,
-                     topk_cnt, topk_len, rowmax);
+                     topk_cnt, topk_len, rowmax, fence, FW);
   return hipGetLastError();
 }
 
 hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
-                            const uint32_t* topk_len, uint32_t n, uint32_t K, uint32_t cmin,
-                            int mutual, uint64_t* keep, uint32_t* row_edges, hipStream_t st) {
+                            const uint32_t* topk_len, const uint32_t* fence, uint32_t FW,
+                            uint32_t n, uint32_t K, uint32_t cmin, int mutual, uint64_t* keep,
+                            uint32_t* row_edges, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  uint32_t topstep = 1;
-  while (topstep * 2 <= K) topstep *= 2;
   hipLaunchKernelGGL(emit_mark_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt,
-                     topk_len, n, K, topstep, cmin, mutual, (K + 63) / 64, keep, row_edges);
+                     topk_len, fence, FW, n, K, cmin, mutual, (K + 63) / 64, keep, row_edges);
   return hipGetLastError();
 }
 
