@@ -11,6 +11,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <numeric>
 #include <vector>
 
 #include "../../include/rbg.h"
@@ -130,7 +131,7 @@ struct rbg_asm {
   uint64_t seq_bytes = 0;
   DevBuf seqs, offs;
   // depth features (device) and their host copies
-  DevBuf unit_pad, raw_pad, unit, raw, colnorm, anynz, q8;
+  DevBuf unit_pad, raw_pad, unit, raw, colnorm, anynz, q8, perm;
   uint32_t NC = 1;
   bool has_anynz = false;
   std::vector<uint8_t> anynz_h;
@@ -194,10 +195,20 @@ rbg_status asm_upload(rbg_ctx* ctx, rbg_asm* a, const char* seqs, const uint64_t
                                     ctx->stream));
     RBG_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (S > 1 && S <= 64) {
-      gate_quantize(unit.data(), n, S, a->gq);
+      // The pair pass visits rows in abundance order (pair_pass_order); the
+      // int8 rows are stored in that order and candidates map back through perm.
+      std::vector<uint32_t> perm;
+      pair_pass_order(unit.data(), n, S, perm);
+      std::vector<float> sorted((size_t)n * S);
+      for (uint32_t r = 0; r < n; ++r)
+        std::memcpy(&sorted[(size_t)r * S], &unit[(size_t)perm[r] * S], S * sizeof(float));
+      gate_quantize(sorted.data(), n, S, a->gq);
       RBG_TRY_HIP(ctx, a->q8.ensure(a->gq.q.size()));
       RBG_TRY_HIP(ctx, hipMemcpy(a->q8.p, a->gq.q.data(), a->gq.q.size(), hipMemcpyHostToDevice));
       std::vector<int8_t>().swap(a->gq.q);
+      RBG_TRY_HIP(ctx, a->perm.ensure((size_t)n * sizeof(uint32_t)));
+      RBG_TRY_HIP(ctx, hipMemcpy(a->perm.p, perm.data(), (size_t)n * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice));
     }
   }
   if (!(S > 1 && S <= 64)) {
@@ -209,6 +220,11 @@ rbg_status asm_upload(rbg_ctx* ctx, rbg_asm* a, const char* seqs, const uint64_t
     RBG_TRY_HIP(ctx, hipMemsetAsync(a->q8.p, 0, (size_t)n * 32, ctx->stream));
     RBG_TRY_HIP(ctx, a->unit_pad.ensure((size_t)n * 16 * sizeof(float)));
     RBG_TRY_HIP(ctx, hipMemsetAsync(a->unit_pad.p, 0, (size_t)n * 16 * sizeof(float), ctx->stream));
+    std::vector<uint32_t> ident(n);
+    std::iota(ident.begin(), ident.end(), 0u);
+    RBG_TRY_HIP(ctx, a->perm.ensure((size_t)n * sizeof(uint32_t)));
+    RBG_TRY_HIP(ctx, hipMemcpy(a->perm.p, ident.data(), (size_t)n * sizeof(uint32_t),
+                               hipMemcpyHostToDevice));
   }
   // work items of the fused pair pass (256-row stripes x up to 16 tiles)
   std::vector<uint32_t> tl;
@@ -335,6 +351,7 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
       RBG_TRY_HIP(ctx, hipMemsetAsync(counters, 0, 2 * sizeof(unsigned long long), st));
       RBG_TRY_HIP(ctx, hipEventRecord(ev[2], st));
       RBG_TRY_HIP(ctx, launch_gate_mfma((const int8_t*)a->q8.p, a->gq.kpad, n, qcut,
+                                        (const uint32_t*)a->perm.p,
                                         (const uint32_t*)a->tiles.p, a->ntiles,
                                         (uint64_t*)a->cand.p, counters, cap, st));
       RBG_TRY_HIP(ctx, hipEventRecord(ev[6], st));

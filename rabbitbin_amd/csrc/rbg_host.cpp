@@ -335,6 +335,31 @@ void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g) {
   }
 }
 
+void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32_t>& perm) {
+  std::vector<uint32_t> key(n, 0);
+  const uint32_t top = std::min<uint32_t>(3, S);
+  for (size_t r = 0; r < n; ++r) {
+    const float* u = unit + r * S;
+    uint32_t k = 0;
+    uint64_t used = 0;
+    for (uint32_t t = 0; t < top; ++t) {
+      uint32_t best = 0;
+      bool have = false;
+      for (uint32_t s = 0; s < S; ++s)
+        if (!((used >> s) & 1u) && (!have || u[s] > u[best])) {
+          best = s;
+          have = true;
+        }
+      used |= 1ull << best;
+      k = (k << 8) | best;
+    }
+    key[r] = k;
+  }
+  perm.resize(n);
+  std::iota(perm.begin(), perm.end(), 0u);
+  std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+}
+
 int32_t gate_qcut(const GateQuant& g, float thr) {
   if (g.scale <= 0) return INT32_MIN;
   // FP32 dot: each product passes through at most ceil(S/16) lane FMAs, the

@@ -63,6 +63,11 @@ struct GateQuant {
   uint32_t S = 0;
 };
 void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g);
+// Row order of the fused pair pass: rows sorted by the indices of their three
+// largest unit-vector entries (ties to the smaller sample, then by row), so
+// contigs with the same abundance pattern are adjacent and the pairs that
+// pass the abundance gate fall into few tiles.  perm[r] = original row.
+void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32_t>& perm);
 // Smallest integer dot a kept pair can have; INT32_MIN disables the prefilter.
 int32_t gate_qcut(const GateQuant& g, float thr);
 

@@ -20,9 +20,12 @@ hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, uint32_t n, u
                                hipStream_t st);
 // Work items of the fused pair pass: (bi << 16 | bj0, tile count) per item.
 void gate_work_items(uint32_t n, std::vector<uint32_t>& items);
+// q holds the int8 rows in pair-pass order; perm maps that order back to
+// contig ids (candidates are written as (min id, max id)).
 hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, uint32_t n, int32_t qcut,
-                            const uint32_t* items, uint32_t nitems, uint64_t* cand,
-                            unsigned long long* ncand, uint64_t cap, hipStream_t st);
+                            const uint32_t* perm, const uint32_t* items, uint32_t nitems,
+                            uint64_t* cand, unsigned long long* ncand, uint64_t cap,
+                            hipStream_t st);
 // Exact gate + register compare of the candidates, in place (rejected ->
 // all ones, survivors -> count in the low 16 bits; deg counts survivors per row).
 hipError_t launch_gate_compare(uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,

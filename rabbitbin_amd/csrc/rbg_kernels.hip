@@ -424,7 +424,8 @@ __device__ __forceinline__ void load_tile_ops(const int8_t* __restrict__ q, uint
 
 template <int KC>
 __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict__ q, uint32_t n,
-                                                        int32_t qcut, const uint2* __restrict__ items,
+                                                        int32_t qcut, const uint32_t* __restrict__ perm,
+                                                        const uint2* __restrict__ items,
                                                         uint64_t* __restrict__ cand,
                                                         unsigned long long* __restrict__ ncand,
                                                         uint64_t cap) {
@@ -467,7 +468,8 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
     base = __shfl(base, 0);
     for (uint32_t t = ln; t < bn; t += 64) {
       const uint32_t e = buf[wv][t];
-      const uint64_t i = ib + (e >> 12), j = jbase + (e & 0xFFFu);
+      const uint64_t a = perm[ib + (e >> 12)], b = perm[jbase + (e & 0xFFFu)];
+      const uint64_t i = a < b ? a : b, j = a < b ? b : a;
       if (base + t < cap) cand[base + t] = (i << 40) | (j << 16);
     }
     bn = 0;
@@ -1371,13 +1373,14 @@ void gate_work_items(uint32_t n, std::vector<uint32_t>& items) {
 }
 
 hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, uint32_t n, int32_t qcut,
-                            const uint32_t* items, uint32_t nitems, uint64_t* cand,
-                            unsigned long long* ncand, uint64_t cap, hipStream_t st) {
+                            const uint32_t* perm, const uint32_t* items, uint32_t nitems,
+                            uint64_t* cand, unsigned long long* ncand, uint64_t cap,
+                            hipStream_t st) {
   if (nitems == 0) return hipSuccess;
   const uint2* it = reinterpret_cast<const uint2*>(items);
   switch (kpad / 32) {
-    case 1: hipLaunchKernelGGL(gate_mfma_kernel<1>, dim3(nitems), dim3(256), 0, st, q, n, qcut, it, cand, ncand, cap); break;
-    case 2: hipLaunchKernelGGL(gate_mfma_kernel<2>, dim3(nitems), dim3(256), 0, st, q, n, qcut, it, cand, ncand, cap); break;
+    case 1: hipLaunchKernelGGL(gate_mfma_kernel<1>, dim3(nitems), dim3(256), 0, st, q, n, qcut, perm, it, cand, ncand, cap); break;
+    case 2: hipLaunchKernelGGL(gate_mfma_kernel<2>, dim3(nitems), dim3(256), 0, st, q, n, qcut, perm, it, cand, ncand, cap); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
