@@ -534,6 +534,33 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
   if (bn) flush();
 }
 
+// atomicAdd(&ctr[key], 1) for every active lane, returning each lane's old
+// value, with one atomic per distinct key among the first few groups of the
+// wave (lanes with equal keys get consecutive slots); lanes left after the
+// budget use their own atomics.  All lanes of the wave must call it.
+__device__ __forceinline__ uint32_t wave_atomic_slot(uint32_t* __restrict__ ctr, uint32_t key,
+                                                     bool active) {
+  const uint32_t ln = lane_id();
+  bool todo = active;
+  uint32_t slot = 0;
+  uint64_t pending = __ballot(todo);
+  for (int it = 0; pending != 0 && it < 8; ++it) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t k = __shfl(key, leader);
+    const uint64_t grp = __ballot(todo && key == k);
+    uint32_t base = 0;
+    if ((int)ln == leader) base = atomicAdd(&ctr[k], (uint32_t)__popcll(grp));
+    base = __shfl(base, leader);
+    if (todo && key == k) {
+      slot = base + lanes_below(grp);
+      todo = false;
+    }
+    pending &= ~grp;
+  }
+  if (todo) slot = atomicAdd(&ctr[key], 1u);
+  return slot;
+}
+
 // Gated pairs are packed as (i << 40) | (j << 16) | count; a candidate that
 // the exact gate rejects is overwritten with kDeadPair.
 constexpr uint64_t kDeadPair = ~0ull;
@@ -574,21 +601,25 @@ __global__ __launch_bounds__(256) void gate_compare_kernel(
   const uint32_t q = (uint32_t)(gt & 3u);
   const bool live = t < nc;
   const uint64_t e = live ? cand[t] : 0;
-  const uint32_t i = (uint32_t)(e >> 40), j = (uint32_t)(e >> 16) & 0xFFFFFFu;
+  // Row indices are taken as 64-bit fields and masked in 64 bits: with the
+  // 32-bit form hipcc (ROCm 7.2) dropped the 24-bit mask of j in the row
+  // address of some instantiations while keeping it elsewhere.
+  const uint64_t i64 = e >> 40, j64 = (e >> 16) & 0xFFFFFFull;
+  const uint32_t i = (uint32_t)i64, j = (uint32_t)j64;
   const bool has_words = W >= 4 || q < (uint32_t)W;
   float4 ua[NC], ub[NC];
   uint4 pa[2 * WPL], pb[2 * WPL];
   if (live) {
-    const float4* ra = reinterpret_cast<const float4*>(unit + (size_t)i * 16 * NC);
-    const float4* rb = reinterpret_cast<const float4*>(unit + (size_t)j * 16 * NC);
+    const float4* ra = reinterpret_cast<const float4*>(unit + i64 * 16 * NC);
+    const float4* rb = reinterpret_cast<const float4*>(unit + j64 * 16 * NC);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       ua[c] = ra[4 * c + q];
       ub[c] = rb[4 * c + q];
     }
     if (has_words) {
-      const uint4* wa = reinterpret_cast<const uint4*>(planes + (size_t)i * 8u * W + 8u * WPL * q);
-      const uint4* wb = reinterpret_cast<const uint4*>(planes + (size_t)j * 8u * W + 8u * WPL * q);
+      const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W + 8u * WPL * q);
+      const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W + 8u * WPL * q);
 #pragma unroll
       for (int x = 0; x < 2 * WPL; ++x) {
         pa[x] = wa[x];
@@ -643,31 +674,31 @@ __global__ __launch_bounds__(256) void gate_compare_kernel(
   }
   mism += __shfl_xor(mism, 1);
   mism += __shfl_xor(mism, 2);
-  if (live && q == 0) {
-    if (pass) {
-      cand[t] = (e & ~0xFFFFull) | (uint64_t)(32u * W - mism);
-      atomicAdd(&deg[i], 1u);
-      atomicAdd(&deg[j], 1u);
-    } else {
-      cand[t] = kDeadPair;
-    }
-  }
+  const bool add = live && q == 0 && pass;
+  wave_atomic_slot(deg, i, add);
+  wave_atomic_slot(deg, j, add);
+  if (live && q == 0) cand[t] = pass ? (e & ~0xFFFFull) | (uint64_t)(32u * W - mism) : kDeadPair;
 }
 
 // Scatter each gated pair into both rows' candidate lists: entry = (cnt<<32)|id.
+// Candidates come in abundance-ordered clusters, so a wave often holds the
+// same row several times: the slots are claimed with wave-aggregated atomics.
 __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict__ surv, uint64_t ns,
                                                       const uint64_t* __restrict__ off,
                                                       uint32_t* __restrict__ cursor,
                                                       uint64_t* __restrict__ list) {
   const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= ns) return;
-  const uint64_t e = surv[t];
-  if (e == kDeadPair) return;
-  uint32_t i, j, c;
-  unpack_pair(e, i, j, c);
-  const uint64_t hi = (uint64_t)c << 32;
-  list[off[i] + atomicAdd(&cursor[i], 1u)] = hi | j;
-  list[off[j] + atomicAdd(&cursor[j], 1u)] = hi | i;
+  const uint64_t e = t < ns ? surv[t] : kDeadPair;
+  const bool live = e != kDeadPair;
+  uint32_t i = 0, j = 0, c = 0;
+  if (live) unpack_pair(e, i, j, c);
+  const uint32_t si = wave_atomic_slot(cursor, i, live);
+  const uint32_t sj = wave_atomic_slot(cursor, j, live);
+  if (live) {
+    const uint64_t hi = (uint64_t)c << 32;
+    list[off[i] + si] = hi | j;
+    list[off[j] + sj] = hi | i;
+  }
 }
 
 // ---------------------------------------------------------------------------
