@@ -137,6 +137,11 @@ struct rbg_asm {
   std::vector<uint8_t> anynz_h;
   GateQuant gq;  // bound statistics of q8 (the int8 rows live on the device)
   // pipeline intermediates
+  DevBuf bscal, svd, cal_rnd, cal_members, cal_roff;
+  uint32_t np = 0;            // b0 pair draws (uploaded once)
+  bool pairs_ready = false;
+  int cal_regime = 0;         // calibration plan cached for (n, seed, regime)
+  unsigned cal_seed = 0;
   DevBuf wx, planes, pi, pj, pcnt, level, svf, mat, cand, nsurv, keep, fence, scnt, deg, rowmax, off, cursor,
       list, topk_id, topk_cnt, topk_len, row_edges, row_off, efrom, eto, ecnt, escomp, escore,
       scan_tmp, tiles;
@@ -251,26 +256,98 @@ rbg_status run_sketch(rbg_ctx* ctx, rbg_asm* a, uint32_t m, uint64_t seed, bool 
   return RBG_OK;
 }
 
-rbg_status run_baseline(rbg_ctx* ctx, rbg_asm* a, uint32_t m, double* b0) {
+rbg_status ensure_baseline_pairs(rbg_ctx* ctx, rbg_asm* a) {
+  if (a->pairs_ready) return RBG_OK;
   std::vector<uint32_t> pi, pj;
   baseline_pairs(a->n, pi, pj);
-  if (pi.empty()) {
-    *b0 = 0.0;
-    return RBG_OK;
+  a->np = (uint32_t)pi.size();
+  const size_t nb = std::max<size_t>(a->np, 1) * sizeof(uint32_t);
+  RBG_TRY_HIP(ctx, a->pi.ensure(nb));
+  RBG_TRY_HIP(ctx, a->pj.ensure(nb));
+  RBG_TRY_HIP(ctx, a->pcnt.ensure(nb));
+  if (a->np) {
+    RBG_TRY_HIP(ctx, hipMemcpy(a->pi.p, pi.data(), a->np * 4, hipMemcpyHostToDevice));
+    RBG_TRY_HIP(ctx, hipMemcpy(a->pj.p, pj.data(), a->np * 4, hipMemcpyHostToDevice));
   }
-  const uint32_t np = (uint32_t)pi.size();
-  RBG_TRY_HIP(ctx, a->pi.ensure(np * sizeof(uint32_t)));
-  RBG_TRY_HIP(ctx, a->pj.ensure(np * sizeof(uint32_t)));
-  RBG_TRY_HIP(ctx, a->pcnt.ensure(np * sizeof(uint32_t)));
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(a->pi.p, pi.data(), np * 4, hipMemcpyHostToDevice, ctx->stream));
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(a->pj.p, pj.data(), np * 4, hipMemcpyHostToDevice, ctx->stream));
-  RBG_TRY_HIP(ctx, launch_list_counts((const uint32_t*)a->planes.p, words_for(m),
-                                      (const uint32_t*)a->pi.p, (const uint32_t*)a->pj.p, np,
-                                      (uint32_t*)a->pcnt.p, ctx->stream));
-  std::vector<uint32_t> cnt(np);
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(cnt.data(), a->pcnt.p, np * 4, hipMemcpyDeviceToHost, ctx->stream));
-  RBG_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *b0 = baseline_median(cnt, m);
+  a->pairs_ready = true;
+  return RBG_OK;
+}
+
+rbg_status ensure_calib_plan(rbg_ctx* ctx, rbg_asm* a, int regime, unsigned seed) {
+  if (a->cal_regime == regime && a->cal_seed == seed) return RBG_OK;
+  CalibPlan plan;
+  calibration_plan(a->n, seed, regime, plan);
+  RBG_TRY_HIP(ctx, a->cal_rnd.ensure(kCalibRand * sizeof(int32_t)));
+  RBG_TRY_HIP(ctx, a->cal_members.ensure(std::max<size_t>(plan.members.size(), 1) * sizeof(uint32_t)));
+  RBG_TRY_HIP(ctx, a->cal_roff.ensure(11 * sizeof(uint32_t)));
+  plan.round_off.resize(11, plan.round_off.back());
+  RBG_TRY_HIP(ctx, hipMemcpy(a->cal_rnd.p, plan.rnd.data(), kCalibRand * 4, hipMemcpyHostToDevice));
+  if (!plan.members.empty())
+    RBG_TRY_HIP(ctx, hipMemcpy(a->cal_members.p, plan.members.data(), plan.members.size() * 4,
+                               hipMemcpyHostToDevice));
+  RBG_TRY_HIP(ctx, hipMemcpy(a->cal_roff.p, plan.round_off.data(), 11 * 4, hipMemcpyHostToDevice));
+  a->cal_regime = regime;
+  a->cal_seed = seed;
+  return RBG_OK;
+}
+
+// Fused regime: int8 MFMA prefilter of the abundance gate (one host sync for
+// the candidate count), exact gate + register compare in place, per-row
+// candidate lists (off, list).
+rbg_status fused_pair_pass(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, uint32_t S,
+                           double w_comp, uint32_t W, const uint8_t* anynz) {
+  const uint32_t n = a->n;
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  // abundance-first gate (src/impl/rb_graph.cpp:693-706)
+  const bool gate = S > 1 && w_comp < 1.0 && w_comp < P->min_edge_weight;
+  if (gate && S > 64) return ctx->fail(RBG_ERR_UNSUPPORTED, "abundance gate needs S <= 64");
+  const float thr = gate ? (float)((P->min_edge_weight - w_comp) / (1.0 - w_comp) - 1e-4)
+                         : -__builtin_huge_valf();
+  const uint32_t NC = a->NC;
+  const uint32_t acc1_end = gate ? 2 * (S / 32) : 0;
+  // prefilter cut: with the gate off every pair passes (cut = INT32_MIN, thr = -inf)
+  const int32_t qcut = gate ? gate_qcut(a->gq, thr) : INT32_MIN;
+  const uint64_t total_pairs = (uint64_t)n * (n - 1) / 2;
+  uint64_t cap = std::min<uint64_t>(total_pairs, std::max<uint64_t>(1ull << 26, 2048ull * n));
+  RBG_TRY_HIP(ctx, a->nsurv.ensure(2 * sizeof(unsigned long long)));
+  unsigned long long* counters = (unsigned long long*)a->nsurv.p;  // [0] candidates
+  uint64_t nc = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    RBG_TRY_HIP(ctx, a->cand.ensure(std::max<uint64_t>(cap, 1) * sizeof(uint64_t)));
+    RBG_TRY_HIP(ctx, hipMemsetAsync(counters, 0, 2 * sizeof(unsigned long long), st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[2], st));
+    RBG_TRY_HIP(ctx, launch_gate_mfma((const int8_t*)a->q8.p, a->gq.kpad, n, qcut,
+                                      (const uint32_t*)a->perm.p, (const uint32_t*)a->tiles.p,
+                                      a->ntiles, (uint64_t*)a->cand.p, counters, cap, st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ev[6], st));
+    unsigned long long got = 0;
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(&got, counters, sizeof(got), hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+    nc = got;
+    if (nc <= cap) break;
+    cap = nc;  // rerun with room for every candidate
+  }
+  // exact FP32 gate and register compare of the candidates, in place
+  RBG_TRY_HIP(ctx, a->deg.ensure((size_t)n * sizeof(uint32_t)));
+  RBG_TRY_HIP(ctx, a->cursor.ensure((size_t)n * sizeof(uint32_t)));
+  RBG_TRY_HIP(ctx, a->off.ensure((size_t)(n + 1) * sizeof(uint64_t)));
+  RBG_TRY_HIP(ctx, hipMemsetAsync(a->deg.p, 0, (size_t)n * 4, st));
+  RBG_TRY_HIP(ctx, hipMemsetAsync(a->cursor.p, 0, (size_t)n * 4, st));
+  RBG_TRY_HIP(ctx, hipEventRecord(ev[7], st));
+  RBG_TRY_HIP(ctx, launch_gate_compare((uint64_t*)a->cand.p, nc, (const float*)a->unit_pad.p, NC,
+                                       anynz, thr, acc1_end, (const uint32_t*)a->planes.p, W,
+                                       (uint32_t*)a->deg.p, st));
+  RBG_TRY_HIP(ctx, hipEventRecord(ev[8], st));
+  size_t tmp_bytes = 0;
+  RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64(nullptr, nullptr, n, nullptr, tmp_bytes, st));
+  RBG_TRY_HIP(ctx, a->scan_tmp.ensure(tmp_bytes));
+  RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)a->deg.p, (uint64_t*)a->off.p, n,
+                                             a->scan_tmp.p, tmp_bytes, st));
+  // off[n] = 2 x survivors <= 2 x candidates
+  RBG_TRY_HIP(ctx, a->list.ensure(std::max<uint64_t>(2 * nc, 1) * sizeof(uint64_t)));
+  RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)a->cand.p, nc, (const uint64_t*)a->off.p,
+                                  (uint32_t*)a->cursor.p, (uint64_t*)a->list.p, st));
   return RBG_OK;
 }
 
@@ -294,18 +371,21 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   RBG_TRY_HIP(ctx, hipEventRecord(ev[0], st));
   RBG_TRY(run_sketch(ctx, a, m, P->pmh_seed, diag && diag->win16));
   RBG_TRY_HIP(ctx, hipEventRecord(ev[1], st));
-  double b0 = 0.0;
-  RBG_TRY(run_baseline(ctx, a, m, &b0));
-  out->b0 = b0;
-  std::vector<double> svd;
-  std::vector<float> svf;
-  std::vector<uint16_t> level;
-  sv_tables(m, b0, svd, svf, level);
-  const uint32_t nlevels = (uint32_t)level[m] + 1;
-  RBG_TRY_HIP(ctx, a->level.ensure((m + 1) * sizeof(uint16_t)));
+  // b0 and the sv tables on the device (the pair draws depend on n only)
+  RBG_TRY(ensure_baseline_pairs(ctx, a));
+  RBG_TRY_HIP(ctx, a->bscal.ensure(sizeof(BuildScalars)));
+  BuildScalars* bs = (BuildScalars*)a->bscal.p;
+  RBG_TRY_HIP(ctx, a->svd.ensure((m + 1) * sizeof(double)));
   RBG_TRY_HIP(ctx, a->svf.ensure((m + 1) * sizeof(float)));
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(a->level.p, level.data(), (m + 1) * 2, hipMemcpyHostToDevice, st));
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(a->svf.p, svf.data(), (m + 1) * 4, hipMemcpyHostToDevice, st));
+  RBG_TRY_HIP(ctx, a->level.ensure((m + 1) * sizeof(uint16_t)));
+  if (a->np)
+    RBG_TRY_HIP(ctx, launch_list_counts((const uint32_t*)a->planes.p, W, (const uint32_t*)a->pi.p,
+                                        (const uint32_t*)a->pj.p, a->np, (uint32_t*)a->pcnt.p, st));
+  RBG_TRY_HIP(ctx, launch_baseline_tables((const uint32_t*)a->pcnt.p, a->np, m, bs,
+                                          (double*)a->svd.p, (float*)a->svf.p,
+                                          (uint16_t*)a->level.p, st));
+  // level bins above the number of distinct similarities stay empty
+  const uint32_t nlevels = m + 1;
   // +16 ids: the mutual test reads whole 16-id segments of a row's list
   RBG_TRY_HIP(ctx, a->topk_id.ensure(((size_t)n * K + 16) * sizeof(uint32_t)));
   const uint32_t FW = fence_words(K);
@@ -316,7 +396,6 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
   const uint8_t* anynz = (S > 0 && a->has_anynz) ? (const uint8_t*)a->anynz.p : nullptr;
 
   RBG_TRY_HIP(ctx, hipEventRecord(ev[2], st));
-  uint64_t off_total = 0;  // fused regime: 2 x gated pairs
   if (regime == 1) {
     RBG_TRY_HIP(ctx, a->mat.ensure((size_t)n * n * sizeof(uint16_t)));
     RBG_TRY_HIP(ctx, launch_dense_counts((const uint32_t*)a->planes.p, W, n, (uint16_t*)a->mat.p, st));
@@ -332,58 +411,7 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
                                          (uint32_t*)a->fence.p, FW, st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[10], st));
   } else {
-    // abundance-first gate (src/impl/rb_graph.cpp:693-706)
-    const bool gate = S > 1 && w_comp < 1.0 && w_comp < P->min_edge_weight;
-    if (gate && S > 64) return ctx->fail(RBG_ERR_UNSUPPORTED, "abundance gate needs S <= 64");
-    const float thr = gate ? (float)((P->min_edge_weight - w_comp) / (1.0 - w_comp) - 1e-4)
-                           : -__builtin_huge_valf();
-    const uint32_t NC = a->NC;
-    const uint32_t acc1_end = gate ? 2 * (S / 32) : 0;
-    // prefilter cut: with the gate off every pair passes (cut = INT32_MIN, thr = -inf)
-    const int32_t qcut = gate ? gate_qcut(a->gq, thr) : INT32_MIN;
-    const uint64_t total_pairs = (uint64_t)n * (n - 1) / 2;
-    uint64_t cap = std::min<uint64_t>(total_pairs, std::max<uint64_t>(1ull << 26, 2048ull * n));
-    RBG_TRY_HIP(ctx, a->nsurv.ensure(2 * sizeof(unsigned long long)));
-    unsigned long long* counters = (unsigned long long*)a->nsurv.p;  // [0] candidates
-    uint64_t nc = 0;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      RBG_TRY_HIP(ctx, a->cand.ensure(std::max<uint64_t>(cap, 1) * sizeof(uint64_t)));
-      RBG_TRY_HIP(ctx, hipMemsetAsync(counters, 0, 2 * sizeof(unsigned long long), st));
-      RBG_TRY_HIP(ctx, hipEventRecord(ev[2], st));
-      RBG_TRY_HIP(ctx, launch_gate_mfma((const int8_t*)a->q8.p, a->gq.kpad, n, qcut,
-                                        (const uint32_t*)a->perm.p,
-                                        (const uint32_t*)a->tiles.p, a->ntiles,
-                                        (uint64_t*)a->cand.p, counters, cap, st));
-      RBG_TRY_HIP(ctx, hipEventRecord(ev[6], st));
-      unsigned long long got = 0;
-      RBG_TRY_HIP(ctx, hipMemcpyAsync(&got, counters, sizeof(got), hipMemcpyDeviceToHost, st));
-      RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
-      nc = got;
-      if (nc <= cap) break;
-      cap = nc;  // rerun with room for every candidate
-    }
-    // exact FP32 gate and register compare of the candidates, in place
-    RBG_TRY_HIP(ctx, a->deg.ensure((size_t)n * sizeof(uint32_t)));
-    RBG_TRY_HIP(ctx, a->cursor.ensure((size_t)n * sizeof(uint32_t)));
-    RBG_TRY_HIP(ctx, a->off.ensure((size_t)(n + 1) * sizeof(uint64_t)));
-    RBG_TRY_HIP(ctx, hipMemsetAsync(a->deg.p, 0, (size_t)n * 4, st));
-    RBG_TRY_HIP(ctx, hipMemsetAsync(a->cursor.p, 0, (size_t)n * 4, st));
-    RBG_TRY_HIP(ctx, hipEventRecord(ev[7], st));
-    RBG_TRY_HIP(ctx, launch_gate_compare((uint64_t*)a->cand.p, nc, (const float*)a->unit_pad.p, NC,
-                                         anynz, thr, acc1_end, (const uint32_t*)a->planes.p, W,
-                                         (uint32_t*)a->deg.p, st));
-    RBG_TRY_HIP(ctx, hipEventRecord(ev[8], st));
-    size_t tmp_bytes = 0;
-    RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64(nullptr, nullptr, n, nullptr, tmp_bytes, st));
-    RBG_TRY_HIP(ctx, a->scan_tmp.ensure(tmp_bytes));
-    RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)a->deg.p, (uint64_t*)a->off.p, n,
-                                               a->scan_tmp.p, tmp_bytes, st));
-    // off[n] = 2 x survivors <= 2 x candidates
-    RBG_TRY_HIP(ctx, a->list.ensure(std::max<uint64_t>(2 * nc, 1) * sizeof(uint64_t)));
-    RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)a->cand.p, nc, (const uint64_t*)a->off.p,
-                                    (uint32_t*)a->cursor.p, (uint64_t*)a->list.p, st));
-    RBG_TRY_HIP(ctx, hipMemcpyAsync(&off_total, (const uint64_t*)a->off.p + n, sizeof(uint64_t),
-                                    hipMemcpyDeviceToHost, st));
+    RBG_TRY(fused_pair_pass(ctx, a, P, S, w_comp, W, anynz));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[3], st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[9], st));
     RBG_TRY_HIP(ctx, launch_select_list((const uint64_t*)a->list.p, (const uint64_t*)a->off.p, n,
@@ -393,62 +421,65 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
                                         (uint32_t*)a->fence.p, FW, st));
     RBG_TRY_HIP(ctx, hipEventRecord(ev[10], st));
   }
-  // calibration on the host (glibc rand stream)
-  std::vector<uint32_t> rowmax(n);
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(rowmax.data(), a->rowmax.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
-  if (regime == 2) out->gated_pairs = off_total / 2;
-  const double sim_cutoff = regime == 1 ? calibrate_small(rowmax, svd, P->coverage, P->rand_seed)
-                                        : calibrate_fused(rowmax, svf, P->coverage, P->rand_seed);
-  out->sim_cutoff = sim_cutoff;
-  const double cutoff = sim_cutoff / 1000.;
-  uint32_t cmin = m + 1;
-  for (uint32_t c = 0; c <= m; ++c)
-    if ((double)svf[c] > cutoff) {
-      cmin = c;
-      break;
-    }
+  // calibration on the device: the glibc rand() draws are prepared per (n, seed)
+  RBG_TRY(ensure_calib_plan(ctx, a, regime, P->rand_seed));
+  RBG_TRY_HIP(ctx, launch_calibrate((const uint32_t*)a->rowmax.p, n, m, (const double*)a->svd.p,
+                                    (const float*)a->svf.p, (const int32_t*)a->cal_rnd.p, kCalibRand,
+                                    (const uint32_t*)a->cal_members.p,
+                                    (const uint32_t*)a->cal_roff.p, regime, P->coverage, bs, st));
   RBG_TRY_HIP(ctx, a->row_edges.ensure((size_t)n * 4));
   RBG_TRY_HIP(ctx, a->row_off.ensure((size_t)(n + 1) * 8));
   const int mutual = regime == 2 ? 1 : 0;
   RBG_TRY_HIP(ctx, a->keep.ensure((size_t)n * ((K + 63) / 64) * sizeof(uint64_t)));
   RBG_TRY_HIP(ctx, launch_emit_mark((const uint32_t*)a->topk_id.p, (const uint16_t*)a->topk_cnt.p,
                                     (const uint32_t*)a->topk_len.p, (const uint32_t*)a->fence.p,
-                                    FW, n, K, cmin, mutual, (uint64_t*)a->keep.p,
+                                    FW, n, K, bs, mutual, (uint64_t*)a->keep.p,
                                     (uint32_t*)a->row_edges.p, st));
   size_t tmp_bytes = 0;
   RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64(nullptr, nullptr, n, nullptr, tmp_bytes, st));
   RBG_TRY_HIP(ctx, a->scan_tmp.ensure(tmp_bytes));
   RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)a->row_edges.p, (uint64_t*)a->row_off.p,
                                              n, a->scan_tmp.p, tmp_bytes, st));
-  uint64_t ne = 0;
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(&ne, (uint64_t*)a->row_off.p + n, 8, hipMemcpyDeviceToHost, st));
-  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
-  RBG_TRY_HIP(ctx, a->efrom.ensure(std::max<uint64_t>(ne, 1) * 4));
-  RBG_TRY_HIP(ctx, a->eto.ensure(std::max<uint64_t>(ne, 1) * 4));
-  RBG_TRY_HIP(ctx, a->ecnt.ensure(std::max<uint64_t>(ne, 1) * 2));
-  RBG_TRY_HIP(ctx, a->escomp.ensure(std::max<uint64_t>(ne, 1) * 4));
+  // edge arrays at their capacity (n*K); the edge count stays on the device
+  const uint64_t* ne_dev = (const uint64_t*)a->row_off.p + n;
+  const uint64_t cap_e = (uint64_t)n * K;
+  RBG_TRY_HIP(ctx, a->efrom.ensure(cap_e * 4));
+  RBG_TRY_HIP(ctx, a->eto.ensure(cap_e * 4));
+  RBG_TRY_HIP(ctx, a->ecnt.ensure(cap_e * 2));
+  RBG_TRY_HIP(ctx, a->escomp.ensure(cap_e * 4));
   RBG_TRY_HIP(ctx, launch_emit_write((const uint32_t*)a->topk_id.p, (const uint16_t*)a->topk_cnt.p,
                                      (const uint32_t*)a->topk_len.p, n, K,
                                      (const uint64_t*)a->keep.p, (const uint64_t*)a->row_off.p,
                                      (uint32_t*)a->efrom.p, (uint32_t*)a->eto.p,
                                      (uint16_t*)a->ecnt.p, st));
-  RBG_TRY_HIP(ctx, launch_cnt_to_sv((const uint16_t*)a->ecnt.p, ne, (const float*)a->svf.p,
-                                    (float*)a->escomp.p, st));
+  RBG_TRY_HIP(ctx, launch_cnt_to_sv((const uint16_t*)a->ecnt.p, cap_e, (const float*)a->svf.p,
+                                    (float*)a->escomp.p, st, ne_dev));
   RBG_TRY_HIP(ctx, hipEventRecord(ev[4], st));
   if (S > 0) {
-    RBG_TRY_HIP(ctx, a->escore.ensure(std::max<uint64_t>(ne, 1) * 4));
+    RBG_TRY_HIP(ctx, a->escore.ensure(cap_e * 4));
     const bool padded = S > 1 && S <= 64;  // unit_pad / raw_pad hold the rows
     RBG_TRY_HIP(ctx, launch_edge_weights(
                          (const uint32_t*)a->efrom.p, (const uint32_t*)a->eto.p,
-                         (const float*)a->escomp.p, ne,
+                         (const float*)a->escomp.p, cap_e,
                          (const float*)(padded ? a->unit_pad.p : a->unit.p),
                          (const float*)(padded ? a->raw_pad.p : a->raw.p),
                          S > 1 ? (const double*)a->colnorm.p : nullptr, S, w_comp,
                          P->min_edge_weight, P->neg_depth_thr, (float*)a->escore.p, nullptr, st,
-                         padded));
+                         padded, ne_dev));
   }
   RBG_TRY_HIP(ctx, hipEventRecord(ev[5], st));
+  // the build's scalars back to the host
+  BuildScalars hb{};
+  uint64_t ne = 0, off_total = 0;
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(&hb, bs, sizeof(hb), hipMemcpyDeviceToHost, st));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(&ne, ne_dev, 8, hipMemcpyDeviceToHost, st));
+  if (regime == 2)
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(&off_total, (const uint64_t*)a->off.p + n, 8,
+                                    hipMemcpyDeviceToHost, st));
+  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+  out->b0 = hb.b0;
+  out->sim_cutoff = hb.sim_cutoff;
+  out->gated_pairs = off_total / 2;
   out->n_edges = ne;
   a->last_edges = ne;
   if (!(P->flags & RBG_EDGES_ON_DEVICE)) {  // results to the host
@@ -472,7 +503,8 @@ rbg_status build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params* P, rbg_
       RBG_TRY_HIP(ctx, hipMemcpyAsync(diag->topk_cnt, a->topk_cnt.p, (size_t)n * K * 2, hipMemcpyDeviceToHost, st));
     if (diag->topk_len)
       RBG_TRY_HIP(ctx, hipMemcpyAsync(diag->topk_len, a->topk_len.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-    if (diag->rowmax_cnt) std::memcpy(diag->rowmax_cnt, rowmax.data(), (size_t)n * 4);
+    if (diag->rowmax_cnt)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(diag->rowmax_cnt, a->rowmax.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
   }
   RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
   if (diag && diag->win16) {

@@ -464,6 +464,35 @@ double calibrate_fused(const std::vector<uint32_t>& rowmax, const std::vector<fl
   return sum_p / (double)R;
 }
 
+void calibration_plan(uint32_t n, unsigned seed, int regime, CalibPlan& plan) {
+  plan.n = n;
+  plan.seed = seed;
+  plan.regime = regime;
+  plan.members.clear();
+  plan.round_off.assign(1, 0);
+  RandStream rs(seed);
+  if (regime == 1) {
+    for (uint32_t t = 0; t < n; ++t) rs.next();
+  } else {
+    constexpr size_t R = 10, SAMP = 2500;
+    std::vector<int8_t> round_of(n, -1);
+    std::vector<size_t> idx(n);
+    for (size_t r = 0; r < R; ++r) {
+      std::iota(idx.begin(), idx.end(), 0);
+      draw_prefix(idx, SAMP, rs);
+      for (size_t li = 0; li < SAMP; ++li)
+        if (round_of[idx[li]] < 0) round_of[idx[li]] = (int8_t)r;
+    }
+    for (size_t r = 0; r < R; ++r) {
+      for (uint32_t c = 0; c < n; ++c)
+        if (round_of[c] == (int8_t)r) plan.members.push_back(c);
+      plan.round_off.push_back((uint32_t)plan.members.size());
+    }
+  }
+  plan.rnd.resize(kCalibRand);
+  for (auto& v : plan.rnd) v = rs.next();
+}
+
 void baseline_pairs(size_t n, std::vector<uint32_t>& pi, std::vector<uint32_t>& pj) {
   pi.clear();
   pj.clear();

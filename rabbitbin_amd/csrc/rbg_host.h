@@ -95,6 +95,22 @@ double calibrate_fused(const std::vector<uint32_t>& rowmax, const std::vector<fl
 // mt19937_64 + uniform_int_distribution, src/rabbitbin.cpp:1340-1352);
 // pairs with i == j are dropped as in the reference.
 void baseline_pairs(size_t n, std::vector<uint32_t>& pi, std::vector<uint32_t>& pj);
+
+// The data-independent part of the calibration of n contigs under
+// srand(seed): the contig draws (regime 1: n rand() calls whose order does
+// not matter; regime 2: 10 rounds of a 2500-element Fisher-Yates prefix,
+// src/impl/rb_graph.cpp:502-522) and the kCalibRand rand() values that
+// follow them, which calib_converge consumes (at most ~135 per round).
+constexpr uint32_t kCalibRand = 2048;
+struct CalibPlan {
+  uint32_t n = 0;
+  unsigned seed = 0;
+  int regime = 0;
+  std::vector<int32_t> rnd;          // kCalibRand values
+  std::vector<uint32_t> members;     // regime 2: contigs by lowest sampled round
+  std::vector<uint32_t> round_off;   // regime 2: 11 offsets into members
+};
+void calibration_plan(uint32_t n, unsigned seed, int regime, CalibPlan& plan);
 double baseline_median(const std::vector<uint32_t>& counts, uint32_t m);
 
 }  // namespace rbg

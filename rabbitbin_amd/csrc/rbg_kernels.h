@@ -8,6 +8,27 @@
 
 namespace rbg {
 
+// Scalars of one graph build, kept in device memory and written by the
+// kernels that compute them (b0 and the sv tables, then the calibration).
+struct BuildScalars {
+  double b0;
+  double sim_cutoff;   // per-mille, as the reference's simCutoff
+  uint32_t cmin;       // first count whose float similarity exceeds simCutoff / 1000
+  uint32_t nlevels;    // distinct float similarities
+  uint32_t rand_used;  // rand() values consumed by calib_converge
+  uint32_t pad;
+};
+
+// b0 from the sampled pairs' counts, then svd/svf/level for counts 0..m.
+hipError_t launch_baseline_tables(const uint32_t* pcnt, uint32_t np, uint32_t m, BuildScalars* bs,
+                                  double* svd, float* svf, uint16_t* level, hipStream_t st);
+// simCutoff and cmin from the per-row maxima (regime 1: all n rows; regime 2:
+// the round members), consuming rnd[] in order.
+hipError_t launch_calibrate(const uint32_t* rowmax, uint32_t n, uint32_t m, const double* svd,
+                            const float* svf, const int32_t* rnd, uint32_t nrnd,
+                            const uint32_t* members, const uint32_t* round_off, int regime,
+                            double coverage, BuildScalars* bs, hipStream_t st);
+
 hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
                          uint32_t W, const double* sT, const uint8_t* sX, const uint8_t* cls,
                          const uint8_t* codes,
@@ -48,8 +69,8 @@ inline uint32_t fence_words(uint32_t K) { return 16u * (((K + 15u) / 16u + 15u) 
 // and counts them per row; pass 1 writes them at the rows' offsets.
 hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
                             const uint32_t* topk_len, const uint32_t* fence, uint32_t FW,
-                            uint32_t n, uint32_t K, uint32_t cmin, int mutual, uint64_t* keep,
-                            uint32_t* row_edges, hipStream_t st);
+                            uint32_t n, uint32_t K, const BuildScalars* bs, int mutual,
+                            uint64_t* keep, uint32_t* row_edges, hipStream_t st);
 hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
                              const uint32_t* topk_len, uint32_t n, uint32_t K, const uint64_t* keep,
                              const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
@@ -58,10 +79,11 @@ hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const
                                uint64_t ne, const float* unit, const float* raw,
                                const double* colnorm, uint32_t S, double w_comp, double min_edge,
                                double neg_thr, float* score, double* score64, hipStream_t st,
-                               bool padded = false);
+                               bool padded = false, const uint64_t* ne_dev = nullptr);
 // padded: unit and raw rows are zero-padded to 16*ceil(S/16) floats (S <= 64).
+// ne_dev (optional): device-resident element count; ne is then the capacity.
 hipError_t launch_cnt_to_sv(const uint16_t* c, uint64_t ne, const float* svf, float* out,
-                            hipStream_t st);
+                            hipStream_t st, const uint64_t* ne_dev = nullptr);
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n, void* tmp,
                                      size_t& tmp_bytes, hipStream_t st);
 

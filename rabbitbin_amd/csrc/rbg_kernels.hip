@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "rbg_kernels.h"
 
 namespace rbg {
@@ -1049,9 +1051,10 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
 __global__ __launch_bounds__(256) void emit_mark_kernel(
     const uint32_t* __restrict__ topk_id, const uint16_t* __restrict__ topk_cnt,
     const uint32_t* __restrict__ topk_len, const uint32_t* __restrict__ fence, uint32_t FW,
-    uint32_t n, uint32_t K, uint32_t cmin, int mutual, uint32_t KW, uint64_t* __restrict__ keep,
-    uint32_t* __restrict__ row_edges) {
+    uint32_t n, uint32_t K, const BuildScalars* __restrict__ bs, int mutual, uint32_t KW,
+    uint64_t* __restrict__ keep, uint32_t* __restrict__ row_edges) {
   const bool aligned16 = (K & 3u) == 0;
+  const uint32_t cmin = bs->cmin;
   const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t ln = threadIdx.x & 63u;
   if (i >= n) return;
@@ -1143,6 +1146,11 @@ __global__ __launch_bounds__(256) void emit_write_kernel(
   }
 }
 
+// Element count: *ne_dev when given (device-resident), else ne.
+__device__ __forceinline__ uint64_t count_of(const uint64_t* ne_dev, uint64_t ne) {
+  return ne_dev ? *ne_dev : ne;
+}
+
 // Stage (d): per-edge weight in FP64 (the reference's depth_edge_term with
 // the default fuse = min, src/rabbitbin.cpp:930-970, and the combination and
 // cutoff of src/rabbitbin.cpp:5500-5507).
@@ -1151,9 +1159,10 @@ __global__ __launch_bounds__(256) void edge_weight_kernel(
     const float* __restrict__ scomp, uint64_t ne, const float* __restrict__ unit,
     const float* __restrict__ raw, const double* __restrict__ colnorm, uint32_t S,
     double w_comp, double min_edge, double neg_thr, float* __restrict__ score,
-    double* __restrict__ score64) {
-  const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= ne) return;
+    double* __restrict__ score64, const uint64_t* __restrict__ ne_dev) {
+  const uint64_t cnt = count_of(ne_dev, ne);
+  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt;
+       e += (uint64_t)gridDim.x * 256) {
   const uint32_t i = efrom[e], j = eto[e];
   double w;
   if (S == 0) {
@@ -1197,6 +1206,7 @@ __global__ __launch_bounds__(256) void edge_weight_kernel(
   }
   score[e] = (float)w;
   if (score64) score64[e] = w;
+  }
 }
 
 // The same weight for 1 < S <= 16*NC with unit and raw rows zero-padded to
@@ -1208,10 +1218,11 @@ __global__ __launch_bounds__(256) void edge_weight_pad_kernel(
     const float* __restrict__ scomp, uint64_t ne, const float* __restrict__ unitp,
     const float* __restrict__ rawp, const double* __restrict__ colnorm, uint32_t S,
     double w_comp, double min_edge, double neg_thr, float* __restrict__ score,
-    double* __restrict__ score64) {
+    double* __restrict__ score64, const uint64_t* __restrict__ ne_dev) {
   constexpr int V = 4 * NC;  // float4 per row
-  const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= ne) return;
+  const uint64_t cnt = count_of(ne_dev, ne);
+  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt;
+       e += (uint64_t)gridDim.x * 256) {
   const uint32_t i = efrom[e], j = eto[e];
   const float4* ua = reinterpret_cast<const float4*>(unitp + (size_t)i * 16 * NC);
   const float4* ub = reinterpret_cast<const float4*>(unitp + (size_t)j * 16 * NC);
@@ -1265,14 +1276,217 @@ __global__ __launch_bounds__(256) void edge_weight_pad_kernel(
   }
   score[e] = (float)w;
   if (score64) score64[e] = w;
+  }
 }
 
 __global__ void u16_to_sv_kernel(const uint16_t* __restrict__ c, uint64_t ne,
+                                 const uint64_t* __restrict__ ne_dev,
                                  const float* __restrict__ svf, float* __restrict__ out) {
-  const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e < ne) out[e] = svf[c[e]];
+  const uint64_t cnt = count_of(ne_dev, ne);
+  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (uint64_t)gridDim.x * 256)
+    out[e] = svf[c[e]];
 }
 
+
+// ---------------------------------------------------------------------------
+// Build scalars on the device, so a graph build needs no host round trip
+// between its stages.
+//
+// baseline_tables_kernel: b0 = the median of (float)(cnt * (1/m)) over the
+// sampled pairs (src/rabbitbin.cpp:1340-1369); cnt -> float is monotone, so
+// the median value is that of the median count, found in a count histogram.
+// Then the sv tables of pmh_sv_from_count (src/rabbitbin.cpp:1297-1305) in
+// FP64 (IEEE division and no contraction, as the host build) and the level
+// of each count (dense rank of the distinct float similarities).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void baseline_tables_kernel(const uint32_t* __restrict__ pcnt,
+                                                              uint32_t np, uint32_t m,
+                                                              BuildScalars* __restrict__ bs,
+                                                              double* __restrict__ svd,
+                                                              float* __restrict__ svf,
+                                                              uint16_t* __restrict__ level) {
+  __shared__ uint32_t hist[513];
+  __shared__ float sf[513];
+  __shared__ double b0s;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t c = tid; c <= m; c += 256) hist[c] = 0;
+  __syncthreads();
+  for (uint32_t t = tid; t < np; t += 256) atomicAdd(&hist[pcnt[t]], 1u);
+  __syncthreads();
+  const double inv_m = 1.0 / (double)m;
+  if (tid == 0) {
+    double b0 = 0.0;
+    if (np > 0) {
+      const uint32_t k = np / 2;  // vals[size / 2] after nth_element
+      uint32_t cum = 0, c = 0;
+      for (; c < m; ++c) {
+        cum += hist[c];
+        if (cum > k) break;
+      }
+      b0 = (double)(float)((double)c * inv_m);
+    }
+    b0s = b0;
+    bs->b0 = b0;
+  }
+  __syncthreads();
+  const double b0 = b0s;
+  const bool rescale = b0 > 0.0 && b0 < 1.0;
+  const double inv1mb0 = rescale ? 1.0 / (1.0 - b0) : 1.0;
+  for (uint32_t c = tid; c <= m; c += 256) {
+    double j = (double)c * inv_m;
+    if (rescale) {
+      j = (j - b0) * inv1mb0;
+      if (j < 0.0) j = 0.0;
+    }
+    if (j > 1.0 - 1e-6) j = 1.0 - 1e-6;
+    svd[c] = j;
+    svf[c] = (float)j;
+    sf[c] = (float)j;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint16_t lv = 0;
+    for (uint32_t c = 0; c <= m; ++c) {
+      if (c > 0 && sf[c] != sf[c - 1]) ++lv;
+      level[c] = lv;
+    }
+    bs->nlevels = (uint32_t)lv + 1;
+  }
+}
+
+// calibrate_kernel: simCutoff from the per-row maximum counts, as
+// calibrate_sim_cutoff(coverage, true) (regime 1, src/impl/rb_graph.cpp:
+// 1362-1384) or the 10 x 2500 sampled calibration of gen_fused_calib_graph
+// (regime 2, :502-525, 1243-1254).  The contig draws of both depend only on n
+// and the srand() seed and are done on the host: `rnd` holds the rand()
+// values that follow them (consumed by calib_converge, :390-415) and, in
+// regime 2, members[round_off[r] .. round_off[r+1]) are the contigs whose
+// lowest sampled round is r (their maxima count in round r; the other slots
+// of a round hold 0).  maxsim values are svd[cnt] (regime 1) or (double)
+// svf[cnt] (regime 2) and increase with cnt, so the number of contigs at or
+// above a cutoff is a suffix sum of a count histogram.  Also writes cmin, the
+// first count whose float similarity exceeds simCutoff / 1000.
+// Block-wide exclusive scan (256 threads; defined with the scan kernels).
+__device__ uint64_t block_scan_excl_u64(uint64_t v, uint64_t* wave_tot, uint64_t* total);
+
+// calib_converge over a table on_at[p] = contigs whose maxsim >= p / 1000.
+__device__ uint32_t calib_converge_dev(const uint32_t* on_at, double total, double coverage,
+                                       const int32_t* rnd, uint32_t nrnd, uint32_t& ri) {
+  uint32_t p = 999, pp = 1000;
+  double pcov = 0.0;
+  while (p > 50) {
+    const double cov = (double)on_at[p] / total;
+    if (cov >= coverage) {
+      if (cov - coverage > coverage - pcov) p = pp;
+      break;
+    }
+    pp = p;
+    pcov = cov;
+    const int r = (ri < nrnd ? rnd[ri] : 0) % 3;
+    ++ri;
+    p -= (uint32_t)(p > 990 ? r + 1 : p > 900 ? r + 3 : r + 9);
+  }
+  return p;
+}
+
+__global__ __launch_bounds__(256) void calibrate_kernel(
+    const uint32_t* __restrict__ rowmax, uint32_t n, uint32_t m, const double* __restrict__ svd,
+    const float* __restrict__ svf, const int32_t* __restrict__ rnd, uint32_t nrnd,
+    const uint32_t* __restrict__ members, const uint32_t* __restrict__ round_off, int regime,
+    double coverage, BuildScalars* __restrict__ bs) {
+  __shared__ uint32_t hist[514];
+  __shared__ uint32_t on_at[1000];
+  __shared__ double val[513];
+  __shared__ uint64_t wave_tot[4], tot;
+  __shared__ uint32_t ri_s;
+  __shared__ double sum_p;
+  __shared__ bool stop;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t c = tid; c <= m; c += 256) val[c] = regime == 1 ? svd[c] : (double)svf[c];
+  if (tid == 0) {
+    ri_s = 0;
+    sum_p = 0.0;
+    stop = false;
+  }
+  const uint32_t rounds = regime == 1 ? 1 : 10;
+  const uint32_t per = (m + 1 + 255) / 256;  // histogram bins per thread (<= 3)
+  for (uint32_t r = 0; r < rounds; ++r) {
+    __syncthreads();
+    if (stop) break;
+    for (uint32_t c = tid; c <= m + 1; c += 256) hist[c] = 0;
+    __syncthreads();
+    if (regime == 1) {
+      if (n > 1)
+        for (uint32_t t = tid; t < n; t += 256) atomicAdd(&hist[rowmax[t]], 1u);
+    } else {
+      for (uint32_t t = round_off[r] + tid; t < round_off[r + 1]; t += 256)
+        atomicAdd(&hist[rowmax[members[t]]], 1u);
+    }
+    __syncthreads();
+    // suffix sums: hist[c] <- contigs with count >= c (hist[m+1] = 0)
+    uint32_t loc[3] = {0, 0, 0}, mine = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+      const uint32_t c = tid * per + k;
+      loc[k] = c <= m ? hist[c] : 0u;
+      mine += loc[k];
+    }
+    const uint64_t before = block_scan_excl_u64(mine, wave_tot, &tot);  // bins < tid*per
+    __syncthreads();
+    const uint64_t total_in = tot;
+    uint64_t run = before;
+    for (uint32_t k = 0; k < per; ++k) {
+      const uint32_t c = tid * per + k;
+      if (c <= m) hist[c] = (uint32_t)(total_in - run);
+      run += loc[k];
+    }
+    __syncthreads();
+    // on_at[p]: the first count whose maxsim reaches p / 1000, then its suffix
+    for (uint32_t p = tid; p < 1000; p += 256) {
+      const double cut = (double)p / 1000.;
+      uint32_t lo = 0, hi = m + 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (val[mid] >= cut)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      on_at[p] = hist[lo];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t ri = ri_s;
+      const double total = regime == 1 ? (double)n : 2500.0;
+      const uint32_t p = calib_converge_dev(on_at, total, coverage, rnd, nrnd, ri);
+      ri_s = ri;
+      if (regime == 1) {
+        sum_p = (double)p;
+      } else {
+        double mp = (double)p;
+        if (mp < 51.0) mp = 50.0;
+        sum_p += mp;
+        if (r == 1 && sum_p / 2 < 51.0) {
+          sum_p = 50.0 * 10.0;
+          stop = true;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double sim_cutoff = regime == 1 ? sum_p : sum_p / 10.0;
+    const double cutoff = sim_cutoff / 1000.;
+    uint32_t cmin = m + 1;
+    for (uint32_t c = 0; c <= m; ++c)
+      if ((double)svf[c] > cutoff) {
+        cmin = c;
+        break;
+      }
+    bs->sim_cutoff = sim_cutoff;
+    bs->cmin = cmin;
+    bs->rand_used = ri_s;
+  }
+}
 
 // Exclusive prefix sum u32 -> u64 in three steps: per-block sums (1024
 // elements per block), a one-block scan of the block sums, then a per-block
@@ -1486,11 +1700,11 @@ hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_
 
 hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
                             const uint32_t* topk_len, const uint32_t* fence, uint32_t FW,
-                            uint32_t n, uint32_t K, uint32_t cmin, int mutual, uint64_t* keep,
-                            uint32_t* row_edges, hipStream_t st) {
+                            uint32_t n, uint32_t K, const BuildScalars* bs, int mutual,
+                            uint64_t* keep, uint32_t* row_edges, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(emit_mark_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt,
-                     topk_len, fence, FW, n, K, cmin, mutual, (K + 63) / 64, keep, row_edges);
+                     topk_len, fence, FW, n, K, bs, mutual, (K + 63) / 64, keep, row_edges);
   return hipGetLastError();
 }
 
@@ -1508,14 +1722,15 @@ hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const
                                uint64_t ne, const float* unit, const float* raw,
                                const double* colnorm, uint32_t S, double w_comp, double min_edge,
                                double neg_thr, float* score, double* score64, hipStream_t st,
-                               bool padded) {
+                               bool padded, const uint64_t* ne_dev) {
   if (ne == 0) return hipSuccess;
-  const dim3 grid((unsigned)((ne + 255) / 256));
+  // with a device-resident count, ne is the capacity: a grid-stride launch
+  const dim3 grid((unsigned)std::min<uint64_t>((ne + 255) / 256, ne_dev ? 4096 : UINT32_MAX));
   const uint32_t NC = (S + 15) / 16;
   if (padded && S > 1 && NC <= 4) {
 #define RBG_EW(NCV)                                                                               \
   hipLaunchKernelGGL(edge_weight_pad_kernel<NCV>, grid, dim3(256), 0, st, efrom, eto, scomp, ne, \
-                     unit, raw, colnorm, S, w_comp, min_edge, neg_thr, score, score64)
+                     unit, raw, colnorm, S, w_comp, min_edge, neg_thr, score, score64, ne_dev)
     switch (NC) {
       case 1: RBG_EW(1); break;
       case 2: RBG_EW(2); break;
@@ -1526,15 +1741,15 @@ hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const
     return hipGetLastError();
   }
   hipLaunchKernelGGL(edge_weight_kernel, grid, dim3(256), 0, st, efrom, eto, scomp, ne, unit, raw,
-                     colnorm, S, w_comp, min_edge, neg_thr, score, score64);
+                     colnorm, S, w_comp, min_edge, neg_thr, score, score64, ne_dev);
   return hipGetLastError();
 }
 
 hipError_t launch_cnt_to_sv(const uint16_t* c, uint64_t ne, const float* svf, float* out,
-                            hipStream_t st) {
+                            hipStream_t st, const uint64_t* ne_dev) {
   if (ne == 0) return hipSuccess;
-  hipLaunchKernelGGL(u16_to_sv_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, c, ne,
-                     svf, out);
+  const dim3 grid((unsigned)std::min<uint64_t>((ne + 255) / 256, ne_dev ? 4096 : UINT32_MAX));
+  hipLaunchKernelGGL(u16_to_sv_kernel, grid, dim3(256), 0, st, c, ne, ne_dev, svf, out);
   return hipGetLastError();
 }
 
@@ -1550,6 +1765,23 @@ hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t
   hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nb), dim3(256), 0, st, in, n, sums);
   hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, st, sums, nb);
   hipLaunchKernelGGL(scan_final_kernel, dim3(nb), dim3(256), 0, st, in, n, sums, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_baseline_tables(const uint32_t* pcnt, uint32_t np, uint32_t m, BuildScalars* bs,
+                                  double* svd, float* svf, uint16_t* level, hipStream_t st) {
+  if (m < 2 || m > 512) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(baseline_tables_kernel, dim3(1), dim3(256), 0, st, pcnt, np, m, bs, svd, svf, level);
+  return hipGetLastError();
+}
+
+hipError_t launch_calibrate(const uint32_t* rowmax, uint32_t n, uint32_t m, const double* svd,
+                            const float* svf, const int32_t* rnd, uint32_t nrnd,
+                            const uint32_t* members, const uint32_t* round_off, int regime,
+                            double coverage, BuildScalars* bs, hipStream_t st) {
+  if (m < 2 || m > 512 || (regime != 1 && regime != 2)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(calibrate_kernel, dim3(1), dim3(256), 0, st, rowmax, n, m, svd, svf, rnd, nrnd,
+                     members, round_off, regime, coverage, bs);
   return hipGetLastError();
 }
 
