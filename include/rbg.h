@@ -91,6 +91,11 @@ typedef struct rbg_graph_params {
  * assembly's device buffers until the next build and is copied out with
  * rbg_asm_copy_edges. */
 #define RBG_EDGES_ON_DEVICE 1u
+/* With this flag the build's launch sequence is captured into a hipGraph the
+ * first time a resident assembly (or batch) is built with a configuration,
+ * and the graph is replayed by later builds with the same configuration (no
+ * host round trip inside a build).  Per-stage timings are then not reported. */
+#define RBG_BUILD_CAPTURE 2u
 
 /* Fills `p` with the reference defaults listed above (rand_seed = 1). */
 void rbg_default_params(rbg_graph_params* p);
@@ -107,9 +112,9 @@ typedef struct rbg_graph_result {
   uint64_t gated_pairs;    /* pairs that reached the register compare */
   /* stage timings in milliseconds, measured with HIP events on the stream */
   double ms_sketch, ms_pairs, ms_select, ms_edges, ms_total;
-  /* single-kernel durations: the pair kernel (abundance gate in the fused
-   * regime, all-pairs register compare in the small regime), the register
-   * compare of the gated pairs, and the top-N selection */
+  /* single-kernel durations: the pair kernel (the int8 prefilter of the
+   * abundance gate in the fused regime; 0 in the small regime), the exact
+   * gate + register compare of its candidates, and the top-N selection */
   double ms_k_pair, ms_k_compare, ms_k_select;
 } rbg_graph_result;
 
@@ -147,6 +152,47 @@ rbg_status rbg_asm_build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params*
  * score is only written when the assembly has depth samples). */
 rbg_status rbg_asm_copy_edges(rbg_ctx* ctx, rbg_asm* a, uint32_t* from, uint32_t* to,
                               float* scomp, float* score);
+
+/* ---------------------------------------------------------------------------
+ * Batched form: many assemblies resident together.  One build runs every
+ * stage as one launch over all assemblies' work items (rows of all
+ * assemblies are numbered consecutively; each assembly keeps its own regime,
+ * b0, calibration and parameters), so each assembly's result is identical to
+ * building it alone.  The per-assembly results have contig ids local to the
+ * assembly.  A single-assembly rbg_asm is a batch of one.
+ * No counterpart in the reference (it builds one assembly per process).
+ * ------------------------------------------------------------------------- */
+typedef struct rbg_assembly_in {
+  const char* seqs;          /* contigs back to back, as rbg_sketch_winners */
+  const uint64_t* offsets;   /* n + 1 */
+  uint32_t n;
+  const float* depth;        /* n x S raw per-sample means, or NULL when S == 0 */
+  uint32_t S;                /* at most 64 */
+} rbg_assembly_in;
+
+/* Work schedule of the pair pass: 256-column tiles per work item of the
+ * abundance gate, and rows per work item of the all-pairs counts. */
+typedef struct rbg_schedule {
+  uint32_t gate_tiles_per_item;   /* default 16 */
+  uint32_t dense_rows_per_item;   /* default 64 */
+} rbg_schedule;
+
+typedef struct rbg_batch_stats {
+  uint64_t candidates;       /* int8 prefilter candidates (all fused assemblies) */
+  uint64_t edges;            /* edges of all assemblies */
+  double ms_total;           /* device time of the build */
+  int replayed;              /* 1 when a captured graph was replayed */
+} rbg_batch_stats;
+
+rbg_status rbg_batch_create(rbg_ctx* ctx, const rbg_assembly_in* asms, uint32_t count,
+                            double min_cv, rbg_asm** out);
+rbg_status rbg_batch_set_schedule(rbg_asm* b, const rbg_schedule* s);
+/* flags of params: RBG_BUILD_CAPTURE (RBG_EDGES_ON_DEVICE is implied: the
+ * edges stay in HBM until rbg_batch_result). */
+rbg_status rbg_batch_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* params,
+                           rbg_batch_stats* stats);
+/* Assembly `index` of the last build: scalars and the edge list on the host. */
+rbg_status rbg_batch_result(rbg_ctx* ctx, rbg_asm* b, uint32_t index, rbg_graph_result* out);
 
 /* ---------------------------------------------------------------------------
  * Stage (d) alone (src/rabbitbin.cpp:5482-5509): edge weights for a given

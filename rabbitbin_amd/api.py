@@ -42,6 +42,21 @@ class GraphParams(C.Structure):
 
 
 EDGES_ON_DEVICE = 1
+BUILD_CAPTURE = 2
+
+
+class AssemblyIn(C.Structure):
+    _fields_ = [("seqs", C.c_char_p), ("offsets", C.POINTER(C.c_uint64)), ("n", C.c_uint32),
+                ("depth", C.POINTER(C.c_float)), ("S", C.c_uint32)]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("gate_tiles_per_item", C.c_uint32), ("dense_rows_per_item", C.c_uint32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("candidates", C.c_uint64), ("edges", C.c_uint64), ("ms_total", C.c_double),
+                ("replayed", C.c_int)]
 
 
 class GraphResult(C.Structure):
@@ -108,6 +123,10 @@ def lib():
     L.rbg_edge_weights.argtypes = [vp, u32p, u32p, f32p, C.c_uint64, f32p, f32p, f64p, C.c_uint32,
                                    C.c_uint32, C.c_double, C.c_double, C.c_double, f32p, f64p]
     L.rbg_pmh_baseline.argtypes = [vp, u16p, C.c_uint32, C.c_uint32, f64p]
+    L.rbg_batch_create.argtypes = [vp, C.POINTER(AssemblyIn), C.c_uint32, C.c_double, C.POINTER(vp)]
+    L.rbg_batch_set_schedule.argtypes = [vp, C.POINTER(Schedule)]
+    L.rbg_batch_build.argtypes = [vp, vp, C.POINTER(GraphParams), C.POINTER(BatchStats)]
+    L.rbg_batch_result.argtypes = [vp, vp, C.c_uint32, C.POINTER(GraphResult)]
     _lib = L
     return L
 
@@ -254,12 +273,14 @@ class Assembly:
         ctx._check(ctx._L.rbg_asm_create(ctx._h, buf, _p(off, u64p), self.n, _p(dep, f32p), self.S, min_cv,
                                          C.byref(self._h)), "rbg_asm_create")
 
-    def build_graph(self, params=None, diag=False, on_device=False):
-        """on_device=True leaves the edge list in HBM (fetch with copy_edges)."""
-        p = params or default_params()
+    def build_graph(self, params=None, diag=False, on_device=False, capture=False):
+        """on_device=True leaves the edge list in HBM (fetch with copy_edges);
+        capture=True replays a captured graph of the build after the first call."""
+        p = GraphParams.from_buffer_copy(params or default_params())
         if on_device:
-            p = GraphParams.from_buffer_copy(p)
             p.flags |= EDGES_ON_DEVICE
+        if capture:
+            p.flags |= BUILD_CAPTURE
         d, arrs = self.ctx._diag(self.n, p.max_edges, p.m, diag)
         res = GraphResult()
         self.ctx._check(self.ctx._L.rbg_asm_build_graph(self.ctx._h, self._h, C.byref(p), C.byref(res),
@@ -274,6 +295,56 @@ class Assembly:
         self.ctx._check(self.ctx._L.rbg_asm_copy_edges(self.ctx._h, self._h, _p(frm, u32p), _p(to, u32p),
                                                        _p(sc, f32p), _p(score, f32p)), "rbg_asm_copy_edges")
         return frm, to, sc, score
+
+    def free(self):
+        if self._h:
+            self.ctx._L.rbg_asm_free(self._h)
+            self._h = C.c_void_p()
+
+
+class Batch:
+    """Many assemblies resident together (rbg_batch_create); one build runs every
+    stage as one launch over all of them.  Each element of `assemblies` is
+    (seqs, offsets, depth-or-None)."""
+
+    def __init__(self, ctx, assemblies, min_cv=1.0):
+        self.ctx = ctx
+        self._keep = []
+        ins = (AssemblyIn * len(assemblies))()
+        self.sizes = []
+        for k, (seqs, offsets, depth) in enumerate(assemblies):
+            buf, off = _as_seqs(seqs, offsets)
+            dep = None if depth is None else np.ascontiguousarray(depth, np.float32)
+            S = 0 if dep is None else dep.shape[1]
+            self._keep += [buf, off, dep]
+            ins[k] = AssemblyIn(buf, _p(off, u64p), len(off) - 1, _p(dep, f32p), S)
+            self.sizes.append(len(off) - 1)
+        self._h = C.c_void_p()
+        ctx._check(ctx._L.rbg_batch_create(ctx._h, ins, len(assemblies), min_cv, C.byref(self._h)),
+                   "rbg_batch_create")
+        self._keep = None  # the library copied the inputs
+
+    def set_schedule(self, gate_tiles_per_item=16, dense_rows_per_item=64):
+        st = self.ctx._L.rbg_batch_set_schedule(self._h, C.byref(Schedule(gate_tiles_per_item,
+                                                                          dense_rows_per_item)))
+        if st != 0:
+            raise RbgError(f"rbg_batch_set_schedule: status {st}")
+
+    def build(self, params=None, capture=False):
+        p = GraphParams.from_buffer_copy(params or default_params())
+        if capture:
+            p.flags |= BUILD_CAPTURE
+        stats = BatchStats()
+        self.ctx._check(self.ctx._L.rbg_batch_build(self.ctx._h, self._h, C.byref(p), C.byref(stats)),
+                        "rbg_batch_build")
+        return {"candidates": stats.candidates, "edges": stats.edges, "ms": stats.ms_total,
+                "replayed": bool(stats.replayed)}
+
+    def result(self, index):
+        res = GraphResult()
+        self.ctx._check(self.ctx._L.rbg_batch_result(self.ctx._h, self._h, index, C.byref(res)),
+                        "rbg_batch_result")
+        return self.ctx._result(res, self.sizes[index], 0, None)
 
     def free(self):
         if self._h:

@@ -1,5 +1,11 @@
 // Kernel launchers (internal to librbg).  All launch on `st` and return the
 // launch status; shapes are checked by the callers in rbg_api.cpp.
+//
+// A graph build runs over a batch of assemblies (one assembly is a batch of
+// one).  Rows (contigs) of all assemblies are numbered consecutively; every
+// per-row array is indexed by that global row, asm_of[row] gives the row's
+// assembly, and AsmDesc holds the assembly's parameters.  Pairs, candidate
+// lists, top-K ids and edges carry global rows.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -8,8 +14,8 @@
 
 namespace rbg {
 
-// Scalars of one graph build, kept in device memory and written by the
-// kernels that compute them (b0 and the sv tables, then the calibration).
+// Scalars of one assembly's graph build, kept in device memory and written
+// by the kernels that compute them (b0 and the sv tables, then calibration).
 struct BuildScalars {
   double b0;
   double sim_cutoff;   // per-mille, as the reference's simCutoff
@@ -19,72 +25,128 @@ struct BuildScalars {
   uint32_t pad;
 };
 
-// b0 from the sampled pairs' counts, then svd/svf/level for counts 0..m.
-hipError_t launch_baseline_tables(const uint32_t* pcnt, uint32_t np, uint32_t m, BuildScalars* bs,
-                                  double* svd, float* svf, uint16_t* level, hipStream_t st);
-// simCutoff and cmin from the per-row maxima (regime 1: all n rows; regime 2:
-// the round members), consuming rnd[] in order.
-hipError_t launch_calibrate(const uint32_t* rowmax, uint32_t n, uint32_t m, const double* svd,
-                            const float* svf, const int32_t* rnd, uint32_t nrnd,
-                            const uint32_t* members, const uint32_t* round_off, int regime,
-                            double coverage, BuildScalars* bs, hipStream_t st);
+// Per-assembly parameters of a batch (device-resident, one per assembly).
+struct AsmDesc {
+  uint32_t row0;      // first global row of the assembly
+  uint32_t n;         // contigs
+  uint32_t S;         // depth samples
+  uint32_t regime;    // 1: all pairs (n x n counts); 2: abundance-gated candidates
+  float thr;          // exact abundance-gate threshold (-inf: gate off)
+  int32_t qcut;       // int8 prefilter cut
+  uint32_t acc1_end;  // odd chunks below this accumulate into the second register
+  uint32_t pair0;     // b0 pair draws: [pair0, pair0 + np) of the batch list
+  uint32_t np;
+  uint32_t colnorm0;  // first of the assembly's S colnorm entries
+  uint64_t mat0;      // regime 1: first entry of the assembly's n x n count matrix
+  double w_comp;      // composition weight
+};
 
+// Per-assembly tables: svd/svf/level hold m+1 entries per assembly; the
+// calibration plan holds kCalibRandDev rand() values and 11 round offsets
+// per assembly.
+constexpr uint32_t kCalibRandDev = 2048;
+
+// ---- stage (a)
 hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
                          uint32_t W, const double* sT, const uint8_t* sX, const uint8_t* cls,
-                         const uint8_t* codes,
-                         uint8_t* wx, uint32_t mpad, uint32_t* planes, hipStream_t st);
+                         const uint8_t* codes, uint8_t* wx, uint32_t mpad, uint32_t* planes,
+                         hipStream_t st);
 hipError_t launch_list_counts(const uint32_t* planes, uint32_t W, const uint32_t* pi,
                               const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st);
 hipError_t launch_win16_counts(const uint16_t* win16, uint32_t m, const uint32_t* pi,
                                const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st);
-hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, uint32_t n, uint16_t* out,
+
+// ---- b0, sv tables, calibration (one block per assembly)
+hipError_t launch_baseline_tables(const uint32_t* pcnt, const AsmDesc* desc, uint32_t nasm,
+                                  uint32_t m, BuildScalars* bs, double* svd, float* svf,
+                                  uint16_t* level, hipStream_t st);
+// simCutoff and cmin from the per-row maxima (regime 1: all rows of the
+// assembly; regime 2: its round members), consuming its rand() values in order.
+hipError_t launch_calibrate(const uint32_t* rowmax, const AsmDesc* desc, uint32_t nasm, uint32_t m,
+                            const double* svd, const float* svf, const int32_t* rnd,
+                            const uint32_t* members, const uint32_t* round_off, double coverage,
+                            BuildScalars* bs, hipStream_t st);
+
+// ---- stage (b), regime 1: n x n counts per assembly
+// Work items (assembly, column block of 256, row block, rows per block).
+void dense_work_items(const std::vector<AsmDesc>& desc, uint32_t rows_per_item,
+                      std::vector<uint32_t>& items);
+hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, const AsmDesc* desc,
+                               const uint32_t* items, uint32_t nitems, uint16_t* mat,
                                hipStream_t st);
-// Work items of the fused pair pass: (bi << 16 | bj0, tile count) per item.
-void gate_work_items(uint32_t n, std::vector<uint32_t>& items);
-// q holds the int8 rows in pair-pass order; perm maps that order back to
-// contig ids (candidates are written as (min id, max id)).
-hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, uint32_t n, int32_t qcut,
+
+// ---- stage (b), regime 2: abundance gate + register compare
+// Work items (assembly, bi << 16 | bj0, tile count, 0): a 256-row stripe bi of
+// the assembly against up to tiles_per_item 256-column tiles from bj0.
+void gate_work_items(uint32_t a, uint32_t n, uint32_t tiles_per_item, std::vector<uint32_t>& full,
+                     std::vector<uint32_t>& part);
+// q holds each assembly's int8 rows in its pair-pass order (kpad bytes per
+// row); perm maps that order back to global rows.  Candidates are written as
+// (min row, max row).
+hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, const AsmDesc* desc,
                             const uint32_t* perm, const uint32_t* items, uint32_t nitems,
                             uint64_t* cand, unsigned long long* ncand, uint64_t cap,
                             hipStream_t st);
-// Exact gate + register compare of the candidates, in place (rejected ->
-// all ones, survivors -> count in the low 16 bits; deg counts survivors per row).
-hipError_t launch_gate_compare(uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,
-                               const uint8_t* anynz, float thr, uint32_t acc1_end,
+// Exact gate + register compare of the min(*ncand, cap) candidates, in place
+// (rejected -> all ones, survivors -> count in the low 16 bits); deg counts
+// survivors per row.  unit rows are zero-padded to 16*NC floats.
+hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
+                               const float* unit, uint32_t NC, const uint8_t* anynz,
+                               const AsmDesc* desc, const uint16_t* asm_of,
                                const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st);
-hipError_t launch_scatter(const uint64_t* surv, uint64_t ns, const uint64_t* off, uint32_t* cursor,
-                          uint64_t* list, hipStream_t st);
-hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32_t n,
-                               const uint16_t* level, uint32_t nlevels, uint32_t K,
-                               uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                               uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st);
-hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_t n,
-                              const uint16_t* level, uint32_t nlevels, uint32_t K,
-                              uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                              uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st);
+hipError_t launch_scatter(const uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
+                          const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st);
+
+// ---- stage (c)
+// Per-row top-K (sv desc, id asc) over the row's partners: regime 1 rows read
+// the dense count matrix, regime 2 rows their candidate list (off, list).
+hipError_t launch_select(const uint64_t* list, const uint64_t* off, const uint16_t* mat,
+                         const uint8_t* anynz, const AsmDesc* desc, const uint16_t* asm_of,
+                         uint32_t nrows, const uint16_t* level, uint32_t m, uint32_t K,
+                         uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
+                         uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st);
 // Fence rows of the selected lists: FW words per row (16-id stride, padded
 // with 0xFFFFFFFF); FW = 16 * ceil(ceil(K/16) / 16).
 inline uint32_t fence_words(uint32_t K) { return 16u * (((K + 15u) / 16u + 15u) / 16u); }
-// Edge emission: pass 0 marks kept top-K entries (keep: n x ceil(K/64) u64)
-// and counts them per row; pass 1 writes them at the rows' offsets.
+// Edge emission: pass 0 marks kept top-K entries (keep: nrows x ceil(K/64)
+// u64) and counts them per row; pass 1 writes them at the rows' offsets.
 hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
                             const uint32_t* topk_len, const uint32_t* fence, uint32_t FW,
-                            uint32_t n, uint32_t K, const BuildScalars* bs, int mutual,
-                            uint64_t* keep, uint32_t* row_edges, hipStream_t st);
+                            uint32_t nrows, uint32_t K, const BuildScalars* bs,
+                            const AsmDesc* desc, const uint16_t* asm_of, uint64_t* keep,
+                            uint32_t* row_edges, hipStream_t st);
 hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
-                             const uint32_t* topk_len, uint32_t n, uint32_t K, const uint64_t* keep,
-                             const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
-                             uint16_t* ecnt, hipStream_t st);
+                             const uint32_t* topk_len, uint32_t nrows, uint32_t K,
+                             const uint64_t* keep, const uint64_t* row_off, uint32_t* efrom,
+                             uint32_t* eto, uint16_t* ecnt, hipStream_t st);
+
+// ---- stage (d)
+// sComp of the min(*ne, cap) edges from their counts (the edge's assembly's svf).
+hipError_t launch_cnt_to_sv(const uint16_t* ecnt, uint64_t cap, const uint64_t* ne,
+                            const uint32_t* efrom, const uint16_t* asm_of, uint32_t m,
+                            const float* svf, float* scomp, hipStream_t st);
+// Edge weights of the min(*ne, cap) edges of a batch: unit/raw rows padded to
+// 16*NC floats, each edge with its assembly's S, w_comp and colnorm.
+hipError_t launch_edge_weights_batch(const uint32_t* efrom, const uint32_t* eto,
+                                     const float* scomp, uint64_t cap, const uint64_t* ne,
+                                     const float* unit, const float* raw, uint32_t NC,
+                                     const double* colnorm, const AsmDesc* desc,
+                                     const uint16_t* asm_of, double min_edge, double neg_thr,
+                                     float* score, hipStream_t st);
+// Edge weights of an explicit edge list (unpadded n x S rows, host count).
 hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const float* scomp,
                                uint64_t ne, const float* unit, const float* raw,
                                const double* colnorm, uint32_t S, double w_comp, double min_edge,
-                               double neg_thr, float* score, double* score64, hipStream_t st,
-                               bool padded = false, const uint64_t* ne_dev = nullptr);
-// padded: unit and raw rows are zero-padded to 16*ceil(S/16) floats (S <= 64).
-// ne_dev (optional): device-resident element count; ne is then the capacity.
-hipError_t launch_cnt_to_sv(const uint16_t* c, uint64_t ne, const float* svf, float* out,
-                            hipStream_t st, const uint64_t* ne_dev = nullptr);
+                               double neg_thr, float* score, double* score64, hipStream_t st);
+
+// ---- utilities
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n, void* tmp,
                                      size_t& tmp_bytes, hipStream_t st);
+// Fills count 32-bit words with value (a kernel, so it can sit in a captured graph).
+hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t st);
+// Per assembly a: out[4a..4a+3] = row_off at its first and one-past-last row,
+// and the same for off (0 when off is null).
+hipError_t launch_gather_bounds(const uint64_t* row_off, const uint64_t* off, const AsmDesc* desc,
+                                uint32_t nasm, uint64_t* out, hipStream_t st);
 
 }  // namespace rbg

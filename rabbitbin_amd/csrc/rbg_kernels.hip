@@ -362,19 +362,27 @@ __global__ __launch_bounds__(256) void win16_counts_kernel(const uint16_t* __res
 // matrix.  Thread = column j with its bit planes in VGPRs; rows are uniform
 // across the wave and come in through scalar loads.
 // ---------------------------------------------------------------------------
+// Work item (assembly a, column block cb, row block rb, rows): thread = local
+// column 256*cb + t of assembly a against its rows [rb*rows, +rows); the
+// counts go to the assembly's n x n matrix (local indices).
 template <int W>
 __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __restrict__ planes,
-                                                           uint32_t n, uint32_t rows_per_block,
-                                                           uint16_t* __restrict__ out) {
-  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t r0 = blockIdx.y * rows_per_block;
-  const uint32_t r1 = min(n, r0 + rows_per_block);
+                                                           const AsmDesc* __restrict__ desc,
+                                                           const uint4* __restrict__ items,
+                                                           uint16_t* __restrict__ mat) {
+  const uint4 it = items[blockIdx.x];
+  const uint32_t n = desc[it.x].n;
+  const uint32_t* __restrict__ rows = planes + (size_t)desc[it.x].row0 * 8 * W;
+  uint16_t* __restrict__ out = mat + desc[it.x].mat0;
+  const uint32_t j = it.y * 256 + threadIdx.x;
+  const uint32_t r0 = it.z * it.w;
+  const uint32_t r1 = min(n, r0 + it.w);
   uint32_t b[8 * W];
   const uint32_t jj = j < n ? j : n - 1;
 #pragma unroll
-  for (int w = 0; w < W; ++w) column_word(planes + (size_t)jj * 8 * W + 8 * w, b + 8 * w);
+  for (int w = 0; w < W; ++w) column_word(rows + (size_t)jj * 8 * W + 8 * w, b + 8 * w);
   for (uint32_t r = r0; r < r1; ++r) {
-    const uint32_t* __restrict__ a = planes + (size_t)r * 8 * W;
+    const uint32_t* __restrict__ a = rows + (size_t)r * 8 * W;
     uint32_t mism = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) mism += __popc(word_mismatch(a + 8 * w, b + 8 * w));
@@ -402,7 +410,6 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
 // ---------------------------------------------------------------------------
 typedef int v4i32 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kGateBuf = 512;  // per-wave candidate buffer (a 16-row block adds <= 256)
-constexpr uint32_t kGateTilesPerItem = 16;
 
 template <int KC>
 __device__ __forceinline__ void load_q8(const int8_t* __restrict__ q, uint32_t row, uint32_t n,
@@ -425,18 +432,24 @@ __device__ __forceinline__ void load_tile_ops(const int8_t* __restrict__ q, uint
 }
 
 template <int KC>
-__global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict__ q, uint32_t n,
-                                                        int32_t qcut, const uint32_t* __restrict__ perm,
-                                                        const uint2* __restrict__ items,
+__global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict__ q_all,
+                                                        const AsmDesc* __restrict__ desc,
+                                                        const uint32_t* __restrict__ perm_all,
+                                                        const uint4* __restrict__ items,
                                                         uint64_t* __restrict__ cand,
                                                         unsigned long long* __restrict__ ncand,
                                                         uint64_t cap) {
   __shared__ uint32_t buf[4][kGateBuf];
   __shared__ long bops[2][16][KC][64];
   const uint32_t wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
-  const uint2 item = items[blockIdx.x];
-  const uint32_t ib = (item.x >> 16) * 256u, jbase = (item.x & 0xFFFFu) * 256u;
-  const uint32_t ntile = item.y;
+  const uint4 item = items[blockIdx.x];
+  // the item's assembly: its rows, in pair-pass order, start at row0
+  const uint32_t n = desc[item.x].n;
+  const int32_t qcut = desc[item.x].qcut;
+  const int8_t* __restrict__ q = q_all + (size_t)desc[item.x].row0 * 32 * KC;
+  const uint32_t* __restrict__ perm = perm_all + desc[item.x].row0;
+  const uint32_t ib = (item.y >> 16) * 256u, jbase = (item.y & 0xFFFFu) * 256u;
+  const uint32_t ntile = item.z;
   const uint32_t row0 = ib + 64 * wv;
 
   // Row operands go through LDS (buffer 1 is free until tile 1), so inside
@@ -593,14 +606,12 @@ __device__ __forceinline__ void unpack_pair(uint64_t e, uint32_t& i, uint32_t& j
 // shuffles).  All loads are issued before the gate is known: nearly every
 // candidate survives.  Survivors add 1 to deg[i] and deg[j].
 template <int NC, int W>
-__global__ __launch_bounds__(256) void gate_compare_kernel(
-    uint64_t* __restrict__ cand, uint64_t nc, const float* __restrict__ unit,
-    const uint8_t* __restrict__ anynz, float thr, uint32_t acc1_end,
+__device__ __forceinline__ void gate_compare_one(
+    uint64_t* __restrict__ cand, uint64_t t, uint64_t nc, uint32_t q,
+    const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
+    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
     const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
   constexpr int WPL = W >= 4 ? W / 4 : 1;  // word groups per lane
-  const uint64_t gt = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  const uint64_t t = gt >> 2;
-  const uint32_t q = (uint32_t)(gt & 3u);
   const bool live = t < nc;
   const uint64_t e = live ? cand[t] : 0;
   // Row indices are taken as 64-bit fields and masked in 64 bits: with the
@@ -609,6 +620,9 @@ __global__ __launch_bounds__(256) void gate_compare_kernel(
   const uint64_t i64 = e >> 40, j64 = (e >> 16) & 0xFFFFFFull;
   const uint32_t i = (uint32_t)i64, j = (uint32_t)j64;
   const bool has_words = W >= 4 || q < (uint32_t)W;
+  const AsmDesc& ad = desc[live ? asm_of[i] : 0];
+  const float thr = ad.thr;
+  const uint32_t acc1_end = ad.acc1_end;
   float4 ua[NC], ub[NC];
   uint4 pa[2 * WPL], pb[2 * WPL];
   if (live) {
@@ -682,14 +696,28 @@ __global__ __launch_bounds__(256) void gate_compare_kernel(
   if (live && q == 0) cand[t] = pass ? (e & ~0xFFFFull) | (uint64_t)(32u * W - mism) : kDeadPair;
 }
 
+template <int NC, int W>
+__global__ __launch_bounds__(256) void gate_compare_kernel(
+    uint64_t* __restrict__ cand, uint64_t cap, const unsigned long long* __restrict__ ncand,
+    const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
+    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
+    const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
+  const uint64_t nc = min((uint64_t)*ncand, cap);
+  const uint32_t q = threadIdx.x & 3u;
+  // wave-uniform loop: 16 candidates per wave per step
+  for (uint64_t tw = ((uint64_t)blockIdx.x * 256 + (threadIdx.x & ~63u)) >> 2; tw < nc;
+       tw += ((uint64_t)gridDim.x * 256) >> 2)
+    gate_compare_one<NC, W>(cand, tw + ((threadIdx.x & 63u) >> 2), nc, q, unit, anynz, desc,
+                            asm_of, planes, deg);
+}
+
 // Scatter each gated pair into both rows' candidate lists: entry = (cnt<<32)|id.
 // Candidates come in abundance-ordered clusters, so a wave often holds the
 // same row several times: the slots are claimed with wave-aggregated atomics.
-__global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict__ surv, uint64_t ns,
-                                                      const uint64_t* __restrict__ off,
-                                                      uint32_t* __restrict__ cursor,
-                                                      uint64_t* __restrict__ list) {
-  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void scatter_one(const uint64_t* __restrict__ surv, uint64_t t, uint64_t ns,
+                                            const uint64_t* __restrict__ off,
+                                            uint32_t* __restrict__ cursor,
+                                            uint64_t* __restrict__ list) {
   const uint64_t e = t < ns ? surv[t] : kDeadPair;
   const bool live = e != kDeadPair;
   uint32_t i = 0, j = 0, c = 0;
@@ -703,6 +731,17 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict
   }
 }
 
+__global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict__ surv, uint64_t cap,
+                                                      const unsigned long long* __restrict__ ncand,
+                                                      const uint64_t* __restrict__ off,
+                                                      uint32_t* __restrict__ cursor,
+                                                      uint64_t* __restrict__ list) {
+  const uint64_t ns = min((uint64_t)*ncand, cap);
+  for (uint64_t tw = (uint64_t)blockIdx.x * 256 + (threadIdx.x & ~63u); tw < ns;
+       tw += (uint64_t)gridDim.x * 256)  // wave-uniform loop
+    scatter_one(surv, tw + (threadIdx.x & 63u), ns, off, cursor, list);
+}
+
 // ---------------------------------------------------------------------------
 // Per-row top-K under (sv desc, id asc) (src/rabbitbin.h:418-429).
 // sv is a monotone function of the count, so the order is (level desc, id
@@ -714,16 +753,32 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict
 // sorted by id.  Also reports the row's maximum count (small regime: over all
 // partners j != i, as calibrate_sim_cutoff; fused: over the gated partners).
 // ---------------------------------------------------------------------------
+// The partners of one row (global row i): a row of its assembly's dense
+// count matrix (regime 1; partners are all rows j != i of the assembly that
+// pass is_nz) or its candidate list (regime 2).  Ids are global rows.
 struct RowSource {
-  // dense
-  const uint16_t* mat;
-  const uint8_t* anynz;
-  uint32_t n;
-  // list
-  const uint64_t* list;
+  const uint64_t* list;   // regime 2
   const uint64_t* off;
+  const uint16_t* mrow;   // regime 1: the row's counts against the assembly's rows
+  const uint8_t* anynz;
+  uint32_t row0, local, n;
   bool dense;
 };
+
+__device__ __forceinline__ RowSource row_source(const uint64_t* list, const uint64_t* off,
+                                                const uint16_t* mat, const uint8_t* anynz,
+                                                const AsmDesc& d, uint32_t i) {
+  RowSource s;
+  s.list = list;
+  s.off = off;
+  s.anynz = anynz;
+  s.row0 = d.row0;
+  s.local = i - d.row0;
+  s.n = d.n;
+  s.dense = d.regime == 1;
+  s.mrow = s.dense ? mat + d.mat0 + (size_t)s.local * d.n : nullptr;
+  return s;
+}
 
 __device__ __forceinline__ uint32_t row_len(const RowSource& s, uint32_t i) {
   return s.dense ? s.n : (uint32_t)(s.off[i + 1] - s.off[i]);
@@ -732,10 +787,10 @@ __device__ __forceinline__ uint32_t row_len(const RowSource& s, uint32_t i) {
 __device__ __forceinline__ bool row_entry(const RowSource& s, uint32_t i, uint32_t t, uint32_t& id,
                                           uint32_t& c, bool& for_max) {
   if (s.dense) {
-    id = t;
-    c = s.mat[(size_t)i * s.n + t];
-    for_max = t != i;
-    return t != i && (!s.anynz || s.anynz[i] || s.anynz[t]);
+    id = s.row0 + t;
+    c = s.mrow[t];
+    for_max = t != s.local;
+    return t != s.local && (!s.anynz || s.anynz[i] || s.anynz[id]);
   }
   const uint64_t e = s.list[s.off[i] + t];
   id = (uint32_t)e;
@@ -899,7 +954,10 @@ __device__ void select_staged_row(const RowSource& src, uint32_t i, uint32_t len
 }
 
 __global__ __launch_bounds__(256) void select_topk_kernel(
-    RowSource src, uint32_t n, const uint16_t* __restrict__ level, uint32_t nlevels, uint32_t K,
+    const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
+    const uint16_t* __restrict__ mat, const uint8_t* __restrict__ anynz,
+    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
+    const uint16_t* __restrict__ level_all, uint32_t m, uint32_t K,
     uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
     uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out,
     uint32_t* __restrict__ fence, uint32_t FW) {
@@ -910,6 +968,11 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
   __shared__ uint32_t wave_tot[4];
   const uint32_t i = blockIdx.x;
   const uint32_t tid = threadIdx.x;
+  const uint32_t a = asm_of[i];
+  const RowSource src = row_source(list, off, mat, anynz, desc[a], i);
+  // the assembly's level table; bins above its distinct similarities stay empty
+  const uint16_t* __restrict__ level = level_all + (size_t)a * (m + 1);
+  const uint32_t nlevels = m + 1;
   const uint32_t len = row_len(src, i);
 
   if (!src.dense && len <= kSelectStage && nlevels <= 1024) {
@@ -1051,13 +1114,16 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
 __global__ __launch_bounds__(256) void emit_mark_kernel(
     const uint32_t* __restrict__ topk_id, const uint16_t* __restrict__ topk_cnt,
     const uint32_t* __restrict__ topk_len, const uint32_t* __restrict__ fence, uint32_t FW,
-    uint32_t n, uint32_t K, const BuildScalars* __restrict__ bs, int mutual, uint32_t KW,
-    uint64_t* __restrict__ keep, uint32_t* __restrict__ row_edges) {
+    uint32_t n, uint32_t K, const BuildScalars* __restrict__ bs, const AsmDesc* __restrict__ desc,
+    const uint16_t* __restrict__ asm_of, uint32_t KW, uint64_t* __restrict__ keep,
+    uint32_t* __restrict__ row_edges) {
   const bool aligned16 = (K & 3u) == 0;
-  const uint32_t cmin = bs->cmin;
   const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t ln = threadIdx.x & 63u;
   if (i >= n) return;
+  const uint32_t a = asm_of[i];
+  const uint32_t cmin = bs[a].cmin;
+  const bool mutual = desc[a].regime == 2;
   const uint32_t len = topk_len[i];
   uint32_t total = 0;
   for (uint32_t s0 = 0; s0 * 64 < len; s0 += 4) {
@@ -1146,11 +1212,6 @@ __global__ __launch_bounds__(256) void emit_write_kernel(
   }
 }
 
-// Element count: *ne_dev when given (device-resident), else ne.
-__device__ __forceinline__ uint64_t count_of(const uint64_t* ne_dev, uint64_t ne) {
-  return ne_dev ? *ne_dev : ne;
-}
-
 // Stage (d): per-edge weight in FP64 (the reference's depth_edge_term with
 // the default fuse = min, src/rabbitbin.cpp:930-970, and the combination and
 // cutoff of src/rabbitbin.cpp:5500-5507).
@@ -1159,8 +1220,8 @@ __global__ __launch_bounds__(256) void edge_weight_kernel(
     const float* __restrict__ scomp, uint64_t ne, const float* __restrict__ unit,
     const float* __restrict__ raw, const double* __restrict__ colnorm, uint32_t S,
     double w_comp, double min_edge, double neg_thr, float* __restrict__ score,
-    double* __restrict__ score64, const uint64_t* __restrict__ ne_dev) {
-  const uint64_t cnt = count_of(ne_dev, ne);
+    double* __restrict__ score64) {
+  const uint64_t cnt = ne;
   for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt;
        e += (uint64_t)gridDim.x * 256) {
   const uint32_t i = efrom[e], j = eto[e];
@@ -1209,21 +1270,32 @@ __global__ __launch_bounds__(256) void edge_weight_kernel(
   }
 }
 
-// The same weight for 1 < S <= 16*NC with unit and raw rows zero-padded to
+// The same weight for the edges of a batch, each with its assembly's S
+// (S <= 16*NC), w_comp and colnorm; unit and raw rows are zero-padded to
 // 16*NC floats (128-byte aligned rows read as float4).  Per edge the FP64
 // sums run in the same order as above, so the results are identical.
 template <int NC>
 __global__ __launch_bounds__(256) void edge_weight_pad_kernel(
     const uint32_t* __restrict__ efrom, const uint32_t* __restrict__ eto,
-    const float* __restrict__ scomp, uint64_t ne, const float* __restrict__ unitp,
-    const float* __restrict__ rawp, const double* __restrict__ colnorm, uint32_t S,
-    double w_comp, double min_edge, double neg_thr, float* __restrict__ score,
-    double* __restrict__ score64, const uint64_t* __restrict__ ne_dev) {
+    const float* __restrict__ scomp, uint64_t cap, const uint64_t* __restrict__ ne_dev,
+    const float* __restrict__ unitp, const float* __restrict__ rawp,
+    const double* __restrict__ colnorm_all, const AsmDesc* __restrict__ desc,
+    const uint16_t* __restrict__ asm_of, double min_edge, double neg_thr,
+    float* __restrict__ score) {
   constexpr int V = 4 * NC;  // float4 per row
-  const uint64_t cnt = count_of(ne_dev, ne);
+  const uint64_t cnt = min(*ne_dev, cap);
   for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt;
        e += (uint64_t)gridDim.x * 256) {
   const uint32_t i = efrom[e], j = eto[e];
+  const AsmDesc& ad = desc[asm_of[i]];
+  const uint32_t S = ad.S;
+  const double w_comp = ad.w_comp;
+  if (S <= 1) {  // no depth: the composition similarity alone
+    const double sc = (double)scomp[e];
+    score[e] = (float)(S == 0 && sc < min_edge ? 0.0 : sc);
+    continue;
+  }
+  const double* __restrict__ colnorm = colnorm_all + ad.colnorm0;
   const float4* ua = reinterpret_cast<const float4*>(unitp + (size_t)i * 16 * NC);
   const float4* ub = reinterpret_cast<const float4*>(unitp + (size_t)j * 16 * NC);
   float a[4 * V], b[4 * V];
@@ -1275,16 +1347,17 @@ __global__ __launch_bounds__(256) void edge_weight_pad_kernel(
     if (!(w == w) || w < min_edge) w = 0.0;
   }
   score[e] = (float)w;
-  if (score64) score64[e] = w;
   }
 }
 
-__global__ void u16_to_sv_kernel(const uint16_t* __restrict__ c, uint64_t ne,
+__global__ void u16_to_sv_kernel(const uint16_t* __restrict__ c, uint64_t cap,
                                  const uint64_t* __restrict__ ne_dev,
+                                 const uint32_t* __restrict__ efrom,
+                                 const uint16_t* __restrict__ asm_of, uint32_t m,
                                  const float* __restrict__ svf, float* __restrict__ out) {
-  const uint64_t cnt = count_of(ne_dev, ne);
+  const uint64_t cnt = min(*ne_dev, cap);
   for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (uint64_t)gridDim.x * 256)
-    out[e] = svf[c[e]];
+    out[e] = svf[(size_t)asm_of[efrom[e]] * (m + 1) + c[e]];
 }
 
 
@@ -1299,16 +1372,24 @@ __global__ void u16_to_sv_kernel(const uint16_t* __restrict__ c, uint64_t ne,
 // FP64 (IEEE division and no contraction, as the host build) and the level
 // of each count (dense rank of the distinct float similarities).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void baseline_tables_kernel(const uint32_t* __restrict__ pcnt,
-                                                              uint32_t np, uint32_t m,
-                                                              BuildScalars* __restrict__ bs,
-                                                              double* __restrict__ svd,
-                                                              float* __restrict__ svf,
-                                                              uint16_t* __restrict__ level) {
+__global__ __launch_bounds__(256) void baseline_tables_kernel(const uint32_t* __restrict__ pcnt_all,
+                                                              const AsmDesc* __restrict__ desc,
+                                                              uint32_t m,
+                                                              BuildScalars* __restrict__ bs_all,
+                                                              double* __restrict__ svd_all,
+                                                              float* __restrict__ svf_all,
+                                                              uint16_t* __restrict__ level_all) {
   __shared__ uint32_t hist[513];
   __shared__ float sf[513];
   __shared__ double b0s;
   const uint32_t tid = threadIdx.x;
+  const uint32_t a = blockIdx.x;  // one assembly per block
+  const uint32_t np = desc[a].np;
+  const uint32_t* __restrict__ pcnt = pcnt_all + desc[a].pair0;
+  BuildScalars* __restrict__ bs = bs_all + a;
+  double* __restrict__ svd = svd_all + (size_t)a * (m + 1);
+  float* __restrict__ svf = svf_all + (size_t)a * (m + 1);
+  uint16_t* __restrict__ level = level_all + (size_t)a * (m + 1);
   for (uint32_t c = tid; c <= m; c += 256) hist[c] = 0;
   __syncthreads();
   for (uint32_t t = tid; t < np; t += 256) atomicAdd(&hist[pcnt[t]], 1u);
@@ -1390,10 +1471,20 @@ __device__ uint32_t calib_converge_dev(const uint32_t* on_at, double total, doub
 }
 
 __global__ __launch_bounds__(256) void calibrate_kernel(
-    const uint32_t* __restrict__ rowmax, uint32_t n, uint32_t m, const double* __restrict__ svd,
-    const float* __restrict__ svf, const int32_t* __restrict__ rnd, uint32_t nrnd,
-    const uint32_t* __restrict__ members, const uint32_t* __restrict__ round_off, int regime,
-    double coverage, BuildScalars* __restrict__ bs) {
+    const uint32_t* __restrict__ rowmax_all, const AsmDesc* __restrict__ desc, uint32_t m,
+    const double* __restrict__ svd_all, const float* __restrict__ svf_all,
+    const int32_t* __restrict__ rnd_all, const uint32_t* __restrict__ members,
+    const uint32_t* __restrict__ round_off_all, double coverage, BuildScalars* __restrict__ bs_all) {
+  const uint32_t a = blockIdx.x;  // one assembly per block
+  const uint32_t n = desc[a].n;
+  const int regime = (int)desc[a].regime;
+  const uint32_t* __restrict__ rowmax = rowmax_all + desc[a].row0;  // regime 1 rows
+  const double* __restrict__ svd = svd_all + (size_t)a * (m + 1);
+  const float* __restrict__ svf = svf_all + (size_t)a * (m + 1);
+  const int32_t* __restrict__ rnd = rnd_all + (size_t)a * kCalibRandDev;
+  const uint32_t nrnd = kCalibRandDev;
+  const uint32_t* __restrict__ round_off = round_off_all + (size_t)a * 11;
+  BuildScalars* __restrict__ bs = bs_all + a;
   __shared__ uint32_t hist[514];
   __shared__ uint32_t on_at[1000];
   __shared__ double val[513];
@@ -1420,7 +1511,7 @@ __global__ __launch_bounds__(256) void calibrate_kernel(
         for (uint32_t t = tid; t < n; t += 256) atomicAdd(&hist[rowmax[t]], 1u);
     } else {
       for (uint32_t t = round_off[r] + tid; t < round_off[r + 1]; t += 256)
-        atomicAdd(&hist[rowmax[members[t]]], 1u);
+        atomicAdd(&hist[rowmax_all[members[t]]], 1u);  // members are global rows
     }
     __syncthreads();
     // suffix sums: hist[c] <- contigs with count >= c (hist[m+1] = 0)
@@ -1587,58 +1678,77 @@ hipError_t launch_win16_counts(const uint16_t* win16, uint32_t m, const uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, uint32_t n, uint16_t* out,
+void dense_work_items(const std::vector<AsmDesc>& desc, uint32_t rows_per_item,
+                      std::vector<uint32_t>& items) {
+  items.clear();
+  for (uint32_t a = 0; a < (uint32_t)desc.size(); ++a) {
+    if (desc[a].regime != 1) continue;
+    const uint32_t n = desc[a].n;
+    for (uint32_t rb = 0; rb * rows_per_item < n; ++rb)
+      for (uint32_t cb = 0; cb * 256 < n; ++cb) {
+        items.push_back(a);
+        items.push_back(cb);
+        items.push_back(rb);
+        items.push_back(rows_per_item);
+      }
+  }
+}
+
+hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, const AsmDesc* desc,
+                               const uint32_t* items, uint32_t nitems, uint16_t* mat,
                                hipStream_t st) {
-  const uint32_t rows = 64;
-  dim3 grid((n + 255) / 256, (n + rows - 1) / rows);
+  if (nitems == 0) return hipSuccess;
+  const uint4* it = reinterpret_cast<const uint4*>(items);
+  const dim3 grid(nitems);
   switch (W) {
-    case 16: hipLaunchKernelGGL(dense_counts_kernel<16>, grid, dim3(256), 0, st, planes, n, rows, out); break;
-    case 8: hipLaunchKernelGGL(dense_counts_kernel<8>, grid, dim3(256), 0, st, planes, n, rows, out); break;
-    case 4: hipLaunchKernelGGL(dense_counts_kernel<4>, grid, dim3(256), 0, st, planes, n, rows, out); break;
-    case 2: hipLaunchKernelGGL(dense_counts_kernel<2>, grid, dim3(256), 0, st, planes, n, rows, out); break;
-    case 1: hipLaunchKernelGGL(dense_counts_kernel<1>, grid, dim3(256), 0, st, planes, n, rows, out); break;
+    case 16: hipLaunchKernelGGL(dense_counts_kernel<16>, grid, dim3(256), 0, st, planes, desc, it, mat); break;
+    case 8: hipLaunchKernelGGL(dense_counts_kernel<8>, grid, dim3(256), 0, st, planes, desc, it, mat); break;
+    case 4: hipLaunchKernelGGL(dense_counts_kernel<4>, grid, dim3(256), 0, st, planes, desc, it, mat); break;
+    case 2: hipLaunchKernelGGL(dense_counts_kernel<2>, grid, dim3(256), 0, st, planes, desc, it, mat); break;
+    case 1: hipLaunchKernelGGL(dense_counts_kernel<1>, grid, dim3(256), 0, st, planes, desc, it, mat); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-void gate_work_items(uint32_t n, std::vector<uint32_t>& items) {
-  // (bi << 16 | bj0, tile count) pairs; full items first
+void gate_work_items(uint32_t a, uint32_t n, uint32_t tiles_per_item, std::vector<uint32_t>& full,
+                     std::vector<uint32_t>& part) {
+  // (a, bi << 16 | bj0, tile count, 0); items with the full tile count go first
   const uint32_t T = (n + 255) / 256;
-  std::vector<uint32_t> full, part;
   for (uint32_t bi = 0; bi < T; ++bi)
-    for (uint32_t bj = bi; bj < T; bj += kGateTilesPerItem) {
-      const uint32_t cnt = std::min(kGateTilesPerItem, T - bj);
-      auto& dst = cnt == kGateTilesPerItem ? full : part;
+    for (uint32_t bj = bi; bj < T; bj += tiles_per_item) {
+      const uint32_t cnt = std::min(tiles_per_item, T - bj);
+      auto& dst = cnt == tiles_per_item ? full : part;
+      dst.push_back(a);
       dst.push_back((bi << 16) | bj);
       dst.push_back(cnt);
+      dst.push_back(0);
     }
-  items = std::move(full);
-  items.insert(items.end(), part.begin(), part.end());
 }
 
-hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, uint32_t n, int32_t qcut,
+hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, const AsmDesc* desc,
                             const uint32_t* perm, const uint32_t* items, uint32_t nitems,
                             uint64_t* cand, unsigned long long* ncand, uint64_t cap,
                             hipStream_t st) {
   if (nitems == 0) return hipSuccess;
-  const uint2* it = reinterpret_cast<const uint2*>(items);
+  const uint4* it = reinterpret_cast<const uint4*>(items);
   switch (kpad / 32) {
-    case 1: hipLaunchKernelGGL(gate_mfma_kernel<1>, dim3(nitems), dim3(256), 0, st, q, n, qcut, perm, it, cand, ncand, cap); break;
-    case 2: hipLaunchKernelGGL(gate_mfma_kernel<2>, dim3(nitems), dim3(256), 0, st, q, n, qcut, perm, it, cand, ncand, cap); break;
+    case 1: hipLaunchKernelGGL(gate_mfma_kernel<1>, dim3(nitems), dim3(256), 0, st, q, desc, perm, it, cand, ncand, cap); break;
+    case 2: hipLaunchKernelGGL(gate_mfma_kernel<2>, dim3(nitems), dim3(256), 0, st, q, desc, perm, it, cand, ncand, cap); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_gate_compare(uint64_t* cand, uint64_t nc, const float* unit, uint32_t NC,
-                               const uint8_t* anynz, float thr, uint32_t acc1_end,
+hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
+                               const float* unit, uint32_t NC, const uint8_t* anynz,
+                               const AsmDesc* desc, const uint16_t* asm_of,
                                const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st) {
-  if (nc == 0) return hipSuccess;
-  const dim3 grid((unsigned)((nc * 4 + 255) / 256));
-#define RBG_GC(NCV, WV)                                                                          \
-  hipLaunchKernelGGL((gate_compare_kernel<NCV, WV>), grid, dim3(256), 0, st, cand, nc, unit, anynz, \
-                     thr, acc1_end, planes, deg)
+  if (cap == 0) return hipSuccess;
+  const dim3 grid((unsigned)std::min<uint64_t>((cap * 4 + 255) / 256, 65536));
+#define RBG_GC(NCV, WV)                                                                            \
+  hipLaunchKernelGGL((gate_compare_kernel<NCV, WV>), grid, dim3(256), 0, st, cand, cap, ncand, unit, \
+                     anynz, desc, asm_of, planes, deg)
 #define RBG_GC_W(NCV)                   \
   switch (W) {                          \
     case 16: RBG_GC(NCV, 16); break;    \
@@ -1660,51 +1770,34 @@ hipError_t launch_gate_compare(uint64_t* cand, uint64_t nc, const float* unit, u
   return hipGetLastError();
 }
 
-hipError_t launch_scatter(const uint64_t* surv, uint64_t ns, const uint64_t* off, uint32_t* cursor,
-                          uint64_t* list, hipStream_t st) {
-  if (ns == 0) return hipSuccess;
-  hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, surv, ns,
-                     off, cursor, list);
+hipError_t launch_scatter(const uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
+                          const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st) {
+  if (cap == 0) return hipSuccess;
+  const dim3 grid((unsigned)std::min<uint64_t>((cap + 255) / 256, 65536));
+  hipLaunchKernelGGL(scatter_kernel, grid, dim3(256), 0, st, cand, cap, ncand, off, cursor, list);
   return hipGetLastError();
 }
 
-hipError_t launch_select_dense(const uint16_t* mat, const uint8_t* anynz, uint32_t n,
-                               const uint16_t* level, uint32_t nlevels, uint32_t K,
-                               uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                               uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st) {
-  RowSource s{};
-  s.mat = mat;
-  s.anynz = anynz;
-  s.n = n;
-  s.dense = true;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(select_topk_kernel, dim3(n), dim3(256), 0, st, s, n, level, nlevels, K, topk_id,
-                     topk_cnt, topk_len, rowmax, fence, FW);
-  return hipGetLastError();
-}
-
-hipError_t launch_select_list(const uint64_t* list, const uint64_t* off, uint32_t n,
-                              const uint16_t* level, uint32_t nlevels, uint32_t K,
-                              uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                              uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st) {
-  RowSource s{};
-  s.list = list;
-  s.off = off;
-  s.n = n;
-  s.dense = false;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(select_topk_kernel, dim3(n), dim3(256), 0, st, s, n, level, nlevels, K, topk_id,
-                     topk_cnt, topk_len, rowmax, fence, FW);
+hipError_t launch_select(const uint64_t* list, const uint64_t* off, const uint16_t* mat,
+                         const uint8_t* anynz, const AsmDesc* desc, const uint16_t* asm_of,
+                         uint32_t nrows, const uint16_t* level, uint32_t m, uint32_t K,
+                         uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
+                         uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st) {
+  if (nrows == 0) return hipSuccess;
+  if (m + 1 > 1024 || K > 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(select_topk_kernel, dim3(nrows), dim3(256), 0, st, list, off, mat, anynz, desc,
+                     asm_of, level, m, K, topk_id, topk_cnt, topk_len, rowmax, fence, FW);
   return hipGetLastError();
 }
 
 hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
                             const uint32_t* topk_len, const uint32_t* fence, uint32_t FW,
-                            uint32_t n, uint32_t K, const BuildScalars* bs, int mutual,
-                            uint64_t* keep, uint32_t* row_edges, hipStream_t st) {
+                            uint32_t n, uint32_t K, const BuildScalars* bs,
+                            const AsmDesc* desc, const uint16_t* asm_of, uint64_t* keep,
+                            uint32_t* row_edges, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(emit_mark_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt,
-                     topk_len, fence, FW, n, K, bs, mutual, (K + 63) / 64, keep, row_edges);
+                     topk_len, fence, FW, n, K, bs, desc, asm_of, (K + 63) / 64, keep, row_edges);
   return hipGetLastError();
 }
 
@@ -1718,41 +1811,47 @@ hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
   return hipGetLastError();
 }
 
+hipError_t launch_edge_weights_batch(const uint32_t* efrom, const uint32_t* eto,
+                                     const float* scomp, uint64_t cap, const uint64_t* ne,
+                                     const float* unit, const float* raw, uint32_t NC,
+                                     const double* colnorm, const AsmDesc* desc,
+                                     const uint16_t* asm_of, double min_edge, double neg_thr,
+                                     float* score, hipStream_t st) {
+  if (cap == 0) return hipSuccess;
+  const dim3 grid((unsigned)std::min<uint64_t>((cap + 255) / 256, 65536));
+#define RBG_EW(NCV)                                                                                \
+  hipLaunchKernelGGL(edge_weight_pad_kernel<NCV>, grid, dim3(256), 0, st, efrom, eto, scomp, cap, ne, \
+                     unit, raw, colnorm, desc, asm_of, min_edge, neg_thr, score)
+  switch (NC) {
+    case 1: RBG_EW(1); break;
+    case 2: RBG_EW(2); break;
+    case 3: RBG_EW(3); break;
+    case 4: RBG_EW(4); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef RBG_EW
+  return hipGetLastError();
+}
+
 hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const float* scomp,
                                uint64_t ne, const float* unit, const float* raw,
                                const double* colnorm, uint32_t S, double w_comp, double min_edge,
-                               double neg_thr, float* score, double* score64, hipStream_t st,
-                               bool padded, const uint64_t* ne_dev) {
+                               double neg_thr, float* score, double* score64, hipStream_t st) {
   if (ne == 0) return hipSuccess;
-  // with a device-resident count, ne is the capacity: a grid-stride launch
-  const dim3 grid((unsigned)std::min<uint64_t>((ne + 255) / 256, ne_dev ? 4096 : UINT32_MAX));
-  const uint32_t NC = (S + 15) / 16;
-  if (padded && S > 1 && NC <= 4) {
-#define RBG_EW(NCV)                                                                               \
-  hipLaunchKernelGGL(edge_weight_pad_kernel<NCV>, grid, dim3(256), 0, st, efrom, eto, scomp, ne, \
-                     unit, raw, colnorm, S, w_comp, min_edge, neg_thr, score, score64, ne_dev)
-    switch (NC) {
-      case 1: RBG_EW(1); break;
-      case 2: RBG_EW(2); break;
-      case 3: RBG_EW(3); break;
-      default: RBG_EW(4); break;
-    }
-#undef RBG_EW
-    return hipGetLastError();
-  }
-  hipLaunchKernelGGL(edge_weight_kernel, grid, dim3(256), 0, st, efrom, eto, scomp, ne, unit, raw,
-                     colnorm, S, w_comp, min_edge, neg_thr, score, score64, ne_dev);
+  hipLaunchKernelGGL(edge_weight_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, efrom,
+                     eto, scomp, ne, unit, raw, colnorm, S, w_comp, min_edge, neg_thr, score, score64);
   return hipGetLastError();
 }
 
-hipError_t launch_cnt_to_sv(const uint16_t* c, uint64_t ne, const float* svf, float* out,
-                            hipStream_t st, const uint64_t* ne_dev) {
-  if (ne == 0) return hipSuccess;
-  const dim3 grid((unsigned)std::min<uint64_t>((ne + 255) / 256, ne_dev ? 4096 : UINT32_MAX));
-  hipLaunchKernelGGL(u16_to_sv_kernel, grid, dim3(256), 0, st, c, ne, ne_dev, svf, out);
+hipError_t launch_cnt_to_sv(const uint16_t* ecnt, uint64_t cap, const uint64_t* ne,
+                            const uint32_t* efrom, const uint16_t* asm_of, uint32_t m,
+                            const float* svf, float* scomp, hipStream_t st) {
+  if (cap == 0) return hipSuccess;
+  const dim3 grid((unsigned)std::min<uint64_t>((cap + 255) / 256, 65536));
+  hipLaunchKernelGGL(u16_to_sv_kernel, grid, dim3(256), 0, st, ecnt, cap, ne, efrom, asm_of, m, svf,
+                     scomp);
   return hipGetLastError();
 }
-
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n, void* tmp,
                                      size_t& tmp_bytes, hipStream_t st) {
   // n + 1 outputs; blocks cover 1024 inputs each (the last block also writes out[n]).
@@ -1768,20 +1867,57 @@ hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_baseline_tables(const uint32_t* pcnt, uint32_t np, uint32_t m, BuildScalars* bs,
-                                  double* svd, float* svf, uint16_t* level, hipStream_t st) {
+hipError_t launch_baseline_tables(const uint32_t* pcnt, const AsmDesc* desc, uint32_t nasm,
+                                  uint32_t m, BuildScalars* bs, double* svd, float* svf,
+                                  uint16_t* level, hipStream_t st) {
   if (m < 2 || m > 512) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(baseline_tables_kernel, dim3(1), dim3(256), 0, st, pcnt, np, m, bs, svd, svf, level);
+  if (nasm == 0) return hipSuccess;
+  hipLaunchKernelGGL(baseline_tables_kernel, dim3(nasm), dim3(256), 0, st, pcnt, desc, m, bs, svd,
+                     svf, level);
   return hipGetLastError();
 }
 
-hipError_t launch_calibrate(const uint32_t* rowmax, uint32_t n, uint32_t m, const double* svd,
-                            const float* svf, const int32_t* rnd, uint32_t nrnd,
-                            const uint32_t* members, const uint32_t* round_off, int regime,
-                            double coverage, BuildScalars* bs, hipStream_t st) {
-  if (m < 2 || m > 512 || (regime != 1 && regime != 2)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(calibrate_kernel, dim3(1), dim3(256), 0, st, rowmax, n, m, svd, svf, rnd, nrnd,
-                     members, round_off, regime, coverage, bs);
+hipError_t launch_calibrate(const uint32_t* rowmax, const AsmDesc* desc, uint32_t nasm, uint32_t m,
+                            const double* svd, const float* svf, const int32_t* rnd,
+                            const uint32_t* members, const uint32_t* round_off, double coverage,
+                            BuildScalars* bs, hipStream_t st) {
+  if (m < 2 || m > 512) return hipErrorInvalidValue;
+  if (nasm == 0) return hipSuccess;
+  hipLaunchKernelGGL(calibrate_kernel, dim3(nasm), dim3(256), 0, st, rowmax, desc, m, svd, svf, rnd,
+                     members, round_off, coverage, bs);
+  return hipGetLastError();
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t count, uint32_t value) {
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < count;
+       t += (uint64_t)gridDim.x * 256)
+    p[t] = value;
+}
+
+__global__ void gather_bounds_kernel(const uint64_t* __restrict__ row_off,
+                                     const uint64_t* __restrict__ off, const AsmDesc* __restrict__ desc,
+                                     uint32_t nasm, uint64_t* __restrict__ out) {
+  const uint32_t a = blockIdx.x * 64 + threadIdx.x;
+  if (a >= nasm) return;
+  const uint32_t r0 = desc[a].row0, r1 = r0 + desc[a].n;
+  out[4 * a] = row_off[r0];
+  out[4 * a + 1] = row_off[r1];
+  out[4 * a + 2] = off ? off[r0] : 0;
+  out[4 * a + 3] = off ? off[r1] : 0;
+}
+
+hipError_t launch_gather_bounds(const uint64_t* row_off, const uint64_t* off, const AsmDesc* desc,
+                                uint32_t nasm, uint64_t* out, hipStream_t st) {
+  if (nasm == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_bounds_kernel, dim3((nasm + 63) / 64), dim3(64), 0, st, row_off, off,
+                     desc, nasm, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  const dim3 grid((unsigned)std::min<uint64_t>((count + 255) / 256, 65536));
+  hipLaunchKernelGGL(fill_u32_kernel, grid, dim3(256), 0, st, p, count, value);
   return hipGetLastError();
 }
 
