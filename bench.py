@@ -81,6 +81,175 @@ def cpu_baseline(seqs, offsets, depth, n, budget_s, threads):
     }
 
 
+def pctl(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def batched_form(ctx, steps):
+    """The batched workload (64 assemblies): one batched build per step over
+    all of them (autotuned schedule, replayed from a captured hipGraph) against
+    the plain per-assembly path (each assembly resident, built alone), both
+    with inputs already in HBM."""
+    from rabbitbin_amd import api, synth
+    t0 = time.perf_counter()
+    asms, spec = synth.batch_workload(64)
+    pairs = sum(n * (n - 1) // 2 for n, _, _ in spec)
+    log(f"[bench/batched] generated {len(asms)} assemblies, {sum(n for n, _, _ in spec)} contigs in "
+        f"{time.perf_counter() - t0:.1f}s")
+    params = api.default_params(rand_seed=1)
+    plain_ms, plain_edges = [], []
+    for seqs, off, depth in asms:
+        a = api.Assembly(ctx, seqs, off, depth)
+        a.build_graph(params, on_device=True)
+        best = None
+        for _ in range(2):
+            t1 = time.perf_counter()
+            r = a.build_graph(params, on_device=True)
+            dt = 1e3 * (time.perf_counter() - t1)
+            best = dt if best is None else min(best, dt)
+        plain_ms.append(best)
+        plain_edges.append(int(r["n_edges"]))
+        a.free()
+    b = api.Batch(ctx, asms)
+    (tiles, rows), tune_ms = b.autotune(params, reps=2)
+    first = b.build(params, capture=True)
+    times = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        st = b.build(params, capture=True)
+        times.append(1e3 * (time.perf_counter() - t1))
+    assert st["replayed"] and not first["replayed"]
+    assert st["edges"] == sum(plain_edges), (st["edges"], sum(plain_edges))
+    b.free()
+    ms = sum(times) / len(times)
+    plain_total = sum(plain_ms)
+    out = {"workload": "batch64: 64 assemblies, contigs log-uniform [500, 40000], 50-400 genomes, S in "
+                       "{3, 10, 30}, m=500, max-edges=200",
+           "assemblies": len(asms), "contigs": sum(n for n, _, _ in spec), "pairs": pairs,
+           "ms_per_step": ms, "value": pairs / (ms / 1e3), "unit": "pairs/s",
+           "plain_ms_total": plain_total, "plain_value": pairs / (plain_total / 1e3),
+           "speedup_vs_plain": plain_total / ms, "edges": int(st["edges"]),
+           "schedule": {"gate_tiles_per_item": tiles, "dense_rows_per_item": rows, "autotune_ms": tune_ms},
+           "note": "inputs resident in HBM for both; plain = one resident assembly built at a time (best of 2)"}
+    log(f"[bench/batched] {ms:.1f} ms/step vs plain {plain_total:.1f} ms: x{plain_total / ms:.2f}")
+    return out
+
+
+def served_form(count, rate):
+    """The served workload: sessions arriving as a Poisson process, each then
+    extended 5 times by 2,000 contigs (each extension submitted when the
+    previous result returns).  Reports p50/p99 latency and pairs/s (pairs of
+    the graphs returned: n(n-1)/2 per result) at the offered rate, the same at
+    saturation (every session arriving at once), and the plain path on the
+    same request sequence (every request a full rebuild from host input, one
+    at a time, FIFO latencies simulated from its measured service times)."""
+    import heapq
+    import threading
+    import numpy as np
+    from rabbitbin_amd import api, synth
+    t0 = time.perf_counter()
+    arrivals, pieces = synth.served_workload(count, rate=rate)
+    log(f"[bench/served] generated {count} sessions ({sum(len(p[0][1]) - 1 for p in pieces)} initial contigs) "
+        f"in {time.perf_counter() - t0:.1f}s")
+
+    def run(arr):
+        recs = []
+        lock = threading.Lock()
+        with api.Server(0) as srv:
+            # warm the server's device state with one small request
+            w = pieces[0][1]
+            ses, r = srv.submit(*w)
+            srv.wait(r)
+            srv.release(ses)
+            base = time.perf_counter()
+
+            def client(k):
+                d = base + arr[k] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ses, r = srv.submit(*pieces[k][0])
+                res, info = srv.wait(r)
+                with lock:
+                    recs.append(info)
+                for t in range(1, len(pieces[k])):
+                    r = srv.extend(ses, *pieces[k][t])
+                    res, info = srv.wait(r)
+                    with lock:
+                        recs.append(info)
+                srv.release(ses)
+
+            th = [threading.Thread(target=client, args=(k,)) for k in range(count)]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            st = srv.stats()
+        t_first = min(i["t_submit"] for i in recs)
+        t_last = max(i["t_done"] for i in recs)
+        gp = sum(i["n"] * (i["n"] - 1) // 2 for i in recs)
+        lat = [i["latency_ms"] for i in recs]
+        big = [i["latency_ms"] for i in recs if i["n"] >= 40000 and i["n_new"] == i["n"]]
+        return {"requests": len(recs), "makespan_s": t_last - t_first, "graph_pairs": gp,
+                "value": gp / (t_last - t_first), "p50_ms": pctl(lat, 50), "p99_ms": pctl(lat, 99),
+                "p99_ms_assemblies_ge_40k": pctl(big, 99), "steps": st["steps"],
+                "mean_requests_per_step": len(recs) / max(1, st["steps"]),
+                "incremental": sum(i["incremental"] for i in recs), "busy_ms": st["busy_ms"]}
+
+    offered = run(arrivals)
+    log(f"[bench/served] offered {rate}/s: {offered}")
+    saturated = run(np.zeros(count))
+    log(f"[bench/served] saturated: {saturated}")
+    # plain path: every request a full rebuild of the session's assembly from host input
+    params = api.default_params(rand_seed=1)
+    service = {}
+    with api.Context(0) as ctx:
+        w = pieces[0][1]
+        ctx.build_graph(w[0], w[1], w[2], params)  # warm
+        for k in range(count):
+            for t in range(len(pieces[k])):
+                parts = pieces[k][:t + 1]
+                seqs = np.concatenate([p[0] for p in parts])
+                offs = [np.zeros(1, np.uint64)]
+                base = 0
+                for p in parts:
+                    offs.append(p[1][1:] + np.uint64(base))
+                    base += int(p[1][-1])
+                offs = np.concatenate(offs)
+                depth = np.concatenate([p[2] for p in parts])
+                t1 = time.perf_counter()
+                ctx.build_graph(seqs, offs, depth, params)
+                service[(k, t)] = time.perf_counter() - t1
+    T = len(pieces[0])
+    ns = {(k, t): sum(len(p[1]) - 1 for p in pieces[k][:t + 1]) for k in range(count) for t in range(T)}
+
+    def simulate(arr):
+        heap = [(float(arr[k]), k, 0) for k in range(count)]
+        heapq.heapify(heap)
+        free, lat, last = 0.0, [], 0.0
+        while heap:
+            ready, k, t = heapq.heappop(heap)
+            done = max(free, ready) + service[(k, t)]
+            free = done
+            lat.append(1e3 * (done - ready))
+            last = max(last, done)
+            if t + 1 < T:
+                heapq.heappush(heap, (done, k, t + 1))
+        return lat, last - float(min(arr))
+
+    gp = sum(n * (n - 1) // 2 for n in ns.values())
+    lat_o, span_o = simulate(arrivals)
+    lat_s, span_s = simulate(np.zeros(count))
+    plain = {"service_s_total": sum(service.values()), "value": gp / sum(service.values()),
+             "offered": {"p50_ms": pctl(lat_o, 50), "p99_ms": pctl(lat_o, 99), "makespan_s": span_o},
+             "saturated": {"p50_ms": pctl(lat_s, 50), "p99_ms": pctl(lat_s, 99), "makespan_s": span_s}}
+    log(f"[bench/served] plain: {plain}")
+    return {"workload": f"served: {count} sessions, Poisson arrivals {rate}/s, contigs log-uniform "
+                        f"[1000, 100000], S in [3, 30], + 5 x 2000-contig extensions each",
+            "unit": "pairs/s", "offered": offered, "saturated": saturated, "plain": plain,
+            "speedup_vs_plain_saturated": saturated["value"] / plain["value"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +259,10 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="override the contig count")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="plain,batched,served",
+                    help="comma list of: plain (the headline line), batched, served")
+    ap.add_argument("--served-sessions", type=int, default=24)
+    ap.add_argument("--served-rate", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -169,8 +342,19 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": repr(e)}
     asm.free()
-    ctx.close()
     del ctx_keep
+    forms = args.forms.split(",")
+    if "batched" in forms:
+        try:
+            out["batched"] = batched_form(ctx, args.steps)
+        except Exception as e:  # reported beside the headline, never required for it
+            out["batched"] = {"error": repr(e)}
+    ctx.close()
+    if "served" in forms:
+        try:
+            out["served"] = served_form(args.served_sessions, args.served_rate)
+        except Exception as e:
+            out["served"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 

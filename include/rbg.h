@@ -37,6 +37,7 @@ typedef int rbg_status;
 #define RBG_ERR_NOMEM (-3)       /* device or host allocation failed */
 #define RBG_ERR_UNSUPPORTED (-4) /* configuration outside the implemented path */
 #define RBG_ERR_BUDGET (-5)      /* request exceeds the HBM budget */
+#define RBG_ERR_TIMEOUT (-6)     /* a wait ended before the request was done */
 
 typedef struct rbg_ctx rbg_ctx;
 
@@ -191,8 +192,87 @@ rbg_status rbg_batch_set_schedule(rbg_asm* b, const rbg_schedule* s);
  * edges stay in HBM until rbg_batch_result). */
 rbg_status rbg_batch_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* params,
                            rbg_batch_stats* stats);
+/* Autotuner of the batch's work schedule: builds the batch (eagerly) under
+ * candidate schedules, one dimension at a time (gate tiles per work item in
+ * {2, 4, 8, 16, 32} when the batch has fused assemblies, then dense rows per
+ * work item in {16, 32, 64, 128, 256} when it has small ones), times each
+ * build with HIP events on the context's stream (best of `reps`), and keeps
+ * the fastest schedule in the batch (and in *best, with its time in
+ * *best_ms).  The batch is left built under that schedule.  Results do not
+ * depend on the schedule; only the time does. */
+rbg_status rbg_batch_autotune(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* params, uint32_t reps,
+                              rbg_schedule* best, double* best_ms);
 /* Assembly `index` of the last build: scalars and the edge list on the host. */
 rbg_status rbg_batch_result(rbg_ctx* ctx, rbg_asm* b, uint32_t index, rbg_graph_result* out);
+
+/* ---------------------------------------------------------------------------
+ * Served form: a request loop on one device.  rbg_server_submit starts a
+ * session with an assembly; rbg_server_extend adds a batch of contigs to a
+ * session's assembly (appended after its earlier contigs, with the session's
+ * sample count).  A worker thread serves pending requests in steps: each
+ * step is one batched build over the requests admitted to it (at most one per
+ * session, in submission order), so requests that arrive while others are on
+ * the device share the next step's launches.  Admission keeps the device
+ * memory of the resident sessions plus the step's working set within
+ * hbm_budget; a request that cannot fit even alone fails with RBG_ERR_BUDGET.
+ * A session keeps its sketches, depth rows and (fused regime) gated
+ * candidate lists on the device: an extension uploads and sketches only its
+ * new contigs and evaluates only the pairs that involve them.  Every result
+ * equals rbg_build_graph on the session's whole assembly as of that request
+ * (ids local to the session: earlier contigs first, in order).
+ * Input buffers are caller-owned and must stay valid until the request's
+ * rbg_server_wait returns.  All calls are thread-safe.
+ * No counterpart in the reference (it builds one assembly per process).
+ * ------------------------------------------------------------------------- */
+typedef struct rbg_server rbg_server;
+
+typedef struct rbg_server_config {
+  uint64_t hbm_budget;          /* device bytes for sessions + step working set (0: 90% of free) */
+  uint32_t max_step_requests;   /* requests per step (default 64) */
+  uint32_t max_step_rows;       /* contigs per step, < 2^24 (default 2^22) */
+  rbg_graph_params params;      /* build parameters of every request (flags ignored) */
+  rbg_schedule schedule;        /* pair-pass work schedule (see rbg_batch_autotune) */
+} rbg_server_config;
+
+typedef struct rbg_request_info {
+  uint64_t session;
+  uint32_t n;              /* contigs of the session's assembly after this request */
+  uint32_t n_new;          /* contigs this request added */
+  int incremental;         /* 1: only pairs with a new contig were evaluated */
+  int status;              /* the request's rbg_status */
+  uint64_t step;           /* the step that served it (1-based) */
+  uint32_t step_requests;  /* requests in that step */
+  uint64_t pass_pairs;     /* pairs the step evaluated for it */
+  double t_submit, t_admit, t_done;  /* seconds on the server's clock */
+  double latency_ms;       /* t_done - t_submit */
+  char error[160];         /* message when status != 0 */
+} rbg_request_info;
+
+typedef struct rbg_server_stats {
+  uint64_t steps, requests, sessions, pending, running;
+  uint64_t graph_pairs;    /* sum over served requests of n(n-1)/2 of their graphs */
+  uint64_t pass_pairs;     /* pairs evaluated */
+  uint64_t resident_bytes, workspace_bytes, budget;
+  double busy_ms;          /* wall time inside steps */
+} rbg_server_stats;
+
+void rbg_server_default_config(rbg_server_config* c);
+rbg_status rbg_server_start(int device, const rbg_server_config* cfg, rbg_server** out);
+/* Fails the requests still pending, waits for the running step, frees all. */
+void rbg_server_stop(rbg_server* s);
+const char* rbg_server_last_error(const rbg_server* s);
+rbg_status rbg_server_submit(rbg_server* s, const rbg_assembly_in* a, uint64_t* session,
+                             uint64_t* request);
+rbg_status rbg_server_extend(rbg_server* s, uint64_t session, const rbg_assembly_in* more,
+                             uint64_t* request);
+/* Blocks until the request is done (timeout_s < 0: no limit) and hands over
+ * its result (free with rbg_graph_result_free) and info; returns the
+ * request's status.  Each request is waited for once. */
+rbg_status rbg_server_wait(rbg_server* s, uint64_t request, double timeout_s,
+                           rbg_graph_result* out, rbg_request_info* info);
+/* Frees the session's device state once its queued requests are done. */
+rbg_status rbg_server_release(rbg_server* s, uint64_t session);
+rbg_status rbg_server_get_stats(rbg_server* s, rbg_server_stats* out);
 
 /* ---------------------------------------------------------------------------
  * Stage (d) alone (src/rabbitbin.cpp:5482-5509): edge weights for a given

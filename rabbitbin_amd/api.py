@@ -81,6 +81,25 @@ class GraphResult(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_step_requests", C.c_uint32), ("max_step_rows", C.c_uint32),
+                ("params", GraphParams), ("schedule", Schedule)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("session", C.c_uint64), ("n", C.c_uint32), ("n_new", C.c_uint32), ("incremental", C.c_int),
+                ("status", C.c_int), ("step", C.c_uint64), ("step_requests", C.c_uint32),
+                ("pass_pairs", C.c_uint64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("latency_ms", C.c_double), ("error", C.c_char * 160)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("requests", C.c_uint64), ("sessions", C.c_uint64),
+                ("pending", C.c_uint64), ("running", C.c_uint64), ("graph_pairs", C.c_uint64),
+                ("pass_pairs", C.c_uint64), ("resident_bytes", C.c_uint64), ("workspace_bytes", C.c_uint64),
+                ("budget", C.c_uint64), ("busy_ms", C.c_double)]
+
+
 class GraphDiag(C.Structure):
     _fields_ = [
         ("topk_id", u32p),
@@ -127,6 +146,18 @@ def lib():
     L.rbg_batch_set_schedule.argtypes = [vp, C.POINTER(Schedule)]
     L.rbg_batch_build.argtypes = [vp, vp, C.POINTER(GraphParams), C.POINTER(BatchStats)]
     L.rbg_batch_result.argtypes = [vp, vp, C.c_uint32, C.POINTER(GraphResult)]
+    L.rbg_batch_autotune.argtypes = [vp, vp, C.POINTER(GraphParams), C.c_uint32, C.POINTER(Schedule),
+                                     C.POINTER(C.c_double)]
+    L.rbg_server_default_config.argtypes = [C.POINTER(ServerConfig)]
+    L.rbg_server_start.argtypes = [C.c_int, C.POINTER(ServerConfig), C.POINTER(vp)]
+    L.rbg_server_stop.argtypes = [vp]
+    L.rbg_server_last_error.argtypes = [vp]
+    L.rbg_server_last_error.restype = C.c_char_p
+    L.rbg_server_submit.argtypes = [vp, C.POINTER(AssemblyIn), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.rbg_server_extend.argtypes = [vp, C.c_uint64, C.POINTER(AssemblyIn), C.POINTER(C.c_uint64)]
+    L.rbg_server_wait.argtypes = [vp, C.c_uint64, C.c_double, C.POINTER(GraphResult), C.POINTER(RequestInfo)]
+    L.rbg_server_release.argtypes = [vp, C.c_uint64]
+    L.rbg_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
     _lib = L
     return L
 
@@ -143,13 +174,25 @@ def default_params(**kw):
     return p
 
 
+class _Seqs:
+    """Contig bytes passed by address (no copy); keeps the array alive."""
+
+    def __init__(self, seqs):
+        if isinstance(seqs, (bytes, bytearray)):
+            seqs = np.frombuffer(bytes(seqs), dtype=np.uint8)
+        self.arr = np.ascontiguousarray(seqs, dtype=np.uint8)
+        if self.arr.size == 0:
+            self.arr = np.zeros(1, np.uint8)
+        self.ptr = C.c_char_p(self.arr.ctypes.data)
+
+    @property
+    def _as_parameter_(self):
+        return self.ptr
+
+
 def _as_seqs(seqs, offsets):
-    if isinstance(seqs, (bytes, bytearray)):
-        buf = bytes(seqs)
-    else:
-        buf = np.ascontiguousarray(seqs, dtype=np.uint8).tobytes()
     off = np.ascontiguousarray(offsets, dtype=np.uint64)
-    return buf, off
+    return _Seqs(seqs), off
 
 
 class Context:
@@ -317,7 +360,7 @@ class Batch:
             dep = None if depth is None else np.ascontiguousarray(depth, np.float32)
             S = 0 if dep is None else dep.shape[1]
             self._keep += [buf, off, dep]
-            ins[k] = AssemblyIn(buf, _p(off, u64p), len(off) - 1, _p(dep, f32p), S)
+            ins[k] = AssemblyIn(buf.ptr, _p(off, u64p), len(off) - 1, _p(dep, f32p), S)
             self.sizes.append(len(off) - 1)
         self._h = C.c_void_p()
         ctx._check(ctx._L.rbg_batch_create(ctx._h, ins, len(assemblies), min_cv, C.byref(self._h)),
@@ -340,6 +383,15 @@ class Batch:
         return {"candidates": stats.candidates, "edges": stats.edges, "ms": stats.ms_total,
                 "replayed": bool(stats.replayed)}
 
+    def autotune(self, params=None, reps=2):
+        """rbg_batch_autotune: ((gate_tiles_per_item, dense_rows_per_item), ms) of the
+        fastest schedule; the batch keeps it and is left built under it."""
+        p = GraphParams.from_buffer_copy(params or default_params())
+        best, ms = Schedule(), C.c_double()
+        self.ctx._check(self.ctx._L.rbg_batch_autotune(self.ctx._h, self._h, C.byref(p), reps, C.byref(best),
+                                                       C.byref(ms)), "rbg_batch_autotune")
+        return (best.gate_tiles_per_item, best.dense_rows_per_item), ms.value
+
     def result(self, index):
         res = GraphResult()
         self.ctx._check(self.ctx._L.rbg_batch_result(self.ctx._h, self._h, index, C.byref(res)),
@@ -350,6 +402,116 @@ class Batch:
         if self._h:
             self.ctx._L.rbg_asm_free(self._h)
             self._h = C.c_void_p()
+
+
+TIMEOUT = -6
+
+
+class Server:
+    """The served form (rbg_server_*): sessions of device-resident assemblies,
+    served by a worker thread in batched steps under an HBM budget.
+
+    submit(seqs, offsets, depth) starts a session; extend(session, ...) adds
+    contigs to it; wait(request) returns (result dict, info dict).  Input
+    arrays are kept referenced here until their request has been waited for
+    (the library reads them while the request runs)."""
+
+    def __init__(self, device=0, hbm_budget=0, max_step_requests=64, max_step_rows=1 << 22, params=None,
+                 schedule=None):
+        self._L = lib()
+        cfg = ServerConfig()
+        self._L.rbg_server_default_config(C.byref(cfg))
+        cfg.hbm_budget = hbm_budget
+        cfg.max_step_requests = max_step_requests
+        cfg.max_step_rows = max_step_rows
+        if params is not None:
+            cfg.params = params
+        if schedule is not None:
+            cfg.schedule = Schedule(*schedule)
+        self.params = GraphParams.from_buffer_copy(cfg.params)
+        self._h = C.c_void_p()
+        st = self._L.rbg_server_start(device, C.byref(cfg), C.byref(self._h))
+        if st != 0:
+            raise RbgError(f"rbg_server_start: status {st}")
+        self._inputs = {}
+
+    def _err(self):
+        return self._L.rbg_server_last_error(self._h).decode()
+
+    @staticmethod
+    def _in(seqs, offsets, depth):
+        buf, off = _as_seqs(seqs, offsets)
+        dep = None if depth is None else np.ascontiguousarray(depth, np.float32)
+        S = 0 if dep is None else dep.shape[1]
+        return AssemblyIn(buf.ptr, _p(off, u64p), len(off) - 1, _p(dep, f32p), S), (buf, off, dep)
+
+    def submit(self, seqs, offsets, depth=None):
+        """New session; returns (session, request)."""
+        a, keep = self._in(seqs, offsets, depth)
+        ses, req = C.c_uint64(), C.c_uint64()
+        st = self._L.rbg_server_submit(self._h, C.byref(a), C.byref(ses), C.byref(req))
+        if st != 0:
+            raise RbgError(f"rbg_server_submit: status {st}: {self._err()}")
+        self._inputs[req.value] = keep
+        return ses.value, req.value
+
+    def extend(self, session, seqs, offsets, depth=None):
+        """Contigs appended to the session's assembly; returns the request."""
+        a, keep = self._in(seqs, offsets, depth)
+        req = C.c_uint64()
+        st = self._L.rbg_server_extend(self._h, session, C.byref(a), C.byref(req))
+        if st != 0:
+            raise RbgError(f"rbg_server_extend: status {st}: {self._err()}")
+        self._inputs[req.value] = keep
+        return req.value
+
+    def wait(self, request, timeout=-1.0):
+        res, info = GraphResult(), RequestInfo()
+        st = self._L.rbg_server_wait(self._h, request, timeout, C.byref(res), C.byref(info))
+        if st == TIMEOUT:
+            raise TimeoutError(f"request {request} not done within {timeout} s")
+        self._inputs.pop(request, None)
+        d = {k: getattr(info, k) for k, _ in RequestInfo._fields_ if k != "error"}
+        d["error"] = info.error.decode()
+        if st != 0:
+            raise RbgError(f"request {request}: status {st}: {d['error']}")
+        ne = res.n_edges
+
+        def arr(ptr):
+            return np.ctypeslib.as_array(ptr, (ne,)).copy() if (ne and ptr) else None
+
+        out = {"n_edges": ne, "from": arr(res.from_), "to": arr(res.to), "scomp": arr(res.scomp),
+               "score": arr(res.score), "sim_cutoff": res.sim_cutoff, "b0": res.b0, "regime": res.regime,
+               "gated_pairs": res.gated_pairs}
+        for k, dt in (("from", np.uint32), ("to", np.uint32), ("scomp", np.float32)):
+            if out[k] is None:
+                out[k] = np.zeros(0, dt)
+        self._L.rbg_graph_result_free(C.byref(res))
+        return out, d
+
+    def release(self, session):
+        st = self._L.rbg_server_release(self._h, session)
+        if st != 0:
+            raise RbgError(f"rbg_server_release: status {st}: {self._err()}")
+
+    def stats(self):
+        s = ServerStats()
+        st = self._L.rbg_server_get_stats(self._h, C.byref(s))
+        if st != 0:
+            raise RbgError(f"rbg_server_get_stats: status {st}")
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def stop(self):
+        if self._h:
+            self._L.rbg_server_stop(self._h)
+            self._h = C.c_void_p()
+            self._inputs.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
 
 
 def depth_features(depth, min_cv=1.0):

@@ -1,0 +1,806 @@
+// librbg: the graph-build pipeline over a batch of device-resident assemblies
+// (upload, plan, launch sequence, results), shared by the C ABI and the
+// request server.
+#include "rbg_build.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <numeric>
+
+using namespace rbg;
+
+namespace rbg {
+
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+hipError_t DevBuf::ensure(size_t nbytes) {
+  if (nbytes <= bytes && p) return hipSuccess;
+  release();
+  if (nbytes == 0) nbytes = 1;
+  hipError_t e = hipMalloc(&p, nbytes);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return e;
+  }
+  bytes = nbytes;
+  return hipSuccess;
+}
+
+uint32_t words_for(uint32_t m) {
+  uint32_t w = (m + 31) / 32, p = 1;
+  while (p < w) p <<= 1;
+  return p;
+}
+
+rbg_status get_tables(rbg_ctx* ctx, uint32_t m, uint64_t seed, rbg_ctx::Tables** out) {
+  for (auto& t : ctx->tables)
+    if (t->m == m && t->seed == seed) {
+      *out = t.get();
+      return RBG_OK;
+    }
+  const ProposalTables& pt = proposal_tables(m, seed);
+  auto t = std::make_unique<rbg_ctx::Tables>();
+  t->m = m;
+  t->seed = seed;
+  RBG_TRY_HIP(ctx, t->T.ensure(pt.T.size() * sizeof(double)));
+  RBG_TRY_HIP(ctx, t->cls.ensure(pt.cls.size()));
+  RBG_TRY_HIP(ctx, t->codes.ensure(kElems));
+  RBG_TRY_HIP(ctx, hipMemcpy(t->T.p, pt.T.data(), pt.T.size() * sizeof(double), hipMemcpyHostToDevice));
+  RBG_TRY_HIP(ctx, hipMemcpy(t->cls.p, pt.cls.data(), pt.cls.size(), hipMemcpyHostToDevice));
+  RBG_TRY_HIP(ctx, hipMemcpy(t->codes.p, canonical_codes(), kElems, hipMemcpyHostToDevice));
+  RBG_TRY_HIP(ctx, t->sT.ensure(pt.sT.size() * sizeof(double)));
+  RBG_TRY_HIP(ctx, t->sX.ensure(pt.sX.size()));
+  RBG_TRY_HIP(ctx, hipMemcpy(t->sT.p, pt.sT.data(), pt.sT.size() * sizeof(double), hipMemcpyHostToDevice));
+  RBG_TRY_HIP(ctx, hipMemcpy(t->sX.p, pt.sX.data(), pt.sX.size(), hipMemcpyHostToDevice));
+  *out = t.get();
+  ctx->tables.push_back(std::move(t));
+  return RBG_OK;
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+  return (double)ms;
+}
+
+}  // namespace rbg
+
+size_t rbg_asm::device_bytes() const {
+  const DevBuf* all[] = {&seqs, &offs, &sk_rows, &asm_of, &unit_pad, &raw_pad, &colnorm, &anynz,
+                         &q8, &perm, &pi, &pj, &pcnt, &merge, &desc, &bscal, &svd, &svf, &level,
+                         &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
+                         &mat, &cand, &counters, &deg, &cursor, &off, &list, &topk_id, &topk_cnt,
+                         &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
+                         &ecnt, &escomp, &escore, &scan_tmp, &bounds};
+  size_t s = 0;
+  for (const DevBuf* d : all) s += d->bytes;
+  return s;
+}
+
+void rbg_asm::release_buffers() {
+  drop_graph();
+  DevBuf* all[] = {&seqs, &offs, &sk_rows, &asm_of, &unit_pad, &raw_pad, &colnorm, &anynz,
+                   &q8, &perm, &pi, &pj, &pcnt, &merge, &desc, &bscal, &svd, &svf, &level,
+                   &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
+                   &mat, &cand, &counters, &deg, &cursor, &off, &list, &topk_id, &topk_cnt,
+                   &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
+                   &ecnt, &escomp, &escore, &scan_tmp, &bounds};
+  for (DevBuf* d : all) d->release();
+  planned = false;
+  built = false;
+}
+
+namespace rbg {
+
+namespace {
+
+// Candidate capacity of a fused assembly whose pass covers `pairs` pairs.
+uint64_t cand_capacity(uint64_t pairs, uint32_t n) {
+  return std::min<uint64_t>(pairs, std::max<uint64_t>(1ull << 26, 2048ull * n));
+}
+
+// Pairs of an assembly's fused pass: all pairs, or the pairs with a new row.
+uint64_t pass_pairs(uint32_t n, uint32_t n_old, bool incr) {
+  const uint64_t all = (uint64_t)n * (n - 1) / 2;
+  if (!incr) return all;
+  return all - (uint64_t)n_old * (n_old - 1) / 2;
+}
+
+}  // namespace
+
+rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t count, double min_cv) {
+  hipStream_t st = ctx->stream;
+  b->planned = false;
+  b->built = false;
+  b->drop_graph();
+  b->asms.assign(count, AsmHost{});
+  uint64_t rows = 0, bytes = 0, pairs = 0, ncol = 0, nsk = 0;
+  uint32_t nc_max = 1, kc_max = 1;
+  b->has_src = false;
+  b->n_incr = 0;
+  b->list_extra = 0;
+  for (uint32_t a = 0; a < count; ++a) {
+    const rbg_assembly_in& x = in[a].in;
+    const Resident* src = in[a].src;
+    const uint32_t n_old = src ? src->n : 0;
+    if ((x.n > 0 && (!x.seqs || !x.offsets)) || (x.S > 0 && x.n > 0 && !x.depth))
+      return ctx->fail(RBG_ERR_ARG, "assembly " + std::to_string(a) + ": missing input");
+    if (n_old + (uint64_t)x.n == 0)
+      return ctx->fail(RBG_ERR_ARG, "assembly " + std::to_string(a) + ": no contigs");
+    if (src && src->S != x.S)
+      return ctx->fail(RBG_ERR_ARG, "assembly " + std::to_string(a) + ": added contigs need the session's "
+                                    "sample count");
+    if (x.S > 64)
+      return ctx->fail(RBG_ERR_UNSUPPORTED, "graph builds support at most 64 depth samples");
+    AsmHost& h = b->asms[a];
+    h.n = n_old + x.n;
+    h.n_old = n_old;
+    h.src = src;
+    h.S = x.S;
+    h.row0 = (uint32_t)rows;
+    h.colnorm0 = (uint32_t)ncol;
+    rows += h.n;
+    ncol += x.S;
+    nsk += x.n;
+    if (x.n) bytes += x.offsets[x.n] - x.offsets[0];
+    if (x.S > 1) {
+      nc_max = std::max(nc_max, nc_for(x.S));
+      kc_max = std::max(kc_max, kpad_for(x.S) / 32);
+    }
+    if (src) b->has_src = true;
+  }
+  if (rows >= (1u << 24)) return ctx->fail(RBG_ERR_UNSUPPORTED, "more than 2^24 contigs in a batch");
+  b->nrows = (uint32_t)rows;
+  b->n_sketch = (uint32_t)nsk;
+  b->NC = nc_max;
+  b->kpad = 32 * kc_max;
+  const uint32_t NC = nc_max, kpad = b->kpad;
+  // The sketch kernel reads whole 16-byte chunks: keep 64 readable bytes past
+  // the last contig (their contents are never counted).
+  RBG_TRY_HIP(ctx, b->seqs.ensure(bytes + 64));
+  RBG_TRY_HIP(ctx, b->offs.ensure((nsk + 1) * 8));
+  RBG_TRY_HIP(ctx, b->sk_rows.ensure(std::max<uint64_t>(nsk, 1) * 4));
+  RBG_TRY_HIP(ctx, b->asm_of.ensure(rows * 2));
+  RBG_TRY_HIP(ctx, b->anynz.ensure(rows));
+  RBG_TRY_HIP(ctx, b->unit_pad.ensure(rows * 16 * NC * 4));
+  RBG_TRY_HIP(ctx, b->raw_pad.ensure(rows * 16 * NC * 4));
+  RBG_TRY_HIP(ctx, b->colnorm.ensure(std::max<uint64_t>(ncol, 1) * 8));
+  RBG_TRY_HIP(ctx, b->q8.ensure(rows * kpad));
+  RBG_TRY_HIP(ctx, b->perm.ensure(rows * 4));
+  std::vector<uint64_t> offs(nsk + 1);
+  std::vector<uint32_t> sk_rows(nsk);
+  std::vector<uint16_t> asm_of(rows);
+  std::vector<uint8_t> anynz(rows, 1);  // no depth: is_nz always holds
+  std::vector<double> colnorm(std::max<uint64_t>(ncol, 1), 0.0);
+  std::vector<uint32_t> perm(rows), pi, pj;
+  // Padded rows start as zeros; rows from the host are staged per assembly and
+  // copied as one range, rows from a session are copied on the device.
+  RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->unit_pad.p, rows * 16 * NC, 0u, st));
+  RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->raw_pad.p, rows * 16 * NC, 0u, st));
+  RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->q8.p, rows * kpad / 4, 0u, st));
+  // staging buffers stay alive until the final synchronisation
+  std::deque<std::vector<float>> stage_f;
+  std::deque<std::vector<int8_t>> stage_q;
+  uint64_t seq0 = 0, sk = 0;
+  for (uint32_t a = 0; a < count; ++a) {
+    const rbg_assembly_in& x = in[a].in;
+    AsmHost& h = b->asms[a];
+    const Resident* src = h.src;
+    const uint32_t n_old = h.n_old, n_new = x.n, S = x.S;
+    // contigs to sketch
+    const uint64_t nb = n_new ? x.offsets[n_new] - x.offsets[0] : 0;
+    if (nb)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->seqs.p + seq0, x.seqs + x.offsets[0], nb,
+                                      hipMemcpyHostToDevice, st));
+    for (uint32_t r = 0; r < n_new; ++r) {
+      offs[sk + r] = seq0 + (x.offsets[r] - x.offsets[0]);
+      sk_rows[sk + r] = h.row0 + n_old + r;
+    }
+    sk += n_new;
+    seq0 += nb;
+    offs[sk] = seq0;
+    for (uint32_t r = 0; r < h.n; ++r) asm_of[h.row0 + r] = (uint16_t)a;
+    // depth features of the new rows (row-local), colnorm from running column
+    // sums (the same additions in the same order as over all rows at once)
+    h.anynz_local.assign(h.n, 1);
+    if (src) std::copy(src->anynz.begin(), src->anynz.end(), h.anynz_local.begin());
+    std::vector<float> unit, raw;
+    if (S > 0) {
+      unit.assign((size_t)n_new * S, 0.f);
+      raw.assign((size_t)n_new * S, 0.f);
+      depth_features(x.depth, n_new, S, min_cv, nullptr, unit.data(), raw.data(), nullptr,
+                     h.anynz_local.data() + n_old);
+      h.colsum.assign(S, 0.0);
+      if (src) h.colsum = src->colsum;
+      for (uint32_t c = 0; c < S; ++c) {
+        double sum = h.colsum[c];
+        for (uint32_t r = 0; r < n_new; ++r) sum += x.depth[(size_t)r * S + c];
+        h.colsum[c] = sum;
+        const double mean = sum / (double)std::max<uint32_t>(h.n, 1);
+        colnorm[h.colnorm0 + c] = S > 1 ? ((mean > 1e-12) ? 1.0 / mean : 0.0) : 0.0;
+      }
+    }
+    std::copy(h.anynz_local.begin(), h.anynz_local.end(), anynz.begin() + h.row0);
+    if (S > 1 && n_new) {
+      stage_f.emplace_back((size_t)n_new * 16 * NC, 0.f);
+      std::vector<float>& upad = stage_f.back();
+      stage_f.emplace_back((size_t)n_new * 16 * NC, 0.f);
+      std::vector<float>& rpad = stage_f.back();
+      for (uint32_t r = 0; r < n_new; ++r) {
+        std::memcpy(&upad[(size_t)r * 16 * NC], &unit[(size_t)r * S], S * 4);
+        std::memcpy(&rpad[(size_t)r * 16 * NC], &raw[(size_t)r * S], S * 4);
+      }
+      const size_t o = (size_t)(h.row0 + n_old) * 16 * NC * 4;
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->unit_pad.p + o, upad.data(), upad.size() * 4,
+                                      hipMemcpyHostToDevice, st));
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->raw_pad.p + o, rpad.data(), rpad.size() * 4,
+                                      hipMemcpyHostToDevice, st));
+    }
+    if (src && S > 1 && n_old) {
+      const size_t sp = 16 * nc_for(S) * 4, dp = 16 * NC * 4;
+      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((char*)b->unit_pad.p + (size_t)h.row0 * dp, dp, src->unit_pad.p, sp,
+                                        sp, n_old, hipMemcpyDeviceToDevice, st));
+      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((char*)b->raw_pad.p + (size_t)h.row0 * dp, dp, src->raw_pad.p, sp,
+                                        sp, n_old, hipMemcpyDeviceToDevice, st));
+    }
+    // pair-pass order and int8 rows.  The pass visits the assembly's rows in
+    // abundance order (pair_pass_order): a session keeps its earlier rows'
+    // order and appends the new rows in their own order.
+    h.perm_local.resize(h.n);
+    if (src) std::copy(src->perm.begin(), src->perm.end(), h.perm_local.begin());
+    if (S > 1 && n_new) {
+      std::vector<uint32_t> lp;
+      pair_pass_order(unit.data(), n_new, S, lp);
+      std::vector<float> sorted((size_t)n_new * S);
+      for (uint32_t r = 0; r < n_new; ++r) {
+        std::memcpy(&sorted[(size_t)r * S], &unit[(size_t)lp[r] * S], S * 4);
+        h.perm_local[n_old + r] = n_old + lp[r];
+      }
+      if (src) {
+        h.gq = src->gq;
+        gate_quantize_more(sorted.data(), n_new, S, h.gq);
+      } else {
+        gate_quantize(sorted.data(), n_new, S, h.gq);
+      }
+      stage_q.emplace_back((size_t)n_new * kpad, 0);
+      std::vector<int8_t>& q8 = stage_q.back();
+      for (uint32_t r = 0; r < n_new; ++r)
+        std::memcpy(&q8[(size_t)r * kpad], &h.gq.q[(size_t)r * h.gq.kpad], h.gq.kpad);
+      std::vector<int8_t>().swap(h.gq.q);
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((int8_t*)b->q8.p + (size_t)(h.row0 + n_old) * kpad, q8.data(),
+                                      q8.size(), hipMemcpyHostToDevice, st));
+    } else {
+      for (uint32_t r = 0; r < n_new; ++r) h.perm_local[n_old + r] = n_old + r;
+      if (src) h.gq = src->gq;
+    }
+    if (src && S > 1 && n_old) {
+      const size_t sp = kpad_for(S);
+      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((int8_t*)b->q8.p + (size_t)h.row0 * kpad, kpad, src->q8.p, sp, sp,
+                                        n_old, hipMemcpyDeviceToDevice, st));
+    }
+    for (uint32_t r = 0; r < h.n; ++r) perm[h.row0 + r] = h.row0 + h.perm_local[r];
+    // b0 pair draws (they depend on n only)
+    std::vector<uint32_t> ai, aj;
+    baseline_pairs(h.n, ai, aj);
+    h.pair0 = (uint32_t)pi.size();
+    h.np = (uint32_t)ai.size();
+    for (size_t t = 0; t < ai.size(); ++t) {
+      pi.push_back(h.row0 + ai[t]);
+      pj.push_back(h.row0 + aj[t]);
+    }
+    pairs += ai.size();
+  }
+  b->np_total = (uint32_t)pairs;
+  RBG_TRY_HIP(ctx, b->pi.ensure(std::max<size_t>(pi.size(), 1) * 4));
+  RBG_TRY_HIP(ctx, b->pj.ensure(std::max<size_t>(pj.size(), 1) * 4));
+  RBG_TRY_HIP(ctx, b->pcnt.ensure(std::max<size_t>(pi.size(), 1) * 4));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(b->offs.p, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, st));
+  if (nsk)
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->sk_rows.p, sk_rows.data(), nsk * 4, hipMemcpyHostToDevice, st));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(b->asm_of.p, asm_of.data(), asm_of.size() * 2, hipMemcpyHostToDevice, st));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(b->anynz.p, anynz.data(), anynz.size(), hipMemcpyHostToDevice, st));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(b->colnorm.p, colnorm.data(), colnorm.size() * 8, hipMemcpyHostToDevice, st));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(b->perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, st));
+  if (!pi.empty()) {
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->pi.p, pi.data(), pi.size() * 4, hipMemcpyHostToDevice, st));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->pj.p, pj.data(), pj.size() * 4, hipMemcpyHostToDevice, st));
+  }
+  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));  // the host staging vectors go out of scope
+  return RBG_OK;
+}
+
+rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
+  const uint32_t m = P->m, K = P->max_edges;
+  if (m < 2 || m > 512) return ctx->fail(RBG_ERR_UNSUPPORTED, "sketch-m must be in [2, 512]");
+  if (K == 0 || K > 1024) return ctx->fail(RBG_ERR_UNSUPPORTED, "max-edges must be in [1, 1024]");
+  PlanKey key;
+  key.m = m;
+  key.K = K;
+  key.rand_seed = P->rand_seed;
+  key.tiles = b->sched.gate_tiles_per_item;
+  key.rows = b->sched.dense_rows_per_item;
+  key.regime = P->regime;
+  key.pmh_seed = P->pmh_seed;
+  key.cand_cap = b->cand_cap_min;
+  key.min_edge = P->min_edge_weight;
+  key.coverage = P->coverage;
+  key.neg_thr = P->neg_depth_thr;
+  if (b->planned && b->key == key) return RBG_OK;
+  b->drop_graph();
+  b->planned = false;
+  rbg_ctx::Tables* t;
+  RBG_TRY(get_tables(ctx, m, P->pmh_seed, &t));  // uploaded before any capture
+  const uint32_t A = (uint32_t)b->asms.size(), N = b->nrows, W = words_for(m);
+  std::vector<AsmDesc>& D = b->desc_h;
+  D.assign(A, AsmDesc{});
+  uint64_t mat_total = 0, cap = 0;
+  std::vector<uint32_t> gfull, gpart;
+  std::vector<int32_t> rnd((size_t)A * kCalibRandDev);
+  std::vector<uint32_t> members, roff((size_t)A * 11, 0);
+  std::vector<MergeDesc> md;
+  uint32_t max_old = 0;
+  b->list_extra = 0;
+  for (uint32_t a = 0; a < A; ++a) {
+    AsmHost& h = b->asms[a];
+    AsmDesc& d = D[a];
+    const int regime = regime_for(P->regime, h.n);
+    if (regime != 1 && regime != 2) return ctx->fail(RBG_ERR_ARG, "regime must be 0, 1 or 2");
+    if (regime == 2 && h.n < 2500)
+      return ctx->fail(RBG_ERR_ARG, "fused regime needs n >= 2500 (assembly " + std::to_string(a) + ")");
+    // A session whose earlier build was fused keeps its gated pairs: only the
+    // pairs with a new row are evaluated.
+    h.incr = regime == 2 && h.src && h.src->regime == 2 && h.n_old > 0;
+    d.row0 = h.row0;
+    d.n = h.n;
+    d.S = h.S;
+    d.regime = (uint32_t)regime;
+    // composition weight: auto_w_comp(S) (src/rabbitbin.cpp:470-473)
+    d.w_comp = h.S <= 2 ? 1.0 : 0.0;
+    // abundance-first gate (src/impl/rb_graph.cpp:693-706)
+    const bool gate = regime == 2 && h.S > 1 && d.w_comp < 1.0 && d.w_comp < P->min_edge_weight;
+    d.thr = gate ? (float)((P->min_edge_weight - d.w_comp) / (1.0 - d.w_comp) - 1e-4)
+                 : -__builtin_huge_valf();
+    // prefilter cut: with the gate off every pair is a candidate
+    d.qcut = gate ? gate_qcut(h.gq, d.thr) : INT32_MIN;
+    d.acc1_end = gate ? 2 * (h.S / 32) : 0;
+    d.pair0 = h.pair0;
+    d.np = h.np;
+    d.colnorm0 = h.colnorm0;
+    if (regime == 1) {
+      d.mat0 = mat_total;
+      mat_total += (uint64_t)h.n * h.n;
+    } else {
+      const uint32_t jmin = h.incr ? h.n_old : 0;
+      gate_work_items(a, h.n, jmin, b->sched.gate_tiles_per_item, gfull, gpart);
+      cap += cand_capacity(pass_pairs(h.n, h.n_old, h.incr), h.n);
+      if (h.incr) {
+        md.push_back(MergeDesc{(const uint64_t*)h.src->loff.p, (const uint64_t*)h.src->list.p, h.row0,
+                               h.n_old});
+        max_old = std::max(max_old, h.n_old);
+        b->list_extra += h.src->nlist;
+      }
+    }
+    CalibPlan plan;
+    calibration_plan(h.n, P->rand_seed, regime, plan);
+    std::memcpy(&rnd[(size_t)a * kCalibRandDev], plan.rnd.data(), kCalibRandDev * 4);
+    const uint32_t base = (uint32_t)members.size();
+    for (uint32_t c : plan.members) members.push_back(h.row0 + c);
+    for (uint32_t r = 0; r < 11; ++r)
+      roff[(size_t)a * 11 + r] =
+          base + (r < plan.round_off.size() ? plan.round_off[r] : plan.round_off.back());
+  }
+  cap = std::max(cap, b->cand_cap_min);
+  std::vector<uint32_t> ditems;
+  dense_work_items(D, b->sched.dense_rows_per_item, ditems);
+  gfull.insert(gfull.end(), gpart.begin(), gpart.end());
+  b->n_dense_items = (uint32_t)(ditems.size() / 4);
+  b->n_gate_items = (uint32_t)(gfull.size() / 4);
+  b->has_gate_items = b->n_gate_items > 0;
+  b->n_incr = (uint32_t)md.size();
+  b->mat_total = mat_total;
+  b->cand_cap = cap;
+  b->edge_cap = (uint64_t)N * K;
+  hipStream_t st = ctx->stream;
+  RBG_TRY_HIP(ctx, b->desc.ensure(A * sizeof(AsmDesc)));
+  RBG_TRY_HIP(ctx, b->dense_items.ensure(std::max<size_t>(ditems.size(), 4) * 4));
+  RBG_TRY_HIP(ctx, b->gate_items.ensure(std::max<size_t>(gfull.size(), 4) * 4));
+  RBG_TRY_HIP(ctx, b->cal_rnd.ensure(rnd.size() * 4));
+  RBG_TRY_HIP(ctx, b->cal_members.ensure(std::max<size_t>(members.size(), 1) * 4));
+  RBG_TRY_HIP(ctx, b->cal_roff.ensure(roff.size() * 4));
+  RBG_TRY_HIP(ctx, b->merge.ensure(std::max<size_t>(md.size(), 1) * sizeof(MergeDesc)));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(b->desc.p, D.data(), A * sizeof(AsmDesc), hipMemcpyHostToDevice, st));
+  if (!ditems.empty())
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->dense_items.p, ditems.data(), ditems.size() * 4,
+                                    hipMemcpyHostToDevice, st));
+  if (!gfull.empty())
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->gate_items.p, gfull.data(), gfull.size() * 4,
+                                    hipMemcpyHostToDevice, st));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(b->cal_rnd.p, rnd.data(), rnd.size() * 4, hipMemcpyHostToDevice, st));
+  if (!members.empty())
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->cal_members.p, members.data(), members.size() * 4,
+                                    hipMemcpyHostToDevice, st));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(b->cal_roff.p, roff.data(), roff.size() * 4, hipMemcpyHostToDevice, st));
+  if (!md.empty())
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->merge.p, md.data(), md.size() * sizeof(MergeDesc),
+                                    hipMemcpyHostToDevice, st));
+  b->merge_max_old = max_old;
+  const uint32_t FW = fence_words(K), KW = (K + 63) / 64;
+  RBG_TRY_HIP(ctx, b->planes.ensure((size_t)N * 8 * W * 4));
+  RBG_TRY_HIP(ctx, b->bscal.ensure(A * sizeof(BuildScalars)));
+  RBG_TRY_HIP(ctx, b->svd.ensure((size_t)A * (m + 1) * 8));
+  RBG_TRY_HIP(ctx, b->svf.ensure((size_t)A * (m + 1) * 4));
+  RBG_TRY_HIP(ctx, b->level.ensure((size_t)A * (m + 1) * 2));
+  RBG_TRY_HIP(ctx, b->mat.ensure(std::max<uint64_t>(mat_total, 1) * 2));
+  RBG_TRY_HIP(ctx, b->cand.ensure(std::max<uint64_t>(cap, 1) * 8));
+  RBG_TRY_HIP(ctx, b->counters.ensure(16));
+  RBG_TRY_HIP(ctx, b->deg.ensure((size_t)N * 4));
+  RBG_TRY_HIP(ctx, b->cursor.ensure((size_t)N * 4));
+  RBG_TRY_HIP(ctx, b->off.ensure((size_t)(N + 1) * 8));
+  RBG_TRY_HIP(ctx, b->list.ensure(std::max<uint64_t>(2 * cap + b->list_extra, 1) * 8));
+  // +16 ids: the mutual test reads whole 16-id segments of a row's list
+  RBG_TRY_HIP(ctx, b->topk_id.ensure(((size_t)N * K + 16) * 4));
+  RBG_TRY_HIP(ctx, b->topk_cnt.ensure((size_t)N * K * 2));
+  RBG_TRY_HIP(ctx, b->topk_len.ensure((size_t)N * 4));
+  RBG_TRY_HIP(ctx, b->rowmax.ensure((size_t)N * 4));
+  RBG_TRY_HIP(ctx, b->fence.ensure((size_t)N * FW * 4));
+  RBG_TRY_HIP(ctx, b->keep.ensure((size_t)N * KW * 8));
+  RBG_TRY_HIP(ctx, b->row_edges.ensure((size_t)N * 4));
+  RBG_TRY_HIP(ctx, b->row_off.ensure((size_t)(N + 1) * 8));
+  const uint64_t ce = std::max<uint64_t>(b->edge_cap, 1);
+  RBG_TRY_HIP(ctx, b->efrom.ensure(ce * 4));
+  RBG_TRY_HIP(ctx, b->eto.ensure(ce * 4));
+  RBG_TRY_HIP(ctx, b->ecnt.ensure(ce * 2));
+  RBG_TRY_HIP(ctx, b->escomp.ensure(ce * 4));
+  RBG_TRY_HIP(ctx, b->escore.ensure(ce * 4));
+  RBG_TRY_HIP(ctx, b->bounds.ensure((size_t)A * 4 * 8));
+  size_t tmp = 0;
+  RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64(nullptr, nullptr, N, nullptr, tmp, st));
+  RBG_TRY_HIP(ctx, b->scan_tmp.ensure(tmp));
+  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+  b->key = key;
+  b->planned = true;
+  return RBG_OK;
+}
+
+size_t estimate_build_bytes(const AsmInput* in, uint32_t count, const rbg_graph_params* P) {
+  const uint32_t m = P->m, K = P->max_edges, W = words_for(m);
+  uint64_t N = 0, bytes = 0, mat = 0, cap = 0, extra = 0, nsk = 0;
+  uint32_t NC = 1, kpad = 32;
+  for (uint32_t a = 0; a < count; ++a) {
+    const rbg_assembly_in& x = in[a].in;
+    const Resident* src = in[a].src;
+    const uint32_t n_old = src ? src->n : 0, n = n_old + x.n;
+    N += n;
+    nsk += x.n;
+    if (x.n) bytes += x.offsets[x.n] - x.offsets[0];
+    NC = std::max(NC, nc_for(x.S));
+    kpad = std::max(kpad, kpad_for(x.S));
+    const int regime = regime_for(P->regime, n);
+    if (regime == 1) {
+      mat += (uint64_t)n * n;
+    } else {
+      const bool incr = src && src->regime == 2 && n_old > 0;
+      cap += cand_capacity(pass_pairs(n, n_old, incr), n);
+      if (incr) extra += src->nlist;
+    }
+  }
+  const uint64_t FW = fence_words(K), KW = (K + 63) / 64;
+  uint64_t s = bytes + 64 + (nsk + 1) * 12;                 // seqs, offs, sk_rows
+  s += N * (2 + 1 + 2 * 16 * NC * 4 + kpad + 4);            // asm_of, anynz, unit/raw, q8, perm
+  s += N * 8 * W * 4;                                       // planes
+  s += mat * 2 + cap * 8 + (2 * cap + extra) * 8;           // mat, cand, list
+  s += N * (4 + 4 + 8 + (uint64_t)K * 6 + 4 + 4 + FW * 4 + KW * 8 + 4 + 8);  // per-row arrays
+  s += N * (uint64_t)K * (4 + 4 + 2 + 4 + 4);               // edges
+  s += (uint64_t)count * (64 * 4 + (m + 1) * 14 + kCalibRandDev * 4 + 30000 * 12);
+  return (size_t)s + (64u << 20);                           // small tables and slack
+}
+
+// Every launch of one build, on ctx->stream, without a host round trip (so it
+// can be captured).  `timed` records the stage events of the eager path.
+static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P, bool want_wx,
+                                bool timed) {
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  const uint32_t m = P->m, K = P->max_edges, W = words_for(m), N = b->nrows;
+  const uint32_t A = (uint32_t)b->asms.size(), FW = fence_words(K);
+  rbg_ctx::Tables* t;
+  RBG_TRY(get_tables(ctx, m, P->pmh_seed, &t));
+  const AsmDesc* desc = (const AsmDesc*)b->desc.p;
+  const uint16_t* asm_of = (const uint16_t*)b->asm_of.p;
+  BuildScalars* bs = (BuildScalars*)b->bscal.p;
+  auto mark = [&](int k) -> rbg_status {
+    if (timed) RBG_TRY_HIP(ctx, hipEventRecord(ev[k], st));
+    return RBG_OK;
+  };
+  RBG_TRY(mark(0));
+  const uint32_t mpad = 32 * W;
+  if (want_wx) RBG_TRY_HIP(ctx, b->wx.ensure((size_t)N * mpad));
+  // sketches: rows of resident sessions are copied, the others computed
+  if (b->has_src)
+    for (const AsmHost& h : b->asms)
+      if (h.src && h.n_old)
+        RBG_TRY_HIP(ctx, hipMemcpyAsync((uint32_t*)b->planes.p + (size_t)h.row0 * 8 * W, h.src->planes.p,
+                                        (size_t)h.n_old * 8 * W * 4, hipMemcpyDeviceToDevice, st));
+  RBG_TRY_HIP(ctx, launch_sketch((const uint8_t*)b->seqs.p, (const uint64_t*)b->offs.p, b->n_sketch, m, W,
+                                 (const double*)t->sT.p, (const uint8_t*)t->sX.p,
+                                 (const uint8_t*)t->cls.p, (const uint8_t*)t->codes.p,
+                                 want_wx ? (uint8_t*)b->wx.p : nullptr, mpad, (uint32_t*)b->planes.p,
+                                 b->has_src ? (const uint32_t*)b->sk_rows.p : nullptr, st));
+  RBG_TRY(mark(1));
+  // b0 and the sv tables (the pair draws depend on n only)
+  if (b->np_total)
+    RBG_TRY_HIP(ctx, launch_list_counts((const uint32_t*)b->planes.p, W, (const uint32_t*)b->pi.p,
+                                        (const uint32_t*)b->pj.p, b->np_total,
+                                        (uint32_t*)b->pcnt.p, st));
+  RBG_TRY_HIP(ctx, launch_baseline_tables((const uint32_t*)b->pcnt.p, desc, A, m, bs,
+                                          (double*)b->svd.p, (float*)b->svf.p,
+                                          (uint16_t*)b->level.p, st));
+  RBG_TRY(mark(2));
+  if (b->n_dense_items)
+    RBG_TRY_HIP(ctx, launch_dense_counts((const uint32_t*)b->planes.p, W, desc,
+                                         (const uint32_t*)b->dense_items.p, b->n_dense_items,
+                                         (uint16_t*)b->mat.p, st));
+  unsigned long long* ncand = (unsigned long long*)b->counters.p;
+  if (b->has_gate_items) {
+    RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->counters.p, 4, 0u, st));
+    RBG_TRY_HIP(ctx, launch_gate_mfma((const int8_t*)b->q8.p, b->kpad, desc,
+                                      (const uint32_t*)b->perm.p, (const uint32_t*)b->gate_items.p,
+                                      b->n_gate_items, (uint64_t*)b->cand.p, ncand, b->cand_cap, st));
+  }
+  RBG_TRY(mark(6));
+  RBG_TRY(mark(7));
+  if (b->has_gate_items) {
+    // exact FP32 gate and register compare of the candidates, in place
+    RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->deg.p, N, 0u, st));
+    RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->cursor.p, N, 0u, st));
+    RBG_TRY_HIP(ctx, launch_gate_compare((uint64_t*)b->cand.p, b->cand_cap, ncand,
+                                         (const float*)b->unit_pad.p, b->NC,
+                                         (const uint8_t*)b->anynz.p, desc, asm_of,
+                                         (const uint32_t*)b->planes.p, W, (uint32_t*)b->deg.p, st));
+  }
+  RBG_TRY(mark(8));
+  if (b->has_gate_items) {
+    const MergeDesc* md = (const MergeDesc*)b->merge.p;
+    RBG_TRY_HIP(ctx, launch_merge_deg(md, b->n_incr, b->merge_max_old, (uint32_t*)b->deg.p, st));
+    size_t tmp = b->scan_tmp.bytes;
+    RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)b->deg.p, (uint64_t*)b->off.p, N,
+                                               b->scan_tmp.p, tmp, st));
+    RBG_TRY_HIP(ctx, launch_merge_lists(md, b->n_incr, b->merge_max_old, (const uint64_t*)b->off.p,
+                                        (uint32_t*)b->cursor.p, (uint64_t*)b->list.p, st));
+    RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)b->cand.p, b->cand_cap, ncand,
+                                    (const uint64_t*)b->off.p, (uint32_t*)b->cursor.p,
+                                    (uint64_t*)b->list.p, st));
+  }
+  RBG_TRY(mark(3));
+  RBG_TRY(mark(9));
+  RBG_TRY_HIP(ctx, launch_select((const uint64_t*)b->list.p, (const uint64_t*)b->off.p,
+                                 (const uint16_t*)b->mat.p, (const uint8_t*)b->anynz.p, desc, asm_of,
+                                 N, (const uint16_t*)b->level.p, m, K, (uint32_t*)b->topk_id.p,
+                                 (uint16_t*)b->topk_cnt.p, (uint32_t*)b->topk_len.p,
+                                 (uint32_t*)b->rowmax.p, (uint32_t*)b->fence.p, FW, st));
+  RBG_TRY(mark(10));
+  // calibration (the glibc rand() draws are prepared per (n, seed))
+  RBG_TRY_HIP(ctx, launch_calibrate((const uint32_t*)b->rowmax.p, desc, A, m,
+                                    (const double*)b->svd.p, (const float*)b->svf.p,
+                                    (const int32_t*)b->cal_rnd.p, (const uint32_t*)b->cal_members.p,
+                                    (const uint32_t*)b->cal_roff.p, P->coverage, bs, st));
+  RBG_TRY_HIP(ctx, launch_emit_mark((const uint32_t*)b->topk_id.p, (const uint16_t*)b->topk_cnt.p,
+                                    (const uint32_t*)b->topk_len.p, (const uint32_t*)b->fence.p, FW,
+                                    N, K, bs, desc, asm_of, (uint64_t*)b->keep.p,
+                                    (uint32_t*)b->row_edges.p, st));
+  {
+    size_t tmp = b->scan_tmp.bytes;
+    RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)b->row_edges.p,
+                                               (uint64_t*)b->row_off.p, N, b->scan_tmp.p, tmp, st));
+  }
+  const uint64_t* ne = (const uint64_t*)b->row_off.p + N;
+  RBG_TRY_HIP(ctx, launch_emit_write((const uint32_t*)b->topk_id.p, (const uint16_t*)b->topk_cnt.p,
+                                     (const uint32_t*)b->topk_len.p, N, K, (const uint64_t*)b->keep.p,
+                                     (const uint64_t*)b->row_off.p, (uint32_t*)b->efrom.p,
+                                     (uint32_t*)b->eto.p, (uint16_t*)b->ecnt.p, st));
+  RBG_TRY_HIP(ctx, launch_cnt_to_sv((const uint16_t*)b->ecnt.p, b->edge_cap, ne,
+                                    (const uint32_t*)b->efrom.p, asm_of, m, (const float*)b->svf.p,
+                                    (float*)b->escomp.p, st));
+  RBG_TRY(mark(4));
+  RBG_TRY_HIP(ctx, launch_edge_weights_batch((const uint32_t*)b->efrom.p, (const uint32_t*)b->eto.p,
+                                             (const float*)b->escomp.p, b->edge_cap, ne,
+                                             (const float*)b->unit_pad.p, (const float*)b->raw_pad.p,
+                                             b->NC, (const double*)b->colnorm.p, desc, asm_of,
+                                             P->min_edge_weight, P->neg_depth_thr,
+                                             (float*)b->escore.p, st));
+  RBG_TRY_HIP(ctx, launch_gather_bounds((const uint64_t*)b->row_off.p,
+                                        b->has_gate_items ? (const uint64_t*)b->off.p : nullptr, desc,
+                                        A, (uint64_t*)b->bounds.p, st));
+  RBG_TRY(mark(5));
+  return RBG_OK;
+}
+
+rbg_status run_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P, bool want_wx,
+                     bool capture, bool* replayed) {
+  hipStream_t st = ctx->stream;
+  const uint32_t A = (uint32_t)b->asms.size();
+  if (replayed) *replayed = false;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    RBG_TRY(batch_plan(ctx, b, P));
+    if (capture && !want_wx && !b->has_src) {
+      const bool had = b->exec != nullptr;
+      if (!had) {
+        RBG_TRY_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const rbg_status s = enqueue_build(ctx, b, P, false, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (s != RBG_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return s;
+        }
+        RBG_TRY_HIP(ctx, e);
+        b->graph = g;
+        RBG_TRY_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      }
+      if (replayed) *replayed = had;
+      RBG_TRY_HIP(ctx, hipEventRecord(ctx->ev[0], st));
+      RBG_TRY_HIP(ctx, hipGraphLaunch(b->exec, st));
+      RBG_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+    } else {
+      RBG_TRY(enqueue_build(ctx, b, P, want_wx, true));
+    }
+    b->bs_h.resize(A);
+    b->bounds_h.resize((size_t)A * 4);
+    unsigned long long nc = 0;
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->bs_h.data(), b->bscal.p, A * sizeof(BuildScalars),
+                                    hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(b->bounds_h.data(), b->bounds.p, (size_t)A * 32,
+                                    hipMemcpyDeviceToHost, st));
+    if (b->has_gate_items)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(&nc, b->counters.p, 8, hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+    b->ncand_last = nc;
+    if (nc <= b->cand_cap) {
+      b->built = true;
+      return RBG_OK;
+    }
+    b->cand_cap_min = nc;  // rebuild with room for every candidate
+    b->built = false;
+  }
+  return ctx->fail(RBG_ERR_NOMEM, "candidate list overflow after a resize");
+}
+
+rbg_status assembly_result(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_params* P,
+                           bool on_device, rbg_graph_result* out, rbg_graph_diag* diag) {
+  std::memset(out, 0, sizeof(*out));
+  if (!b->built) return ctx->fail(RBG_ERR_ARG, "no graph has been built");
+  if (a >= b->asms.size()) return ctx->fail(RBG_ERR_ARG, "assembly index out of range");
+  hipStream_t st = ctx->stream;
+  const AsmHost& h = b->asms[a];
+  const BuildScalars& s = b->bs_h[a];
+  const uint64_t e0 = b->bounds_h[4 * a], e1 = b->bounds_h[4 * a + 1];
+  const uint64_t ne = e1 - e0;
+  out->n_edges = ne;
+  out->b0 = s.b0;
+  out->sim_cutoff = s.sim_cutoff;
+  out->regime = (int)b->desc_h[a].regime;
+  out->gated_pairs = (b->bounds_h[4 * a + 3] - b->bounds_h[4 * a + 2]) / 2;
+  const uint32_t K = P->max_edges, m = P->m, n = h.n, r0 = h.row0;
+  if (!on_device) {
+    const size_t nb = std::max<uint64_t>(ne, 1);
+    out->from = (uint32_t*)std::malloc(nb * 4);
+    out->to = (uint32_t*)std::malloc(nb * 4);
+    out->scomp = (float*)std::malloc(nb * 4);
+    out->score = h.S > 0 ? (float*)std::malloc(nb * 4) : nullptr;
+    if (!out->from || !out->to || !out->scomp || (h.S > 0 && !out->score))
+      return ctx->fail(RBG_ERR_NOMEM, "host allocation of the edge list");
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->from, (uint32_t*)b->efrom.p + e0, ne * 4, hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->to, (uint32_t*)b->eto.p + e0, ne * 4, hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->scomp, (float*)b->escomp.p + e0, ne * 4, hipMemcpyDeviceToHost, st));
+    if (h.S > 0)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(out->score, (float*)b->escore.p + e0, ne * 4, hipMemcpyDeviceToHost, st));
+  }
+  if (diag) {
+    if (diag->topk_id)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(diag->topk_id, (uint32_t*)b->topk_id.p + (size_t)r0 * K,
+                                      (size_t)n * K * 4, hipMemcpyDeviceToHost, st));
+    if (diag->topk_cnt)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(diag->topk_cnt, (uint16_t*)b->topk_cnt.p + (size_t)r0 * K,
+                                      (size_t)n * K * 2, hipMemcpyDeviceToHost, st));
+    if (diag->topk_len)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(diag->topk_len, (uint32_t*)b->topk_len.p + r0, (size_t)n * 4,
+                                      hipMemcpyDeviceToHost, st));
+    if (diag->rowmax_cnt)
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(diag->rowmax_cnt, (uint32_t*)b->rowmax.p + r0, (size_t)n * 4,
+                                      hipMemcpyDeviceToHost, st));
+  }
+  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if (r0) {  // global rows -> ids local to the assembly
+    if (!on_device)
+      for (uint64_t e = 0; e < ne; ++e) {
+        out->from[e] -= r0;
+        out->to[e] -= r0;
+      }
+    if (diag && diag->topk_id && diag->topk_len)
+      for (uint32_t r = 0; r < n; ++r)
+        for (uint32_t t2 = 0; t2 < diag->topk_len[r]; ++t2) diag->topk_id[(size_t)r * K + t2] -= r0;
+  }
+  if (diag && diag->win16) {
+    if (!b->wx.p) return ctx->fail(RBG_ERR_ARG, "winners were not kept by this build");
+    const ProposalTables& pt = proposal_tables(m, P->pmh_seed);
+    const uint32_t mpad = 32 * words_for(m);
+    std::vector<uint8_t> wx((size_t)n * mpad);
+    RBG_TRY_HIP(ctx, hipMemcpy(wx.data(), (uint8_t*)b->wx.p + (size_t)r0 * mpad, wx.size(),
+                               hipMemcpyDeviceToHost));
+    for (uint32_t r = 0; r < n; ++r)
+      for (uint32_t k = 0; k < m; ++k) {
+        const uint8_t x = wx[(size_t)r * mpad + k];
+        diag->win16[(size_t)r * m + k] = x == 0xFF ? 0 : pt.W16[(size_t)x * m + k];
+      }
+  }
+  return RBG_OK;
+}
+
+rbg_status save_resident(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_params* P,
+                         Resident* r) {
+  if (!b->built || a >= b->asms.size()) return ctx->fail(RBG_ERR_ARG, "no build to keep");
+  hipStream_t st = ctx->stream;
+  AsmHost& h = b->asms[a];
+  const uint32_t n = h.n, S = h.S, W = words_for(P->m);
+  // Device rows are copied before any of r's buffers is replaced: r may be
+  // this build's source, and a buffer that grows is reallocated.
+  DevBuf planes, unit, raw, q8, loff, list;
+  RBG_TRY_HIP(ctx, planes.ensure((size_t)n * 8 * W * 4));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(planes.p, (uint32_t*)b->planes.p + (size_t)h.row0 * 8 * W,
+                                  (size_t)n * 8 * W * 4, hipMemcpyDeviceToDevice, st));
+  if (S > 1) {
+    const size_t sp = 16 * nc_for(S) * 4, bp = 16 * b->NC * 4;
+    RBG_TRY_HIP(ctx, unit.ensure((size_t)n * sp));
+    RBG_TRY_HIP(ctx, raw.ensure((size_t)n * sp));
+    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(unit.p, sp, (char*)b->unit_pad.p + (size_t)h.row0 * bp, bp, sp, n,
+                                      hipMemcpyDeviceToDevice, st));
+    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(raw.p, sp, (char*)b->raw_pad.p + (size_t)h.row0 * bp, bp, sp, n,
+                                      hipMemcpyDeviceToDevice, st));
+    const size_t kp = kpad_for(S);
+    RBG_TRY_HIP(ctx, q8.ensure((size_t)n * kp));
+    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(q8.p, kp, (int8_t*)b->q8.p + (size_t)h.row0 * b->kpad, b->kpad, kp, n,
+                                      hipMemcpyDeviceToDevice, st));
+  }
+  const int regime = (int)b->desc_h[a].regime;
+  uint64_t nlist = 0;
+  if (regime == 2) {
+    nlist = b->bounds_h[4 * a + 3] - b->bounds_h[4 * a + 2];
+    RBG_TRY_HIP(ctx, loff.ensure(((size_t)n + 1) * 8));
+    RBG_TRY_HIP(ctx, list.ensure(std::max<uint64_t>(nlist, 1) * 8));
+    RBG_TRY_HIP(ctx, launch_extract_lists((const uint64_t*)b->off.p, (const uint64_t*)b->list.p, h.row0, n,
+                                          (uint64_t*)loff.p, (uint64_t*)list.p, st));
+  }
+  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+  auto take = [](DevBuf& dst, DevBuf& from) {
+    dst.release();
+    dst.p = from.p;
+    dst.bytes = from.bytes;
+    from.p = nullptr;
+    from.bytes = 0;
+  };
+  take(r->planes, planes);
+  take(r->unit_pad, unit);
+  take(r->raw_pad, raw);
+  take(r->q8, q8);
+  take(r->loff, loff);
+  take(r->list, list);
+  r->nlist = nlist;
+  r->n = n;
+  r->S = S;
+  r->regime = regime;
+  r->perm = std::move(h.perm_local);
+  r->anynz = std::move(h.anynz_local);
+  r->colsum = std::move(h.colsum);
+  r->gq = h.gq;
+  return RBG_OK;
+}
+
+}  // namespace rbg

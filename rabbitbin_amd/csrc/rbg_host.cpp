@@ -309,14 +309,12 @@ void depth_features(const float* depth, size_t n, size_t S, double min_cv, float
     }
 }
 
-void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g) {
-  g.S = S;
-  g.kpad = 32 * ((S + 31) / 32);
+namespace {
+
+// q rows of `unit` at g.scale (clamped to [-127, 127]; the clamp error counts
+// in the error norm), folding the rows' norms into g's maxima.
+void quantize_rows(const float* unit, size_t n, uint32_t S, GateQuant& g) {
   g.q.assign(n * g.kpad, 0);
-  double amax = 0;
-  for (size_t t = 0; t < n * S; ++t) amax = std::max(amax, (double)std::fabs(unit[t]));
-  g.scale = amax > 0 ? 127.0 / amax : 0.0;
-  g.max_qnorm = g.max_err = g.max_unorm = 0;
   for (size_t r = 0; r < n; ++r) {
     double qq = 0, ee = 0, uu = 0;
     for (uint32_t k = 0; k < S; ++k) {
@@ -333,6 +331,33 @@ void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g) {
     g.max_err = std::max(g.max_err, std::sqrt(ee));
     g.max_unorm = std::max(g.max_unorm, std::sqrt(uu));
   }
+}
+
+double abs_max(const float* unit, size_t count) {
+  double amax = 0;
+  for (size_t t = 0; t < count; ++t) amax = std::max(amax, (double)std::fabs(unit[t]));
+  return amax;
+}
+
+}  // namespace
+
+void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g) {
+  g.S = S;
+  g.kpad = 32 * ((S + 31) / 32);
+  const double amax = abs_max(unit, n * S);
+  g.scale = amax > 0 ? 127.0 / amax : 0.0;
+  g.max_qnorm = g.max_err = g.max_unorm = 0;
+  quantize_rows(unit, n, S, g);
+}
+
+void gate_quantize_more(const float* unit, size_t n, uint32_t S, GateQuant& g) {
+  g.S = S;
+  g.kpad = 32 * ((S + 31) / 32);
+  if (g.scale <= 0) {  // every earlier row is zero: any scale quantises them exactly
+    const double amax = abs_max(unit, n * S);
+    g.scale = amax > 0 ? 127.0 / amax : 0.0;
+  }
+  quantize_rows(unit, n, S, g);
 }
 
 void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32_t>& perm) {

@@ -63,6 +63,11 @@ struct GateQuant {
   uint32_t S = 0;
 };
 void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g);
+// Rows appended to an existing quantisation: the same scale (values beyond
+// +-127 clamp, and the clamp error counts in max_err), g.q = the new rows only,
+// the norm maxima taken over the earlier and the new rows.  The cut stays a
+// rigorous bound for every pair of rows quantised under g.
+void gate_quantize_more(const float* unit, size_t n, uint32_t S, GateQuant& g);
 // Row order of the fused pair pass: rows sorted by the indices of their three
 // largest unit-vector entries (ties to the smaller sample, then by row), so
 // contigs with the same abundance pattern are adjacent and the pairs that

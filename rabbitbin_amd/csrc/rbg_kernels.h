@@ -47,10 +47,12 @@ struct AsmDesc {
 constexpr uint32_t kCalibRandDev = 2048;
 
 // ---- stage (a)
+// Sketches of contigs [0, n) of (seqs, offs); contig c goes to row out_row[c]
+// of planes / wx (row c when out_row is null).
 hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
                          uint32_t W, const double* sT, const uint8_t* sX, const uint8_t* cls,
                          const uint8_t* codes, uint8_t* wx, uint32_t mpad, uint32_t* planes,
-                         hipStream_t st);
+                         const uint32_t* out_row, hipStream_t st);
 hipError_t launch_list_counts(const uint32_t* planes, uint32_t W, const uint32_t* pi,
                               const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st);
 hipError_t launch_win16_counts(const uint16_t* win16, uint32_t m, const uint32_t* pi,
@@ -76,10 +78,12 @@ hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, const AsmDesc
                                hipStream_t st);
 
 // ---- stage (b), regime 2: abundance gate + register compare
-// Work items (assembly, bi << 16 | bj0, tile count, 0): a 256-row stripe bi of
-// the assembly against up to tiles_per_item 256-column tiles from bj0.
-void gate_work_items(uint32_t a, uint32_t n, uint32_t tiles_per_item, std::vector<uint32_t>& full,
-                     std::vector<uint32_t>& part);
+// Work items (assembly, bi << 16 | bj0, tile count, jmin): a 256-row stripe bi
+// of the assembly against up to tiles_per_item 256-column tiles from bj0, over
+// the pairs i < j with jmin <= j (pair-pass positions; jmin > 0 when the
+// pairs among the first jmin rows were evaluated by an earlier build).
+void gate_work_items(uint32_t a, uint32_t n, uint32_t jmin, uint32_t tiles_per_item,
+                     std::vector<uint32_t>& full, std::vector<uint32_t>& part);
 // q holds each assembly's int8 rows in its pair-pass order (kpad bytes per
 // row); perm maps that order back to global rows.  Candidates are written as
 // (min row, max row).
@@ -94,6 +98,24 @@ hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long
                                const float* unit, uint32_t NC, const uint8_t* anynz,
                                const AsmDesc* desc, const uint16_t* asm_of,
                                const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st);
+// Incremental assemblies: a served session's candidate lists from its earlier
+// build (CSR over its first n_old rows, local ids), merged into this build's
+// lists: merge_deg adds their lengths to deg (before the offsets are
+// scanned), merge_lists copies them to the head of each row's new list and
+// sets the row's cursor after them (before the scatter).
+struct MergeDesc {
+  const uint64_t* loff;  // n_old + 1
+  const uint64_t* list;  // entries (count << 32 | local id)
+  uint32_t row0, n_old;
+};
+hipError_t launch_merge_deg(const MergeDesc* md, uint32_t nmd, uint32_t max_old, uint32_t* deg,
+                            hipStream_t st);
+hipError_t launch_merge_lists(const MergeDesc* md, uint32_t nmd, uint32_t max_old, const uint64_t* off,
+                              uint32_t* cursor, uint64_t* list, hipStream_t st);
+// Rows [row0, row0 + n) of a build's lists as a local CSR (loff: n + 1 offsets
+// from off[row0]; dst: entries with ids made local).
+hipError_t launch_extract_lists(const uint64_t* off, const uint64_t* list, uint32_t row0, uint32_t n,
+                                uint64_t* loff, uint64_t* dst, hipStream_t st);
 hipError_t launch_scatter(const uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
                           const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st);
 

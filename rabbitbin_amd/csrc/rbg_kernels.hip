@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void sketch_kernel(
     const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
     uint32_t W, const double* __restrict__ sT, const uint8_t* __restrict__ sX,
     const uint8_t* __restrict__ cls, const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx,
-    uint32_t mpad, uint32_t* __restrict__ planes) {
+    uint32_t mpad, uint32_t* __restrict__ planes, const uint32_t* __restrict__ out_row) {
   static_assert(CPB == kSketchWaves, "one contig per wave");
   __shared__ uint32_t hist[CPB][256];
   __shared__ uint32_t bases[CPB][4];
@@ -319,15 +319,16 @@ __global__ __launch_bounds__(256) void sketch_kernel(
 #pragma unroll
     for (int q = 0; q < CPB; ++q) {
       const uint32_t c = c0 + q;
+      const size_t orow = c < n ? (out_row ? out_row[c] : c) : 0;  // output row of contig c
       uint32_t code = 0xFFu;
       if (k < m && bx[q] >= 0) code = cls[(size_t)k * kElemCount + bx[q]];
       if (c < n && wx != nullptr && k < mpad)
-        wx[(size_t)c * mpad + k] = (k < m && bx[q] >= 0) ? (uint8_t)bx[q] : (uint8_t)0xFF;
+        wx[orow * mpad + k] = (k < m && bx[q] >= 0) ? (uint8_t)bx[q] : (uint8_t)0xFF;
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         const uint64_t bits = __ballot((code >> p) & 1u);
         if (c < n && (ln == 0 || ln == 32) && w0 < W)
-          planes[(size_t)c * 8u * W + 8u * w0 + p] = ln == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
+          planes[orow * 8u * W + 8u * w0 + p] = ln == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
       }
     }
   }
@@ -450,6 +451,7 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
   const uint32_t* __restrict__ perm = perm_all + desc[item.x].row0;
   const uint32_t ib = (item.y >> 16) * 256u, jbase = (item.y & 0xFFFFu) * 256u;
   const uint32_t ntile = item.z;
+  const uint32_t jmin = item.w;  // pairs with j < jmin were evaluated by an earlier build
   const uint32_t row0 = ib + 64 * wv;
 
   // Row operands go through LDS (buffer 1 is free until tile 1), so inside
@@ -527,7 +529,7 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
       if (__ballot(mx >= qcut) != 0) {  // some pair of these 1024 is near the gate
         const uint32_t j = j0 + (ln & 15u);
         const uint32_t jl = j - jbase;
-        const bool full = !diag && jb + 256 <= n;
+        const bool full = !diag && jb + 256 <= n && jb >= jmin;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           if (__ballot(mr[r] >= qcut) == 0) continue;  // only the 16-row block with candidates
@@ -535,7 +537,8 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const uint32_t i = row0 + 16 * r + 4 * (ln >> 4) + v;
-            const bool c = full ? acc[r][v] >= qcut : (acc[r][v] >= qcut && i < j && j < n);
+            const bool c = full ? acc[r][v] >= qcut
+                                : (acc[r][v] >= qcut && i < j && j < n && j >= jmin);
             const uint64_t mask = __ballot(c);
             if (mask) {
               if (c) buf[wv][bn + lanes_below(mask)] = ((i - ib) << 12) | jl;
@@ -740,6 +743,56 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict
   for (uint64_t tw = (uint64_t)blockIdx.x * 256 + (threadIdx.x & ~63u); tw < ns;
        tw += (uint64_t)gridDim.x * 256)  // wave-uniform loop
     scatter_one(surv, tw + (threadIdx.x & 63u), ns, off, cursor, list);
+}
+
+// ---------------------------------------------------------------------------
+// Incremental builds of a served session (fused regime).  The gated pairs of
+// the session's earlier rows are not evaluated again: their candidate lists,
+// kept from the earlier build with local ids (CSR), are merged into this
+// build's lists.  An old row's degree gains its old list length before the
+// offsets are scanned; its old entries then fill the head of its new list
+// (ids made global) and its cursor starts after them, so the scatter of the
+// new pairs appends.  Lists are sets: the selection does not depend on the
+// order of a row's entries.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void merge_deg_kernel(const MergeDesc* __restrict__ md,
+                                                        uint32_t* __restrict__ deg) {
+  const MergeDesc d = md[blockIdx.y];
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r < d.n_old) deg[d.row0 + r] += (uint32_t)(d.loff[r + 1] - d.loff[r]);
+}
+
+// One wave per old row.
+__global__ __launch_bounds__(256) void merge_lists_kernel(const MergeDesc* __restrict__ md,
+                                                          const uint64_t* __restrict__ off,
+                                                          uint32_t* __restrict__ cursor,
+                                                          uint64_t* __restrict__ list) {
+  const MergeDesc d = md[blockIdx.y];
+  const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63u;
+  if (r >= d.n_old) return;
+  const uint64_t s0 = d.loff[r], s1 = d.loff[r + 1];
+  const uint32_t g = d.row0 + r;
+  uint64_t* __restrict__ dst = list + off[g];
+  // entry = count << 32 | id: adding row0 (id + row0 < 2^24) only changes the id
+  for (uint64_t t = s0 + ln; t < s1; t += 64) dst[t - s0] = d.list[t] + d.row0;
+  if (ln == 0) cursor[g] = (uint32_t)(s1 - s0);
+}
+
+// Rows [row0, row0 + n) of a build's lists as a local CSR: loff[r] =
+// off[row0 + r] - off[row0] (r <= n), entries with ids made local.  One wave
+// per row.
+__global__ __launch_bounds__(256) void extract_lists_kernel(const uint64_t* __restrict__ off,
+                                                            const uint64_t* __restrict__ list,
+                                                            uint32_t row0, uint32_t n,
+                                                            uint64_t* __restrict__ loff,
+                                                            uint64_t* __restrict__ dst) {
+  const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63u;
+  if (r > n) return;
+  const uint64_t base = off[row0];
+  if (ln == 0) loff[r] = off[row0 + r] - base;
+  if (r == n) return;
+  const uint64_t s0 = off[row0 + r], s1 = off[row0 + r + 1];
+  for (uint64_t t = s0 + ln; t < s1; t += 64) dst[t - base] = list[t] - row0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1653,12 +1706,12 @@ __global__ __launch_bounds__(256) void scan_final_kernel(const uint32_t* __restr
 hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, uint32_t m,
                          uint32_t W, const double* sT, const uint8_t* sX, const uint8_t* cls,
                          const uint8_t* codes, uint8_t* wx, uint32_t mpad, uint32_t* planes,
-                         hipStream_t st) {
+                         const uint32_t* out_row, hipStream_t st) {
   constexpr int CPB = 4;
   const uint32_t blocks = (n + CPB - 1) / CPB;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(sketch_kernel<CPB>, dim3(blocks), dim3(256), 0, st, seqs, offs, n, m, W, sT,
-                     sX, cls, codes, wx, mpad, planes);
+                     sX, cls, codes, wx, mpad, planes, out_row);
   return hipGetLastError();
 }
 
@@ -1711,18 +1764,18 @@ hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, const AsmDesc
   return hipGetLastError();
 }
 
-void gate_work_items(uint32_t a, uint32_t n, uint32_t tiles_per_item, std::vector<uint32_t>& full,
-                     std::vector<uint32_t>& part) {
-  // (a, bi << 16 | bj0, tile count, 0); items with the full tile count go first
+void gate_work_items(uint32_t a, uint32_t n, uint32_t jmin, uint32_t tiles_per_item,
+                     std::vector<uint32_t>& full, std::vector<uint32_t>& part) {
+  // (a, bi << 16 | bj0, tile count, jmin); items with the full tile count go first
   const uint32_t T = (n + 255) / 256;
   for (uint32_t bi = 0; bi < T; ++bi)
-    for (uint32_t bj = bi; bj < T; bj += tiles_per_item) {
+    for (uint32_t bj = std::max(bi, jmin / 256); bj < T; bj += tiles_per_item) {
       const uint32_t cnt = std::min(tiles_per_item, T - bj);
       auto& dst = cnt == tiles_per_item ? full : part;
       dst.push_back(a);
       dst.push_back((bi << 16) | bj);
       dst.push_back(cnt);
-      dst.push_back(0);
+      dst.push_back(jmin);
     }
 }
 
@@ -1775,6 +1828,28 @@ hipError_t launch_scatter(const uint64_t* cand, uint64_t cap, const unsigned lon
   if (cap == 0) return hipSuccess;
   const dim3 grid((unsigned)std::min<uint64_t>((cap + 255) / 256, 65536));
   hipLaunchKernelGGL(scatter_kernel, grid, dim3(256), 0, st, cand, cap, ncand, off, cursor, list);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_deg(const MergeDesc* md, uint32_t nmd, uint32_t max_old, uint32_t* deg,
+                            hipStream_t st) {
+  if (nmd == 0 || max_old == 0) return hipSuccess;
+  hipLaunchKernelGGL(merge_deg_kernel, dim3((max_old + 255) / 256, nmd), dim3(256), 0, st, md, deg);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_lists(const MergeDesc* md, uint32_t nmd, uint32_t max_old, const uint64_t* off,
+                              uint32_t* cursor, uint64_t* list, hipStream_t st) {
+  if (nmd == 0 || max_old == 0) return hipSuccess;
+  hipLaunchKernelGGL(merge_lists_kernel, dim3((max_old + 3) / 4, nmd), dim3(256), 0, st, md, off,
+                     cursor, list);
+  return hipGetLastError();
+}
+
+hipError_t launch_extract_lists(const uint64_t* off, const uint64_t* list, uint32_t row0, uint32_t n,
+                                uint64_t* loff, uint64_t* dst, hipStream_t st) {
+  hipLaunchKernelGGL(extract_lists_kernel, dim3((n + 1 + 3) / 4), dim3(256), 0, st, off, list, row0,
+                     n, loff, dst);
   return hipGetLastError();
 }
 

@@ -83,3 +83,14 @@ def test_single_assembly_capture(ctx):
             asm.build_graph(p)
     finally:
         asm.free()
+
+
+def test_autotuned_batch_equals_plain(ctx, mixed, plain):
+    b = api.Batch(ctx, mixed)
+    try:
+        (tiles, rows), ms = b.autotune(api.default_params(rand_seed=1), reps=1)
+        assert tiles in (2, 4, 8, 16, 32) and rows in (16, 32, 64, 128, 256) and ms > 0
+        for k in range(len(mixed)):
+            _same(b.result(k), plain[k])
+    finally:
+        b.free()
