@@ -76,10 +76,14 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // Stage (a): sketch.  One workgroup of four waves sketches four contigs.
 //
-// Phase 1 (wave w, contig c0 + w): canonical 4-mer histogram in LDS
-// (src/rabbitbin.cpp:1459-1484).  Each lane takes 16-byte aligned chunks of
-// the sequence buffer plus the dword before the chunk (the 3-base overlap of
-// 4-mers that end in the chunk) and decodes them four bytes at a time:
+// Phase 1: canonical 4-mer histograms in LDS (src/rabbitbin.cpp:1459-1484).
+// The 16-byte aligned chunks of the four contigs are numbered one after
+// another and the workgroup's 256 threads stride over all of them, so the
+// workgroup's time follows the sum of its contigs' lengths, not the longest
+// (contig lengths are log-normal).  Each wave counts into its own histogram
+// per contig; the four are summed when the weights are formed.  A thread
+// takes a chunk plus the dword before it (the 3-base overlap of 4-mers that
+// end in the chunk) and decodes them four bytes at a time:
 //   u = byte & 0xDF folds a/c/g/t onto A/C/G/T; the 2-bit code is
 //   ((u >> 1) ^ (u >> 2)) & 3 (A 0, C 1, G 2, T 3); a byte is a base when u
 //   is one of the four letters (any other byte breaks every 4-mer covering
@@ -174,20 +178,14 @@ __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_
   }
 }
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
 template <int CPB>
 __global__ __launch_bounds__(256) void sketch_kernel(
     const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
     uint32_t W, const double* __restrict__ sT, const uint8_t* __restrict__ sX,
     const uint8_t* __restrict__ cls, const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx,
     uint32_t mpad, uint32_t* __restrict__ planes, const uint32_t* __restrict__ out_row) {
-  static_assert(CPB == kSketchWaves, "one contig per wave");
-  __shared__ uint32_t hist[CPB][256];
+  static_assert(CPB == kSketchWaves, "four contigs per workgroup");
+  __shared__ uint32_t hist[kSketchWaves][CPB][256];  // per wave, per contig
   __shared__ uint32_t bases[CPB][4];
   __shared__ uint32_t total[CPB];
   __shared__ double winv[CPB][kElemCount];
@@ -196,45 +194,76 @@ __global__ __launch_bounds__(256) void sketch_kernel(
   const uint32_t wv = tid >> 6, ln = tid & 63u;
   const uint32_t c0 = blockIdx.x * CPB;
 
-  for (uint32_t v = tid; v < CPB * 256; v += 256) (&hist[0][0])[v] = 0;
-  if (tid < CPB) wmin_bits[tid] = 0x7FF0000000000000ull;  // +inf: no element present
+  for (uint32_t v = tid; v < kSketchWaves * CPB * 256; v += 256) (&hist[0][0][0])[v] = 0;
+  if (tid < CPB) {
+    wmin_bits[tid] = 0x7FF0000000000000ull;  // +inf: no element present
+    total[tid] = 0;
+  }
+  if (tid < 4 * CPB) (&bases[0][0])[tid] = 0;
+  // chunk ranges of the four contigs: contig q owns chunks cj[q] .. cj[q] +
+  // nch - 1 of the buffer, numbered cpre[q] .. cpre[q+1] - 1 in the block
+  __shared__ int64_t cbeg[CPB], cend[CPB], cj[CPB];
+  __shared__ uint32_t cpre[CPB + 1];
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (int q = 0; q < CPB; ++q) {
+      const uint32_t c = c0 + q;
+      const int64_t b = c < n ? (int64_t)offs[c] : 0, e = c < n ? (int64_t)offs[c + 1] : 0;
+      cbeg[q] = b;
+      cend[q] = e;
+      cj[q] = b >> 4;
+      cpre[q] = acc;
+      acc += e > b ? (uint32_t)(((e - 1) >> 4) - (b >> 4) + 1) : 0u;
+    }
+    cpre[CPB] = acc;
+  }
   __syncthreads();
   {
-    const uint32_t c = c0 + wv;
+    const uint32_t nall = cpre[CPB];
     KmerTally tl;
-    if (c < n) {
-      const int64_t beg = (int64_t)offs[c], end = (int64_t)offs[c + 1];
-      if (end > beg) {
-        const int64_t j1 = (end - 1) >> 4;
-        int64_t j = (beg >> 4) + ln;
-        uint4 cur = make_uint4(0, 0, 0, 0);
-        uint32_t prev = 0;
-        if (j <= j1) {
-          cur = *reinterpret_cast<const uint4*>(seqs + 16 * j);
-          prev = j > 0 ? *reinterpret_cast<const uint32_t*>(seqs + 16 * j - 4) : 0u;
-        }
-        while (j <= j1) {
-          const int64_t jn = j + 64;
-          uint4 nxt = make_uint4(0, 0, 0, 0);
-          uint32_t nprev = 0;
-          if (jn <= j1) {  // next chunk in flight while this one is counted
-            nxt = *reinterpret_cast<const uint4*>(seqs + 16 * jn);
-            nprev = *reinterpret_cast<const uint32_t*>(seqs + 16 * jn - 4);
-          }
-          chunk_kmers(hist[wv], prev, cur, 16 * j - 4, beg, end, tl);
-          cur = nxt;
-          prev = nprev;
-          j = jn;
-        }
-      }
+    uint32_t tq = 0;  // contig of the tally (a thread's chunks come in contig order)
+    uint32_t g = tid, q = 0;
+    int64_t j = 0;
+    uint4 cur = make_uint4(0, 0, 0, 0);
+    uint32_t prev = 0;
+    if (g < nall) {
+      q = (g >= cpre[1]) + (g >= cpre[2]) + (g >= cpre[3]);
+      j = cj[q] + (int64_t)(g - cpre[q]);
+      cur = *reinterpret_cast<const uint4*>(seqs + 16 * j);
+      prev = j > 0 ? *reinterpret_cast<const uint32_t*>(seqs + 16 * j - 4) : 0u;
     }
+    while (g < nall) {
+      const uint32_t gn = g + 256;
+      uint32_t qn = 0;
+      int64_t jn = 0;
+      uint4 nxt = make_uint4(0, 0, 0, 0);
+      uint32_t nprev = 0;
+      if (gn < nall) {  // next chunk in flight while this one is counted
+        qn = (gn >= cpre[1]) + (gn >= cpre[2]) + (gn >= cpre[3]);
+        jn = cj[qn] + (int64_t)(gn - cpre[qn]);
+        nxt = *reinterpret_cast<const uint4*>(seqs + 16 * jn);
+        nprev = jn > 0 ? *reinterpret_cast<const uint32_t*>(seqs + 16 * jn - 4) : 0u;
+      }
+      if (q != tq) {  // the tally so far belongs to contig tq
+        if (tl.kmers | tl.base[0] | tl.base[1] | tl.base[2] | tl.base[3]) {
+          atomicAdd(&total[tq], tl.kmers);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) tl.base[b] = wave_sum(tl.base[b]);
-    tl.kmers = wave_sum(tl.kmers);
-    if (ln == 0) {
+          for (int b = 0; b < 4; ++b) atomicAdd(&bases[tq][b], tl.base[b]);
+        }
+        tl = KmerTally();
+        tq = q;
+      }
+      chunk_kmers(hist[wv][q], prev, cur, 16 * j - 4, cbeg[q], cend[q], tl);
+      cur = nxt;
+      prev = nprev;
+      g = gn;
+      q = qn;
+      j = jn;
+    }
+    if (tl.kmers | tl.base[0] | tl.base[1] | tl.base[2] | tl.base[3]) {
+      atomicAdd(&total[tq], tl.kmers);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) bases[wv][b] = tl.base[b];
-      total[wv] = tl.kmers;
+      for (int b = 0; b < 4; ++b) atomicAdd(&bases[tq][b], tl.base[b]);
     }
   }
   __syncthreads();
@@ -242,7 +271,10 @@ __global__ __launch_bounds__(256) void sketch_kernel(
     const uint32_t q = e / kElemCount, x = e % kElemCount;
     const uint32_t c = c0 + q;
     double wi = 0.0;
-    const uint32_t cnt = hist[q][codes[x]];
+    const uint32_t v = codes[x];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int w = 0; w < kSketchWaves; ++w) cnt += hist[w][q][v];
     const uint32_t tot = total[q];
     if (c < n && cnt != 0 && tot != 0) {
       const uint64_t vb = (uint64_t)bases[q][0] + bases[q][1] + bases[q][2] + bases[q][3];
@@ -251,7 +283,7 @@ __global__ __launch_bounds__(256) void sketch_kernel(
       for (int b = 0; b < 4; ++b) p[b] = vb > 0 ? (double)bases[q][b] / (double)vb : 0.25;
       const double inv = 1.0 / (double)tot;
       double pf = 1.0, pr = 1.0;
-      uint32_t code = codes[x];
+      uint32_t code = v;
 #pragma unroll
       for (int pos = 0; pos < 4; ++pos) {
         const uint32_t b = code & 3u;
@@ -699,6 +731,19 @@ __device__ __forceinline__ void gate_compare_one(
   if (live && q == 0) cand[t] = pass ? (e & ~0xFFFFull) | (uint64_t)(32u * W - mism) : kDeadPair;
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b on XCD b %
+// 8).  A kernel that walks a list in order gives workgroup b the contiguous
+// segment xcd_segment(b) of it, so the workgroups of one XCD hold adjacent
+// segments and the rows those touch stay in that XCD's L2 (gridDim.x is a
+// multiple of 8).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ void xcd_segment(uint64_t total, uint64_t& s0, uint64_t& s1) {
+  const uint64_t G = gridDim.x;
+  const uint64_t L = (uint64_t)(blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds;
+  s0 = total * L / G;
+  s1 = total * (L + 1) / G;
+}
+
 template <int NC, int W>
 __global__ __launch_bounds__(256) void gate_compare_kernel(
     uint64_t* __restrict__ cand, uint64_t cap, const unsigned long long* __restrict__ ncand,
@@ -706,11 +751,12 @@ __global__ __launch_bounds__(256) void gate_compare_kernel(
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
     const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
   const uint64_t nc = min((uint64_t)*ncand, cap);
+  uint64_t s0, s1;
+  xcd_segment(nc, s0, s1);
   const uint32_t q = threadIdx.x & 3u;
-  // wave-uniform loop: 16 candidates per wave per step
-  for (uint64_t tw = ((uint64_t)blockIdx.x * 256 + (threadIdx.x & ~63u)) >> 2; tw < nc;
-       tw += ((uint64_t)gridDim.x * 256) >> 2)
-    gate_compare_one<NC, W>(cand, tw + ((threadIdx.x & 63u) >> 2), nc, q, unit, anynz, desc,
+  // wave-uniform loop over the segment: 16 candidates per wave, 64 per workgroup per step
+  for (uint64_t tw = s0 + ((threadIdx.x & ~63u) >> 2); tw < s1; tw += 64)
+    gate_compare_one<NC, W>(cand, tw + ((threadIdx.x & 63u) >> 2), s1, q, unit, anynz, desc,
                             asm_of, planes, deg);
 }
 
@@ -740,9 +786,10 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict
                                                       uint32_t* __restrict__ cursor,
                                                       uint64_t* __restrict__ list) {
   const uint64_t ns = min((uint64_t)*ncand, cap);
-  for (uint64_t tw = (uint64_t)blockIdx.x * 256 + (threadIdx.x & ~63u); tw < ns;
-       tw += (uint64_t)gridDim.x * 256)  // wave-uniform loop
-    scatter_one(surv, tw + (threadIdx.x & 63u), ns, off, cursor, list);
+  uint64_t s0, s1;
+  xcd_segment(ns, s0, s1);
+  for (uint64_t tw = s0 + (threadIdx.x & ~63u); tw < s1; tw += 256)  // wave-uniform loop
+    scatter_one(surv, tw + (threadIdx.x & 63u), s1, off, cursor, list);
 }
 
 // ---------------------------------------------------------------------------
@@ -1798,7 +1845,8 @@ hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long
                                const AsmDesc* desc, const uint16_t* asm_of,
                                const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st) {
   if (cap == 0) return hipSuccess;
-  const dim3 grid((unsigned)std::min<uint64_t>((cap * 4 + 255) / 256, 65536));
+  // a multiple of the XCD count (xcd_segment)
+  const dim3 grid((unsigned)std::max<uint64_t>(kXcds, std::min<uint64_t>((cap * 4 + 255) / 256, 65536) & ~7ull));
 #define RBG_GC(NCV, WV)                                                                            \
   hipLaunchKernelGGL((gate_compare_kernel<NCV, WV>), grid, dim3(256), 0, st, cand, cap, ncand, unit, \
                      anynz, desc, asm_of, planes, deg)
@@ -1826,7 +1874,7 @@ hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long
 hipError_t launch_scatter(const uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
                           const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st) {
   if (cap == 0) return hipSuccess;
-  const dim3 grid((unsigned)std::min<uint64_t>((cap + 255) / 256, 65536));
+  const dim3 grid((unsigned)std::max<uint64_t>(kXcds, std::min<uint64_t>((cap + 255) / 256, 65536) & ~7ull));
   hipLaunchKernelGGL(scatter_kernel, grid, dim3(256), 0, st, cand, cap, ncand, off, cursor, list);
   return hipGetLastError();
 }

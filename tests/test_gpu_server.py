@@ -142,7 +142,7 @@ def test_server_budget_and_errors():
     small = _part(1000, 10, 3, 93)
     big = _part(26000, 200, 10, 94)
     with api.Server(0, hbm_budget=400 << 20) as srv:
-        _, rb = srv.submit(*big)
+        sb, rb = srv.submit(*big)
         ses, rs = srv.submit(*small)
         with pytest.raises(api.RbgError, match="status -5"):
             srv.wait(rb, timeout=120)
@@ -155,4 +155,8 @@ def test_server_budget_and_errors():
         srv.release(ses)
         with pytest.raises(api.RbgError):
             srv.extend(ses, *small)  # released
+        with pytest.raises(api.RbgError, match="failed earlier"):
+            srv.wait(srv.extend(sb, *_part(100, 2, 10, 96)), timeout=60)  # its first request failed
+        assert srv.stats()["sessions"] == 1  # the failed session stays until released
+        srv.release(sb)
         assert srv.stats()["sessions"] == 0
