@@ -111,6 +111,20 @@ def batched_form(ctx, steps):
         plain_ms.append(best)
         plain_edges.append(int(r["n_edges"]))
         a.free()
+    # host to host: the plain path's rbg_build_graph per assembly (upload,
+    # build, edges back) against one batch created, built and read back
+    t1 = time.perf_counter()
+    for seqs, off, depth in asms:
+        ctx.build_graph(seqs, off, depth, params)
+    plain_host_ms = 1e3 * (time.perf_counter() - t1)
+    t1 = time.perf_counter()
+    bh = api.Batch(ctx, asms)
+    bh.build(params)
+    for k in range(len(asms)):
+        bh.result(k)
+    batch_host_ms = 1e3 * (time.perf_counter() - t1)
+    bh.free()
+    log(f"[bench/batched] host to host: plain {plain_host_ms:.0f} ms, batch {batch_host_ms:.0f} ms")
     b = api.Batch(ctx, asms)
     (tiles, rows), tune_ms = b.autotune(params, reps=2)
     first = b.build(params, capture=True)
@@ -130,6 +144,10 @@ def batched_form(ctx, steps):
            "ms_per_step": ms, "value": pairs / (ms / 1e3), "unit": "pairs/s",
            "plain_ms_total": plain_total, "plain_value": pairs / (plain_total / 1e3),
            "speedup_vs_plain": plain_total / ms, "edges": int(st["edges"]),
+           "host_to_host": {"plain_ms": plain_host_ms, "batch_ms": batch_host_ms,
+                            "speedup": plain_host_ms / batch_host_ms,
+                            "note": "rbg_build_graph per assembly vs one batch (create, build, results), "
+                                    "inputs and edge lists on the host"},
            "schedule": {"gate_tiles_per_item": tiles, "dense_rows_per_item": rows, "autotune_ms": tune_ms},
            "note": "inputs resident in HBM for both; plain = one resident assembly built at a time (best of 2)"}
     log(f"[bench/batched] {ms:.1f} ms/step vs plain {plain_total:.1f} ms: x{plain_total / ms:.2f}")
@@ -194,7 +212,8 @@ def served_form(count, rate):
                 "value": gp / (t_last - t_first), "p50_ms": pctl(lat, 50), "p99_ms": pctl(lat, 99),
                 "p99_ms_assemblies_ge_40k": pctl(big, 99), "steps": st["steps"],
                 "mean_requests_per_step": len(recs) / max(1, st["steps"]),
-                "incremental": sum(i["incremental"] for i in recs), "busy_ms": st["busy_ms"]}
+                "incremental": sum(i["incremental"] for i in recs),
+                "step_ms": {k: st[k] for k in ("busy_ms", "upload_ms", "build_ms", "result_ms", "save_ms")}}
 
     offered = run(arrivals)
     log(f"[bench/served] offered {rate}/s: {offered}")
@@ -250,25 +269,8 @@ def served_form(count, rate):
             "speedup_vs_plain_saturated": saturated["value"] / plain["value"]}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="asm300k")
-    ap.add_argument("--n", type=int, default=None, help="override the contig count")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--forms", default="plain,batched,served",
-                    help="comma list of: plain (the headline line), batched, served")
-    ap.add_argument("--served-sessions", type=int, default=24)
-    ap.add_argument("--served-rate", type=float, default=4.0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              f"(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
+def plain_form(args, ctx):
+    """The headline line: repeated builds of the resident 300,000-contig assembly."""
     import numpy as np
     from rabbitbin_amd import api, synth
 
@@ -279,7 +281,6 @@ def main():
     log(f"[bench] generated {args.workload}: n={n} bases={len(seqs)} S={w['S']} in "
         f"{time.perf_counter() - t0:.1f}s")
     params = api.default_params(rand_seed=1)
-    ctx = api.Context(0)
     t0 = time.perf_counter()
     asm = api.Assembly(ctx, seqs, offsets, depth)
     log(f"[bench] upload + depth features {time.perf_counter() - t0:.2f}s")
@@ -334,7 +335,6 @@ def main():
                    "ms_per_step_with_edges_to_host": ms_host},
         "roofline": roofline,
     }
-    ctx_keep = (asm, ctx)
     if not args.no_cpu_baseline:
         try:
             threads = min(16, os.cpu_count() or 1)
@@ -342,8 +342,33 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": repr(e)}
     asm.free()
-    del ctx_keep
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="asm300k")
+    ap.add_argument("--n", type=int, default=None, help="override the contig count")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="plain,batched,served",
+                    help="comma list of: plain (the headline line), batched, served")
+    ap.add_argument("--served-sessions", type=int, default=24)
+    ap.add_argument("--served-rate", type=float, default=4.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from rabbitbin_amd import api
+
+    ctx = api.Context(0)
     forms = args.forms.split(",")
+    out = plain_form(args, ctx) if "plain" in forms else {"forms": forms}
     if "batched" in forms:
         try:
             out["batched"] = batched_form(ctx, args.steps)

@@ -253,7 +253,11 @@ typedef struct rbg_server_stats {
   uint64_t graph_pairs;    /* sum over served requests of n(n-1)/2 of their graphs */
   uint64_t pass_pairs;     /* pairs evaluated */
   uint64_t resident_bytes, workspace_bytes, budget;
-  double busy_ms;          /* wall time inside steps */
+  double busy_ms;          /* wall time inside steps, split into: */
+  double upload_ms;        /*   host preparation and upload of the steps' inputs */
+  double build_ms;         /*   the batched builds on the device */
+  double result_ms;        /*   edge lists to the host */
+  double save_ms;          /*   sessions' device state kept for later extensions */
 } rbg_server_stats;
 
 void rbg_server_default_config(rbg_server_config* c);

@@ -97,7 +97,8 @@ class ServerStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("requests", C.c_uint64), ("sessions", C.c_uint64),
                 ("pending", C.c_uint64), ("running", C.c_uint64), ("graph_pairs", C.c_uint64),
                 ("pass_pairs", C.c_uint64), ("resident_bytes", C.c_uint64), ("workspace_bytes", C.c_uint64),
-                ("budget", C.c_uint64), ("busy_ms", C.c_double)]
+                ("budget", C.c_uint64), ("busy_ms", C.c_double), ("upload_ms", C.c_double),
+                ("build_ms", C.c_double), ("result_ms", C.c_double), ("save_ms", C.c_double)]
 
 
 class GraphDiag(C.Structure):

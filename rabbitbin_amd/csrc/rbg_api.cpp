@@ -33,6 +33,7 @@ void rbg_close(rbg_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   ctx->tables.clear();
+  release_staging(ctx);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

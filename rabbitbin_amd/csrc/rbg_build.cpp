@@ -4,12 +4,13 @@
 #include "rbg_build.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <deque>
 #include <numeric>
+#include <thread>
 
 using namespace rbg;
 
@@ -63,6 +64,111 @@ rbg_status get_tables(rbg_ctx* ctx, uint32_t m, uint64_t seed, rbg_ctx::Tables**
   *out = t.get();
   ctx->tables.push_back(std::move(t));
   return RBG_OK;
+}
+
+namespace {
+
+constexpr size_t kStageHalf = 32u << 20;   // bytes per staging half
+constexpr size_t kStageMin = 4u << 20;     // smaller copies go direct
+
+// dst = src (sub == 0) or dst[i] = src[i] - sub over 32-bit words, on up to
+// eight threads.
+void parallel_copy(void* dst, const void* src, size_t bytes, uint32_t sub) {
+  const size_t nt = std::min<size_t>(8, std::max<size_t>(1, bytes >> 22));
+  auto part = [&](size_t t) {
+    const size_t words = bytes / 4;
+    const size_t w0 = words * t / nt, w1 = words * (t + 1) / nt;
+    if (sub) {
+      const uint32_t* s = (const uint32_t*)src;
+      uint32_t* d = (uint32_t*)dst;
+      for (size_t w = w0; w < w1; ++w) d[w] = s[w] - sub;
+    } else {
+      std::memcpy((char*)dst + 4 * w0, (const char*)src + 4 * w0, 4 * (w1 - w0));
+    }
+    if (t + 1 == nt && (bytes & 3))  // a tail of bytes (sub is only used on whole words)
+      std::memcpy((char*)dst + (bytes & ~(size_t)3), (const char*)src + (bytes & ~(size_t)3), bytes & 3);
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; ++t) pool.emplace_back(part, t);
+  part(0);
+  for (auto& th : pool) th.join();
+}
+
+rbg_status ensure_staging(rbg_ctx* ctx) {
+  if (ctx->pin.p) return RBG_OK;
+  RBG_TRY_HIP(ctx, hipHostMalloc((void**)&ctx->pin.p, 2 * kStageHalf, hipHostMallocDefault));
+  ctx->pin.half = kStageHalf;
+  for (auto& e : ctx->pin.done) RBG_TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return RBG_OK;
+}
+
+// Device -> host through the staging halves: chunk c lands in half c & 1
+// while the host drains chunk c - 1.
+rbg_status d2h_chunks(rbg_ctx* ctx, void* dst, const void* src, size_t bytes, uint32_t sub) {
+  hipStream_t st = ctx->stream;
+  if (bytes < kStageMin) {
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+    if (sub)
+      for (size_t w = 0; w < bytes / 4; ++w) ((uint32_t*)dst)[w] -= sub;
+    return RBG_OK;
+  }
+  RBG_TRY(ensure_staging(ctx));
+  const size_t H = ctx->pin.half, nch = (bytes + H - 1) / H;
+  for (size_t c = 0; c <= nch; ++c) {
+    if (c < nch) {
+      const size_t off = c * H, len = std::min(H, bytes - off);
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(ctx->pin.p + (c & 1) * H, (const char*)src + off, len,
+                                      hipMemcpyDeviceToHost, st));
+      RBG_TRY_HIP(ctx, hipEventRecord(ctx->pin.done[c & 1], st));
+    }
+    if (c > 0) {
+      const size_t pc = c - 1, off = pc * H, len = std::min(H, bytes - off);
+      RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[pc & 1]));
+      parallel_copy((char*)dst + off, ctx->pin.p + (pc & 1) * H, len, sub);
+    }
+  }
+  return RBG_OK;
+}
+
+}  // namespace
+
+rbg_status h2d_staged(rbg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  hipStream_t st = ctx->stream;
+  if (bytes < kStageMin) {
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    return RBG_OK;
+  }
+  RBG_TRY(ensure_staging(ctx));
+  const size_t H = ctx->pin.half;
+  for (size_t off = 0, c = 0; off < bytes; off += H, ++c) {
+    const size_t len = std::min(H, bytes - off);
+    if (c >= 2) RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[c & 1]));  // the half's last DMA
+    parallel_copy(ctx->pin.p + (c & 1) * H, (const char*)src + off, len, 0);
+    RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)dst + off, ctx->pin.p + (c & 1) * H, len,
+                                    hipMemcpyHostToDevice, st));
+    RBG_TRY_HIP(ctx, hipEventRecord(ctx->pin.done[c & 1], st));
+  }
+  // the staging halves may still be read by the DMA: the next staged copy
+  // waits on their events, and callers synchronise before returning results
+  RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[0]));
+  RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[1]));
+  return RBG_OK;
+}
+
+rbg_status d2h_staged(rbg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return d2h_chunks(ctx, dst, src, bytes, 0);
+}
+
+rbg_status d2h_u32_staged(rbg_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t count, uint32_t sub) {
+  return d2h_chunks(ctx, dst, src, count * 4, sub);
+}
+
+void release_staging(rbg_ctx* ctx) {
+  if (ctx->pin.p) (void)hipHostFree(ctx->pin.p);
+  for (auto& e : ctx->pin.done)
+    if (e) (void)hipEventDestroy(e);
+  ctx->pin = rbg_ctx::Staging{};
 }
 
 double elapsed_ms(hipEvent_t a, hipEvent_t b) {
@@ -186,28 +292,21 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
   RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->unit_pad.p, rows * 16 * NC, 0u, st));
   RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->raw_pad.p, rows * 16 * NC, 0u, st));
   RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->q8.p, rows * kpad / 4, 0u, st));
-  // staging buffers stay alive until the final synchronisation
-  std::deque<std::vector<float>> stage_f;
-  std::deque<std::vector<int8_t>> stage_q;
-  uint64_t seq0 = 0, sk = 0;
-  for (uint32_t a = 0; a < count; ++a) {
+  // Host preparation of each assembly's new rows (depth features, column
+  // norms, pair-pass order, int8 rows, b0 pair draws), assemblies in parallel;
+  // then the copies, in order.
+  struct Prep {
+    std::vector<float> upad, rpad;  // new rows, padded to 16 * NC
+    std::vector<int8_t> q8;         // new rows in pair-pass order, kpad bytes each
+    std::vector<uint32_t> ai, aj;   // b0 pair draws (local rows)
+  };
+  std::vector<Prep> prep(count);
+  auto prepare = [&](uint32_t a, bool parallel) {
     const rbg_assembly_in& x = in[a].in;
     AsmHost& h = b->asms[a];
+    Prep& pr = prep[a];
     const Resident* src = h.src;
     const uint32_t n_old = h.n_old, n_new = x.n, S = x.S;
-    // contigs to sketch
-    const uint64_t nb = n_new ? x.offsets[n_new] - x.offsets[0] : 0;
-    if (nb)
-      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->seqs.p + seq0, x.seqs + x.offsets[0], nb,
-                                      hipMemcpyHostToDevice, st));
-    for (uint32_t r = 0; r < n_new; ++r) {
-      offs[sk + r] = seq0 + (x.offsets[r] - x.offsets[0]);
-      sk_rows[sk + r] = h.row0 + n_old + r;
-    }
-    sk += n_new;
-    seq0 += nb;
-    offs[sk] = seq0;
-    for (uint32_t r = 0; r < h.n; ++r) asm_of[h.row0 + r] = (uint16_t)a;
     // depth features of the new rows (row-local), colnorm from running column
     // sums (the same additions in the same order as over all rows at once)
     h.anynz_local.assign(h.n, 1);
@@ -217,7 +316,7 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
       unit.assign((size_t)n_new * S, 0.f);
       raw.assign((size_t)n_new * S, 0.f);
       depth_features(x.depth, n_new, S, min_cv, nullptr, unit.data(), raw.data(), nullptr,
-                     h.anynz_local.data() + n_old);
+                     h.anynz_local.data() + n_old, parallel);
       h.colsum.assign(S, 0.0);
       if (src) h.colsum = src->colsum;
       for (uint32_t c = 0; c < S; ++c) {
@@ -228,28 +327,13 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
         colnorm[h.colnorm0 + c] = S > 1 ? ((mean > 1e-12) ? 1.0 / mean : 0.0) : 0.0;
       }
     }
-    std::copy(h.anynz_local.begin(), h.anynz_local.end(), anynz.begin() + h.row0);
     if (S > 1 && n_new) {
-      stage_f.emplace_back((size_t)n_new * 16 * NC, 0.f);
-      std::vector<float>& upad = stage_f.back();
-      stage_f.emplace_back((size_t)n_new * 16 * NC, 0.f);
-      std::vector<float>& rpad = stage_f.back();
+      pr.upad.assign((size_t)n_new * 16 * NC, 0.f);
+      pr.rpad.assign((size_t)n_new * 16 * NC, 0.f);
       for (uint32_t r = 0; r < n_new; ++r) {
-        std::memcpy(&upad[(size_t)r * 16 * NC], &unit[(size_t)r * S], S * 4);
-        std::memcpy(&rpad[(size_t)r * 16 * NC], &raw[(size_t)r * S], S * 4);
+        std::memcpy(&pr.upad[(size_t)r * 16 * NC], &unit[(size_t)r * S], S * 4);
+        std::memcpy(&pr.rpad[(size_t)r * 16 * NC], &raw[(size_t)r * S], S * 4);
       }
-      const size_t o = (size_t)(h.row0 + n_old) * 16 * NC * 4;
-      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->unit_pad.p + o, upad.data(), upad.size() * 4,
-                                      hipMemcpyHostToDevice, st));
-      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->raw_pad.p + o, rpad.data(), rpad.size() * 4,
-                                      hipMemcpyHostToDevice, st));
-    }
-    if (src && S > 1 && n_old) {
-      const size_t sp = 16 * nc_for(S) * 4, dp = 16 * NC * 4;
-      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((char*)b->unit_pad.p + (size_t)h.row0 * dp, dp, src->unit_pad.p, sp,
-                                        sp, n_old, hipMemcpyDeviceToDevice, st));
-      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((char*)b->raw_pad.p + (size_t)h.row0 * dp, dp, src->raw_pad.p, sp,
-                                        sp, n_old, hipMemcpyDeviceToDevice, st));
     }
     // pair-pass order and int8 rows.  The pass visits the assembly's rows in
     // abundance order (pair_pass_order): a session keeps its earlier rows'
@@ -270,33 +354,75 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
       } else {
         gate_quantize(sorted.data(), n_new, S, h.gq);
       }
-      stage_q.emplace_back((size_t)n_new * kpad, 0);
-      std::vector<int8_t>& q8 = stage_q.back();
+      pr.q8.assign((size_t)n_new * kpad, 0);
       for (uint32_t r = 0; r < n_new; ++r)
-        std::memcpy(&q8[(size_t)r * kpad], &h.gq.q[(size_t)r * h.gq.kpad], h.gq.kpad);
+        std::memcpy(&pr.q8[(size_t)r * kpad], &h.gq.q[(size_t)r * h.gq.kpad], h.gq.kpad);
       std::vector<int8_t>().swap(h.gq.q);
-      RBG_TRY_HIP(ctx, hipMemcpyAsync((int8_t*)b->q8.p + (size_t)(h.row0 + n_old) * kpad, q8.data(),
-                                      q8.size(), hipMemcpyHostToDevice, st));
     } else {
       for (uint32_t r = 0; r < n_new; ++r) h.perm_local[n_old + r] = n_old + r;
       if (src) h.gq = src->gq;
     }
-    if (src && S > 1 && n_old) {
-      const size_t sp = kpad_for(S);
-      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((int8_t*)b->q8.p + (size_t)h.row0 * kpad, kpad, src->q8.p, sp, sp,
+    baseline_pairs(h.n, pr.ai, pr.aj);  // b0 pair draws (they depend on n only)
+  };
+  if (count == 1) {
+    prepare(0, true);
+  } else {
+    std::atomic<uint32_t> next{0};
+    auto work = [&] {
+      for (uint32_t a; (a = next.fetch_add(1)) < count;) prepare(a, false);
+    };
+    const uint32_t nt = std::min<uint32_t>(count, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+  }
+  uint64_t seq0 = 0, sk = 0;
+  for (uint32_t a = 0; a < count; ++a) {
+    const rbg_assembly_in& x = in[a].in;
+    AsmHost& h = b->asms[a];
+    Prep& pr = prep[a];
+    const Resident* src = h.src;
+    const uint32_t n_old = h.n_old, n_new = x.n, S = x.S;
+    // contigs to sketch
+    const uint64_t nb = n_new ? x.offsets[n_new] - x.offsets[0] : 0;
+    if (nb) RBG_TRY(h2d_staged(ctx, (char*)b->seqs.p + seq0, x.seqs + x.offsets[0], nb));
+    for (uint32_t r = 0; r < n_new; ++r) {
+      offs[sk + r] = seq0 + (x.offsets[r] - x.offsets[0]);
+      sk_rows[sk + r] = h.row0 + n_old + r;
+    }
+    sk += n_new;
+    seq0 += nb;
+    offs[sk] = seq0;
+    for (uint32_t r = 0; r < h.n; ++r) asm_of[h.row0 + r] = (uint16_t)a;
+    std::copy(h.anynz_local.begin(), h.anynz_local.end(), anynz.begin() + h.row0);
+    if (!pr.upad.empty()) {
+      const size_t o = (size_t)(h.row0 + n_old) * 16 * NC * 4;
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->unit_pad.p + o, pr.upad.data(), pr.upad.size() * 4,
+                                      hipMemcpyHostToDevice, st));
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->raw_pad.p + o, pr.rpad.data(), pr.rpad.size() * 4,
+                                      hipMemcpyHostToDevice, st));
+    }
+    if (!pr.q8.empty())
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((int8_t*)b->q8.p + (size_t)(h.row0 + n_old) * kpad, pr.q8.data(),
+                                      pr.q8.size(), hipMemcpyHostToDevice, st));
+    if (src && S > 1 && n_old) {  // the session's rows, copied on the device
+      const size_t sp = 16 * nc_for(S) * 4, dp = 16 * NC * 4, kp = kpad_for(S);
+      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((char*)b->unit_pad.p + (size_t)h.row0 * dp, dp, src->unit_pad.p, sp,
+                                        sp, n_old, hipMemcpyDeviceToDevice, st));
+      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((char*)b->raw_pad.p + (size_t)h.row0 * dp, dp, src->raw_pad.p, sp,
+                                        sp, n_old, hipMemcpyDeviceToDevice, st));
+      RBG_TRY_HIP(ctx, hipMemcpy2DAsync((int8_t*)b->q8.p + (size_t)h.row0 * kpad, kpad, src->q8.p, kp, kp,
                                         n_old, hipMemcpyDeviceToDevice, st));
     }
     for (uint32_t r = 0; r < h.n; ++r) perm[h.row0 + r] = h.row0 + h.perm_local[r];
-    // b0 pair draws (they depend on n only)
-    std::vector<uint32_t> ai, aj;
-    baseline_pairs(h.n, ai, aj);
     h.pair0 = (uint32_t)pi.size();
-    h.np = (uint32_t)ai.size();
-    for (size_t t = 0; t < ai.size(); ++t) {
-      pi.push_back(h.row0 + ai[t]);
-      pj.push_back(h.row0 + aj[t]);
+    h.np = (uint32_t)pr.ai.size();
+    for (size_t t = 0; t < pr.ai.size(); ++t) {
+      pi.push_back(h.row0 + pr.ai[t]);
+      pj.push_back(h.row0 + pr.aj[t]);
     }
-    pairs += ai.size();
+    pairs += pr.ai.size();
   }
   b->np_total = (uint32_t)pairs;
   RBG_TRY_HIP(ctx, b->pi.ensure(std::max<size_t>(pi.size(), 1) * 4));
@@ -594,8 +720,8 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                     (const uint32_t*)b->cal_roff.p, P->coverage, bs, st));
   RBG_TRY_HIP(ctx, launch_emit_mark((const uint32_t*)b->topk_id.p, (const uint16_t*)b->topk_cnt.p,
                                     (const uint32_t*)b->topk_len.p, (const uint32_t*)b->fence.p, FW,
-                                    N, K, bs, desc, asm_of, (uint64_t*)b->keep.p,
-                                    (uint32_t*)b->row_edges.p, st));
+                                    N, K, bs, desc, asm_of, (const uint32_t*)b->perm.p,
+                                    (uint64_t*)b->keep.p, (uint32_t*)b->row_edges.p, st));
   {
     size_t tmp = b->scan_tmp.bytes;
     RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)b->row_edges.p,
@@ -697,11 +823,11 @@ rbg_status assembly_result(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph
     out->score = h.S > 0 ? (float*)std::malloc(nb * 4) : nullptr;
     if (!out->from || !out->to || !out->scomp || (h.S > 0 && !out->score))
       return ctx->fail(RBG_ERR_NOMEM, "host allocation of the edge list");
-    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->from, (uint32_t*)b->efrom.p + e0, ne * 4, hipMemcpyDeviceToHost, st));
-    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->to, (uint32_t*)b->eto.p + e0, ne * 4, hipMemcpyDeviceToHost, st));
-    RBG_TRY_HIP(ctx, hipMemcpyAsync(out->scomp, (float*)b->escomp.p + e0, ne * 4, hipMemcpyDeviceToHost, st));
-    if (h.S > 0)
-      RBG_TRY_HIP(ctx, hipMemcpyAsync(out->score, (float*)b->escore.p + e0, ne * 4, hipMemcpyDeviceToHost, st));
+    // global rows -> ids local to the assembly on the way
+    RBG_TRY(d2h_u32_staged(ctx, out->from, (const uint32_t*)b->efrom.p + e0, ne, r0));
+    RBG_TRY(d2h_u32_staged(ctx, out->to, (const uint32_t*)b->eto.p + e0, ne, r0));
+    RBG_TRY(d2h_staged(ctx, out->scomp, (const float*)b->escomp.p + e0, ne * 4));
+    if (h.S > 0) RBG_TRY(d2h_staged(ctx, out->score, (const float*)b->escore.p + e0, ne * 4));
   }
   if (diag) {
     if (diag->topk_id)
@@ -719,11 +845,6 @@ rbg_status assembly_result(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph
   }
   RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
   if (r0) {  // global rows -> ids local to the assembly
-    if (!on_device)
-      for (uint64_t e = 0; e < ne; ++e) {
-        out->from[e] -= r0;
-        out->to[e] -= r0;
-      }
     if (diag && diag->topk_id && diag->topk_len)
       for (uint32_t r = 0; r < n; ++r)
         for (uint32_t t2 = 0; t2 < diag->topk_len[r]; ++t2) diag->topk_id[(size_t)r * K + t2] -= r0;
@@ -750,48 +871,41 @@ rbg_status save_resident(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_p
   hipStream_t st = ctx->stream;
   AsmHost& h = b->asms[a];
   const uint32_t n = h.n, S = h.S, W = words_for(P->m);
-  // Device rows are copied before any of r's buffers is replaced: r may be
-  // this build's source, and a buffer that grows is reallocated.
-  DevBuf planes, unit, raw, q8, loff, list;
-  RBG_TRY_HIP(ctx, planes.ensure((size_t)n * 8 * W * 4));
-  RBG_TRY_HIP(ctx, hipMemcpyAsync(planes.p, (uint32_t*)b->planes.p + (size_t)h.row0 * 8 * W,
+  // The build is complete, so r's buffers (this build's source when the
+  // session was extended) may be overwritten: every row is copied from the
+  // build.  Buffers grow with headroom so most extensions reuse them.
+  auto fit = [&](DevBuf& d, size_t need) -> hipError_t {
+    return need <= d.bytes && d.p ? hipSuccess : d.ensure(need + need / 4);
+  };
+  RBG_TRY_HIP(ctx, fit(r->planes, (size_t)n * 8 * W * 4));
+  RBG_TRY_HIP(ctx, hipMemcpyAsync(r->planes.p, (uint32_t*)b->planes.p + (size_t)h.row0 * 8 * W,
                                   (size_t)n * 8 * W * 4, hipMemcpyDeviceToDevice, st));
   if (S > 1) {
     const size_t sp = 16 * nc_for(S) * 4, bp = 16 * b->NC * 4;
-    RBG_TRY_HIP(ctx, unit.ensure((size_t)n * sp));
-    RBG_TRY_HIP(ctx, raw.ensure((size_t)n * sp));
-    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(unit.p, sp, (char*)b->unit_pad.p + (size_t)h.row0 * bp, bp, sp, n,
-                                      hipMemcpyDeviceToDevice, st));
-    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(raw.p, sp, (char*)b->raw_pad.p + (size_t)h.row0 * bp, bp, sp, n,
-                                      hipMemcpyDeviceToDevice, st));
+    RBG_TRY_HIP(ctx, fit(r->unit_pad, (size_t)n * sp));
+    RBG_TRY_HIP(ctx, fit(r->raw_pad, (size_t)n * sp));
+    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(r->unit_pad.p, sp, (char*)b->unit_pad.p + (size_t)h.row0 * bp, bp, sp,
+                                      n, hipMemcpyDeviceToDevice, st));
+    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(r->raw_pad.p, sp, (char*)b->raw_pad.p + (size_t)h.row0 * bp, bp, sp,
+                                      n, hipMemcpyDeviceToDevice, st));
     const size_t kp = kpad_for(S);
-    RBG_TRY_HIP(ctx, q8.ensure((size_t)n * kp));
-    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(q8.p, kp, (int8_t*)b->q8.p + (size_t)h.row0 * b->kpad, b->kpad, kp, n,
-                                      hipMemcpyDeviceToDevice, st));
+    RBG_TRY_HIP(ctx, fit(r->q8, (size_t)n * kp));
+    RBG_TRY_HIP(ctx, hipMemcpy2DAsync(r->q8.p, kp, (int8_t*)b->q8.p + (size_t)h.row0 * b->kpad, b->kpad, kp,
+                                      n, hipMemcpyDeviceToDevice, st));
   }
   const int regime = (int)b->desc_h[a].regime;
   uint64_t nlist = 0;
   if (regime == 2) {
     nlist = b->bounds_h[4 * a + 3] - b->bounds_h[4 * a + 2];
-    RBG_TRY_HIP(ctx, loff.ensure(((size_t)n + 1) * 8));
-    RBG_TRY_HIP(ctx, list.ensure(std::max<uint64_t>(nlist, 1) * 8));
+    RBG_TRY_HIP(ctx, fit(r->loff, ((size_t)n + 1) * 8));
+    RBG_TRY_HIP(ctx, fit(r->list, std::max<uint64_t>(nlist, 1) * 8));
     RBG_TRY_HIP(ctx, launch_extract_lists((const uint64_t*)b->off.p, (const uint64_t*)b->list.p, h.row0, n,
-                                          (uint64_t*)loff.p, (uint64_t*)list.p, st));
+                                          (uint64_t*)r->loff.p, (uint64_t*)r->list.p, st));
+  } else {
+    r->loff.release();
+    r->list.release();
   }
   RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
-  auto take = [](DevBuf& dst, DevBuf& from) {
-    dst.release();
-    dst.p = from.p;
-    dst.bytes = from.bytes;
-    from.p = nullptr;
-    from.bytes = 0;
-  };
-  take(r->planes, planes);
-  take(r->unit_pad, unit);
-  take(r->raw_pad, raw);
-  take(r->q8, q8);
-  take(r->loff, loff);
-  take(r->list, list);
   r->nlist = nlist;
   r->n = n;
   r->S = S;

@@ -30,6 +30,13 @@ struct rbg_ctx {
   };
   std::vector<std::unique_ptr<Tables>> tables;
   hipEvent_t ev[12] = {};
+  // Pinned host staging for large transfers: two halves, so the pageable side
+  // of one chunk is copied while the other chunk's DMA runs.
+  struct Staging {
+    char* p = nullptr;
+    size_t half = 0;
+    hipEvent_t done[2] = {nullptr, nullptr};
+  } pin;
   rbg_status fail(rbg_status st, const std::string& msg) {
     err = msg;
     return st;
@@ -55,6 +62,14 @@ struct rbg_ctx {
 namespace rbg {
 
 uint32_t words_for(uint32_t m);
+// Host <-> device copies of large pageable buffers through ctx->pin (the
+// pageable side copied by several threads).  h2d returns once the source may
+// be reused; d2h returns when dst holds the data.  d2h_u32 subtracts `sub`
+// from every 32-bit word on the way.
+rbg_status h2d_staged(rbg_ctx* ctx, void* dst, const void* src, size_t bytes);
+rbg_status d2h_staged(rbg_ctx* ctx, void* dst, const void* src, size_t bytes);
+rbg_status d2h_u32_staged(rbg_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t count, uint32_t sub);
+void release_staging(rbg_ctx* ctx);
 rbg_status get_tables(rbg_ctx* ctx, uint32_t m, uint64_t seed, rbg_ctx::Tables** out);
 double elapsed_ms(hipEvent_t a, hipEvent_t b);
 
@@ -188,7 +203,8 @@ rbg_status run_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P, bool w
 // Assembly `a` of the last build (edges with ids local to the assembly).
 rbg_status assembly_result(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_params* P,
                            bool on_device, rbg_graph_result* out, rbg_graph_diag* diag);
-// Moves assembly `a` of the last build into `r` (device rows, lists, host state).
+// Copies assembly `a` of the last build into `r` (device rows, lists, host
+// state), reusing r's buffers when they are large enough.
 rbg_status save_resident(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_params* P,
                          Resident* r);
 // Device bytes a build of these inputs needs for its batch buffers (an

@@ -248,7 +248,7 @@ void sv_tables(uint32_t m, double b0, std::vector<double>& svd, std::vector<floa
 }
 
 void depth_features(const float* depth, size_t n, size_t S, double min_cv, float* ranked,
-                    float* unit, float* raw, double* colnorm, uint8_t* anynz) {
+                    float* unit, float* raw, double* colnorm, uint8_t* anynz, bool parallel) {
   std::vector<float> rk(depth, depth + n * S);
   if (S > 1) {
     if (raw) std::memcpy(raw, depth, n * S * sizeof(float));
@@ -260,7 +260,7 @@ void depth_features(const float* depth, size_t n, size_t S, double min_cv, float
         colnorm[c] = (mean > 1e-12) ? 1.0 / mean : 0.0;
       }
     // average ranks, ties share the mean position (src/ranker.h:55-66)
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (parallel)
     for (size_t r = 0; r < n; ++r) {
       float* row = &rk[r * S];
       std::vector<uint32_t> order(S);
@@ -283,7 +283,7 @@ void depth_features(const float* depth, size_t n, size_t S, double min_cv, float
   if (unit) {
     std::fill(unit, unit + n * S, 0.0f);
     if (S > 1) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (parallel)
       for (size_t r = 0; r < n; ++r) {
         const float* row = &rk[r * S];
         double mean = 0.0;

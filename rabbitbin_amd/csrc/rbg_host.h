@@ -43,9 +43,10 @@ const ProposalTables& proposal_tables(uint32_t m, uint64_t seed);
 void sv_tables(uint32_t m, double b0, std::vector<double>& svd, std::vector<float>& svf,
                std::vector<uint16_t>& level);
 
-// Depth features (src/rabbitbin.cpp:4784-4988).
+// Depth features (src/rabbitbin.cpp:4784-4988).  `parallel`: rows on the
+// host's OpenMP threads (off when the caller already runs one call per thread).
 void depth_features(const float* depth, size_t n, size_t S, double min_cv, float* ranked,
-                    float* unit, float* raw, double* colnorm, uint8_t* anynz);
+                    float* unit, float* raw, double* colnorm, uint8_t* anynz, bool parallel = true);
 
 // int8 image of the unit rows for the MFMA prefilter of the abundance gate.
 // q = round(u * scale) with scale = 127 / max|u| (zero padded to kpad).  The

@@ -69,8 +69,10 @@ hipError_t launch_calibrate(const uint32_t* rowmax, const AsmDesc* desc, uint32_
                             const uint32_t* members, const uint32_t* round_off, double coverage,
                             BuildScalars* bs, hipStream_t st);
 
-// ---- stage (b), regime 1: n x n counts per assembly
-// Work items (assembly, column block of 256, row block, rows per block).
+// ---- stage (b), regime 1: n x n counts per assembly (each unordered pair
+// computed once and written at both places; the diagonal is not written)
+// Work items (assembly, column block of 256, row block, rows per block), only
+// blocks holding some pair r < j.
 void dense_work_items(const std::vector<AsmDesc>& desc, uint32_t rows_per_item,
                       std::vector<uint32_t>& items);
 hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, const AsmDesc* desc,
@@ -132,11 +134,12 @@ hipError_t launch_select(const uint64_t* list, const uint64_t* off, const uint16
 inline uint32_t fence_words(uint32_t K) { return 16u * (((K + 15u) / 16u + 15u) / 16u); }
 // Edge emission: pass 0 marks kept top-K entries (keep: nrows x ceil(K/64)
 // u64) and counts them per row; pass 1 writes them at the rows' offsets.
+// Rows are visited in `order` (a permutation of the rows; identity when null).
 hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
                             const uint32_t* topk_len, const uint32_t* fence, uint32_t FW,
                             uint32_t nrows, uint32_t K, const BuildScalars* bs,
-                            const AsmDesc* desc, const uint16_t* asm_of, uint64_t* keep,
-                            uint32_t* row_edges, hipStream_t st);
+                            const AsmDesc* desc, const uint16_t* asm_of, const uint32_t* order,
+                            uint64_t* keep, uint32_t* row_edges, hipStream_t st);
 hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
                              const uint32_t* topk_len, uint32_t nrows, uint32_t K,
                              const uint64_t* keep, const uint64_t* row_off, uint32_t* efrom,

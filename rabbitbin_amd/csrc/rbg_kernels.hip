@@ -104,11 +104,14 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
 // after rank r has fl((1/w_x) * T_x) >= fl(wmin * T_r) (both factors are no
 // smaller, and rounding is monotone), so once fl(wmin * T_r) > best no later
 // element can win or tie and the walk stops: typically after a few ranks.
+// Ranks are loaded and applied in groups of eight with the stop tested per
+// group (applying elements past the stop changes nothing).
 //
 // The sequence buffer must be readable 16 bytes past its last contig (the
 // last chunk of a contig is read whole).
 // ---------------------------------------------------------------------------
 constexpr int kSketchWaves = 4;
+constexpr int kRankGroup = 8;  // proposal ranks loaded together in phase 2
 
 // Bit 7 of each byte set iff that byte of x is non-zero.
 __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
@@ -318,33 +321,37 @@ __global__ __launch_bounds__(256) void sketch_kernel(
       bx[q] = -1;
     }
     if (k < m) {
-      double t = sT[k];
-      uint32_t x = sX[k];
-      for (int r = 0; r < kElemCount; ++r) {
-        bool done = true;
+      // Ranks in groups of kRankGroup, loaded together: applying an element
+      // past the break point changes nothing (it can neither win nor tie), so
+      // the break is tested once per group, on the group's last rank.
+      for (int r0 = 0; r0 < kElemCount; r0 += kRankGroup) {
+        double t[kRankGroup];
+        uint32_t x[kRankGroup];
 #pragma unroll
-        for (int q = 0; q < CPB; ++q)  // wmin = +inf: contig without elements
-          done = done && (wmin[q] == __builtin_huge_val() || wmin[q] * t > best[q]);
-        if (done) break;
-        double tn = 0.0;
-        uint32_t xn = 0;
-        if (r + 1 < kElemCount) {  // next rank in flight while this one is applied
-          tn = sT[(size_t)(r + 1) * m + k];
-          xn = sX[(size_t)(r + 1) * m + k];
+        for (int g = 0; g < kRankGroup; ++g) {
+          const int r = min(r0 + g, kElemCount - 1);  // past the last rank: repeat it (harmless)
+          t[g] = sT[(size_t)r * m + k];
+          x[g] = sX[(size_t)r * m + k];
         }
 #pragma unroll
-        for (int q = 0; q < CPB; ++q) {
-          const double wv = winv[q][x];
-          if (wv != 0.0) {
-            const double hv = wv * t;
-            if (hv < best[q] || (hv == best[q] && (int)x < bx[q])) {
-              best[q] = hv;
-              bx[q] = (int)x;
+        for (int g = 0; g < kRankGroup; ++g) {
+#pragma unroll
+          for (int q = 0; q < CPB; ++q) {
+            const double wv = winv[q][x[g]];
+            if (wv != 0.0) {
+              const double hv = wv * t[g];
+              if (hv < best[q] || (hv == best[q] && (int)x[g] < bx[q])) {
+                best[q] = hv;
+                bx[q] = (int)x[g];
+              }
             }
           }
         }
-        t = tn;
-        x = xn;
+        bool done = true;
+#pragma unroll
+        for (int q = 0; q < CPB; ++q)  // wmin = +inf: contig without elements
+          done = done && (wmin[q] == __builtin_huge_val() || wmin[q] * t[kRankGroup - 1] > best[q]);
+        if (done) break;
       }
     }
     const uint32_t w0 = k / 32u;  // lanes 0..31 -> w0, lanes 32..63 -> w0+1
@@ -391,13 +398,17 @@ __global__ __launch_bounds__(256) void win16_counts_kernel(const uint16_t* __res
 }
 
 // ---------------------------------------------------------------------------
-// Small regime (n <= 25000): every ordered pair's count into an n x n u16
-// matrix.  Thread = column j with its bit planes in VGPRs; rows are uniform
-// across the wave and come in through scalar loads.
+// Small regime (n <= 25000): every pair's count into an n x n u16 matrix,
+// computed once per unordered pair.  Thread = column j with its bit planes in
+// VGPRs; rows r < j are uniform across the wave and come in through scalar
+// loads.  The count is symmetric (a register matches only where both sketches
+// set it), so (r, j) is written at [r][j] (a coalesced row segment) and at
+// [j][r] (the thread's own row segment, filled over the item's rows).  The
+// diagonal is neither written nor read.
 // ---------------------------------------------------------------------------
 // Work item (assembly a, column block cb, row block rb, rows): thread = local
-// column 256*cb + t of assembly a against its rows [rb*rows, +rows); the
-// counts go to the assembly's n x n matrix (local indices).
+// column 256*cb + t of assembly a against its rows [rb*rows, +rows), pairs
+// r < j only; items without such a pair are not generated.
 template <int W>
 __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __restrict__ planes,
                                                            const AsmDesc* __restrict__ desc,
@@ -408,8 +419,9 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
   const uint32_t* __restrict__ rows = planes + (size_t)desc[it.x].row0 * 8 * W;
   uint16_t* __restrict__ out = mat + desc[it.x].mat0;
   const uint32_t j = it.y * 256 + threadIdx.x;
+  const uint32_t jlast = min(n, it.y * 256 + 256) - 1;  // the block's last column
   const uint32_t r0 = it.z * it.w;
-  const uint32_t r1 = min(n, r0 + it.w);
+  const uint32_t r1 = min(min(n, r0 + it.w), jlast);    // rows with some column j > r
   uint32_t b[8 * W];
   const uint32_t jj = j < n ? j : n - 1;
 #pragma unroll
@@ -419,7 +431,11 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
     uint32_t mism = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) mism += __popc(word_mismatch(a + 8 * w, b + 8 * w));
-    if (j < n) out[(size_t)r * n + j] = (uint16_t)(32u * W - mism);
+    if (j < n && j > r) {
+      const uint16_t c = (uint16_t)(32u * W - mism);
+      out[(size_t)r * n + j] = c;
+      out[(size_t)j * n + r] = c;
+    }
   }
 }
 
@@ -524,28 +540,32 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
     bn = 0;
   };
 
-  const uint32_t total = ntile * 16;
 #pragma unroll 1
-  for (uint32_t u = 0; u < total; ++u) {
-    const uint32_t t = u >> 4, s = u & 15u;
-    if (s == 0) {
-      // tile t's operands (prefetched) go to LDS; then prefetch tile t+1
+  for (uint32_t t = 0; t < ntile; ++t) {
+    // tile t's operands (prefetched) go to LDS; then prefetch tile t+1
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t e = threadIdx.x + 256u * k;
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t e = threadIdx.x + 256u * k;
 #pragma unroll
-        for (int c = 0; c < KC; ++c) bops[t & 1][e >> 6][c][e & 63u] = nb[k][c];
-      }
-      __syncthreads();
-      if (t + 1 < ntile) load_tile_ops<KC>(q, jbase + 256u * (t + 1), n, nb);
+      for (int c = 0; c < KC; ++c) bops[t & 1][e >> 6][c][e & 63u] = nb[k][c];
     }
+    __syncthreads();
+    if (t + 1 < ntile) load_tile_ops<KC>(q, jbase + 256u * (t + 1), n, nb);
     const uint32_t jb = jbase + 256u * t;
-    const uint32_t j0 = jb + 16u * s;
     const bool diag = ib == jb;
-    if (j0 < n && !(diag && j0 + 15 <= row0)) {  // skip blocks with every j <= i
+    long Bn[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) Bn[c] = bops[t & 1][0][c][ln];
+#pragma unroll 1
+    for (uint32_t s = 0; s < 16; ++s) {
       long B[KC];
 #pragma unroll
-      for (int c = 0; c < KC; ++c) B[c] = bops[t & 1][s][c][ln];
+      for (int c = 0; c < KC; ++c) {
+        B[c] = Bn[c];
+        if (s + 1 < 16) Bn[c] = bops[t & 1][s + 1][c][ln];  // the next step's operand in flight
+      }
+      const uint32_t j0 = jb + 16u * s;
+      if (j0 >= n || (diag && j0 + 15 <= row0)) continue;  // blocks with every j <= i
       v4i32 acc[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -554,28 +574,29 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
         for (int c = 0; c < KC; ++c)
           acc[r] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A[r][c], B[c], acc[r], 0, 0, 0);
       }
-      int32_t mr[4];
+      int32_t mx = acc[0][0];  // the largest of the lane's 16 dots (a chain of max3)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) mr[r] = max(max(acc[r][0], acc[r][1]), max(acc[r][2], acc[r][3]));
-      const int32_t mx = max(max(mr[0], mr[1]), max(mr[2], mr[3]));
-      if (__ballot(mx >= qcut) != 0) {  // some pair of these 1024 is near the gate
-        const uint32_t j = j0 + (ln & 15u);
-        const uint32_t jl = j - jbase;
-        const bool full = !diag && jb + 256 <= n && jb >= jmin;
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (__ballot(mr[r] >= qcut) == 0) continue;  // only the 16-row block with candidates
-          if (bn > kGateBuf - 256) flush();
+        for (int v = (r == 0 ? 1 : 0); v < 4; ++v) mx = max(mx, acc[r][v]);
+      if (__ballot(mx >= qcut) == 0) continue;  // no pair of these 1024 is near the gate
+      const uint32_t j = j0 + (ln & 15u);
+      const uint32_t jl = j - jbase;
+      const bool full = !diag && jb + 256 <= n && jb >= jmin;
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const uint32_t i = row0 + 16 * r + 4 * (ln >> 4) + v;
-            const bool c = full ? acc[r][v] >= qcut
-                                : (acc[r][v] >= qcut && i < j && j < n && j >= jmin);
-            const uint64_t mask = __ballot(c);
-            if (mask) {
-              if (c) buf[wv][bn + lanes_below(mask)] = ((i - ib) << 12) | jl;
-              bn += (uint32_t)__popcll(mask);
-            }
+      for (int r = 0; r < 4; ++r) {
+        const int32_t mr = max(max(acc[r][0], acc[r][1]), max(acc[r][2], acc[r][3]));
+        if (__ballot(mr >= qcut) == 0) continue;  // only the 16-row blocks with candidates
+        if (bn > kGateBuf - 256) flush();
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const uint32_t i = row0 + 16 * r + 4 * (ln >> 4) + v;
+          const bool c = full ? acc[r][v] >= qcut
+                              : (acc[r][v] >= qcut && i < j && j < n && j >= jmin);
+          const uint64_t mask = __ballot(c);
+          if (mask) {
+            if (c) buf[wv][bn + lanes_below(mask)] = ((i - ib) << 12) | jl;
+            bn += (uint32_t)__popcll(mask);
           }
         }
       }
@@ -632,21 +653,26 @@ __device__ __forceinline__ void unpack_pair(uint64_t e, uint32_t& i, uint32_t& j
 // then acc0+acc1 and GCC's _mm512_reduce_add_ps tree
 // (l, l+8) -> (l, l+4) -> (0,2),(1,3) -> final.
 //
-// Four GPU lanes take one candidate.  Lane q of the group holds accumulator
-// lanes 4q..4q+3 (float4 q of every 16-float chunk of both rows); the first
+// Eight GPU lanes take one candidate.  For the gate, lane q uses q4 = q & 3
+// and holds accumulator lanes 4q4..4q4+3 (float4 q4 of every 16-float chunk
+// of both rows; lanes q and q + 4 compute the same values); the first
 // reduction level pairs lanes q and q^2, the second q and q^1, the last runs
-// in lane 0 of the group (IEEE addition is commutative, so which lane holds
-// which operand does not change a sum).  Lane q also holds word groups
-// [q*W/4, (q+1)*W/4) of both rows' planes (mismatch counts meet in the same
-// shuffles).  All loads are issued before the gate is known: nearly every
-// candidate survives.  Survivors add 1 to deg[i] and deg[j].
+// in every lane (IEEE addition is commutative, so which lane holds which
+// operand does not change a sum).  For the register compare, lane q holds the
+// 16-byte pieces q, q + 8, ... of both rows' planes, so each load instruction
+// reads one whole 128-byte line per candidate; a piece is half a word group
+// (planes 0-3 in even lanes, 4-7 in odd lanes), and lanes q and q^1 combine
+// the halves: the AND over all eight planes of b (the unset-register mask)
+// and the OR of the eight mismatch words.  All loads are issued before the
+// gate is known: nearly every candidate survives.  Survivors add 1 to deg[i]
+// and deg[j].
 template <int NC, int W>
 __device__ __forceinline__ void gate_compare_one(
     uint64_t* __restrict__ cand, uint64_t t, uint64_t nc, uint32_t q,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
     const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
-  constexpr int WPL = W >= 4 ? W / 4 : 1;  // word groups per lane
+  constexpr int PPL = (2 * W + 7) / 8;  // 16-byte plane pieces per lane and row
   const bool live = t < nc;
   const uint64_t e = live ? cand[t] : 0;
   // Row indices are taken as 64-bit fields and masked in 64 bits: with the
@@ -654,29 +680,31 @@ __device__ __forceinline__ void gate_compare_one(
   // address of some instantiations while keeping it elsewhere.
   const uint64_t i64 = e >> 40, j64 = (e >> 16) & 0xFFFFFFull;
   const uint32_t i = (uint32_t)i64, j = (uint32_t)j64;
-  const bool has_words = W >= 4 || q < (uint32_t)W;
+  const uint32_t q4 = q & 3u;
   const AsmDesc& ad = desc[live ? asm_of[i] : 0];
   const float thr = ad.thr;
   const uint32_t acc1_end = ad.acc1_end;
   float4 ua[NC], ub[NC];
-  uint4 pa[2 * WPL], pb[2 * WPL];
+  uint4 pa[PPL], pb[PPL];
   if (live) {
     const float4* ra = reinterpret_cast<const float4*>(unit + i64 * 16 * NC);
     const float4* rb = reinterpret_cast<const float4*>(unit + j64 * 16 * NC);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      ua[c] = ra[4 * c + q];
-      ub[c] = rb[4 * c + q];
+      ua[c] = ra[4 * c + q4];
+      ub[c] = rb[4 * c + q4];
     }
-    if (has_words) {
-      const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W + 8u * WPL * q);
-      const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W + 8u * WPL * q);
+    const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W);
+    const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W);
 #pragma unroll
-      for (int x = 0; x < 2 * WPL; ++x) {
-        pa[x] = wa[x];
-        pb[x] = wb[x];
-      }
+    for (int x = 0; x < PPL; ++x) {
+      const uint32_t u = 8u * x + q;
+      pa[x] = u < 2u * W ? wa[u] : make_uint4(0, 0, 0, 0);
+      pb[x] = u < 2u * W ? wb[u] : make_uint4(0, 0, 0, 0);
     }
+  } else {
+#pragma unroll
+    for (int x = 0; x < PPL; ++x) pa[x] = pb[x] = make_uint4(0, 0, 0, 0);
   }
   // gate value
   float acc0[4] = {0.f, 0.f, 0.f, 0.f}, acc1[4] = {0.f, 0.f, 0.f, 0.f};
@@ -702,29 +730,31 @@ __device__ __forceinline__ void gate_compare_one(
   }
   float t8[4], t4[4];
 #pragma unroll
-  for (int l = 0; l < 4; ++l) t8[l] = __shfl_xor(acc0[l], 2) + acc0[l];  // lanes 0,1: l+8 with l
+  for (int l = 0; l < 4; ++l) t8[l] = __shfl_xor(acc0[l], 2) + acc0[l];  // l+8 with l
 #pragma unroll
-  for (int l = 0; l < 4; ++l) t4[l] = __shfl_xor(t8[l], 1) + t8[l];  // lane 0: l+4 with l
+  for (int l = 0; l < 4; ++l) t4[l] = __shfl_xor(t8[l], 1) + t8[l];  // l+4 with l
   const float dot = (t4[0] + t4[2]) + (t4[1] + t4[3]);
   const bool nz = !anynz || (live && (anynz[i] || anynz[j]));
   bool pass = live && nz && !(dot < thr);
-  pass = __shfl(pass, (int)(lane_id() & ~3u));  // the group's lane 0 decides
-  // register compare
+  pass = __shfl(pass, (int)(lane_id() & ~7u));  // the group's lane 0 decides
+  // register compare (every lane takes part in the pair exchanges; pieces
+  // past the row are zero and count nothing)
   uint32_t mism = 0;
-  if (pass && has_words) {
+  const bool half0 = (q & 1u) == 0;
 #pragma unroll
-    for (int w = 0; w < WPL; ++w) {
-      const uint32_t av[8] = {pa[2 * w].x, pa[2 * w].y, pa[2 * w].z, pa[2 * w].w,
-                              pa[2 * w + 1].x, pa[2 * w + 1].y, pa[2 * w + 1].z, pa[2 * w + 1].w};
-      const uint32_t bs[8] = {pb[2 * w].x, pb[2 * w].y, pb[2 * w].z, pb[2 * w].w,
-                              pb[2 * w + 1].x, pb[2 * w + 1].y, pb[2 * w + 1].z, pb[2 * w + 1].w};
-      uint32_t bv[8];
-      column_word(bs, bv);
-      mism += __popc(word_mismatch(av, bv));
-    }
+  for (int x = 0; x < PPL; ++x) {
+    const uint32_t all_half = pb[x].x & pb[x].y & pb[x].z & pb[x].w;
+    const uint32_t unset = all_half & __shfl_xor(all_half, 1);  // all eight planes set
+    const uint32_t b0 = half0 ? pb[x].x & ~unset : pb[x].x;       // plane 0 sits in the even lane
+    const uint32_t mm_half = (pa[x].x ^ b0) | (pa[x].y ^ pb[x].y) | (pa[x].z ^ pb[x].z) |
+                             (pa[x].w ^ pb[x].w);
+    const uint32_t mm = mm_half | __shfl_xor(mm_half, 1);
+    if (half0 && 8u * x + q < 2u * W) mism += __popc(mm);
   }
+  if (!pass) mism = 0;
   mism += __shfl_xor(mism, 1);
   mism += __shfl_xor(mism, 2);
+  mism += __shfl_xor(mism, 4);
   const bool add = live && q == 0 && pass;
   wave_atomic_slot(deg, i, add);
   wave_atomic_slot(deg, j, add);
@@ -753,10 +783,10 @@ __global__ __launch_bounds__(256) void gate_compare_kernel(
   const uint64_t nc = min((uint64_t)*ncand, cap);
   uint64_t s0, s1;
   xcd_segment(nc, s0, s1);
-  const uint32_t q = threadIdx.x & 3u;
-  // wave-uniform loop over the segment: 16 candidates per wave, 64 per workgroup per step
-  for (uint64_t tw = s0 + ((threadIdx.x & ~63u) >> 2); tw < s1; tw += 64)
-    gate_compare_one<NC, W>(cand, tw + ((threadIdx.x & 63u) >> 2), s1, q, unit, anynz, desc,
+  const uint32_t q = threadIdx.x & 7u;
+  // wave-uniform loop over the segment: 8 candidates per wave, 32 per workgroup per step
+  for (uint64_t tw = s0 + ((threadIdx.x & ~63u) >> 3); tw < s1; tw += 32)
+    gate_compare_one<NC, W>(cand, tw + ((threadIdx.x & 63u) >> 3), s1, q, unit, anynz, desc,
                             asm_of, planes, deg);
 }
 
@@ -1085,20 +1115,35 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
   if (tid == 0) {
     nsel = 0;
     rmax = 0;
+    tot = 0;
   }
   __syncthreads();
-  uint32_t my_max = 0;
+  // Level histogram.  Level 0 (similarity clamped to 0) holds about half of a
+  // long row: it is counted in registers; the other levels go to one LDS
+  // histogram per wave (1024 bins each), summed after.
+  uint32_t my_max = 0, zeros = 0;
+  uint32_t* __restrict__ whist = hist + 1024 * (tid >> 6);
   for (uint32_t t = tid; t < len; t += 256) {
     uint32_t id, c;
     bool fm;
     const bool ok = row_entry(src, i, t, id, c, fm);
     if (fm && c > my_max) my_max = c;
-    if (ok) atomicAdd(&hist[level[c]], 1u);
+    if (ok) {
+      const uint32_t lv = level[c];
+      if (lv == 0)
+        ++zeros;
+      else
+        atomicAdd(&whist[lv], 1u);
+    }
   }
   if (my_max) atomicMax(&rmax, my_max);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) zeros += __shfl_xor(zeros, o);
+  if ((tid & 63u) == 0 && zeros) atomicAdd(&hist[3072], zeros);  // every wave's bin 0 is empty
   __syncthreads();
+  for (uint32_t b = tid; b < nlevels; b += 256)  // bins < 1024: each read and written by one thread
+    hist[b] = hist[b] + hist[1024 + b] + hist[2048 + b] + hist[3072 + b];
   // threshold level L*: the largest level whose suffix count reaches K
-  if (tid == 0) tot = 0;
   __syncthreads();
   {
     uint32_t part = 0;
@@ -1215,12 +1260,15 @@ __global__ __launch_bounds__(256) void emit_mark_kernel(
     const uint32_t* __restrict__ topk_id, const uint16_t* __restrict__ topk_cnt,
     const uint32_t* __restrict__ topk_len, const uint32_t* __restrict__ fence, uint32_t FW,
     uint32_t n, uint32_t K, const BuildScalars* __restrict__ bs, const AsmDesc* __restrict__ desc,
-    const uint16_t* __restrict__ asm_of, uint32_t KW, uint64_t* __restrict__ keep,
-    uint32_t* __restrict__ row_edges) {
+    const uint16_t* __restrict__ asm_of, const uint32_t* __restrict__ order, uint32_t KW,
+    uint64_t* __restrict__ keep, uint32_t* __restrict__ row_edges) {
   const bool aligned16 = (K & 3u) == 0;
-  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t pos = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t ln = threadIdx.x & 63u;
-  if (i >= n) return;
+  if (pos >= n) return;
+  // rows in pair-pass order: rows of one abundance cluster, whose partners
+  // overlap, are tested together
+  const uint32_t i = order ? order[pos] : pos;
   const uint32_t a = asm_of[i];
   const uint32_t cmin = bs[a].cmin;
   const bool mutual = desc[a].regime == 2;
@@ -1372,81 +1420,121 @@ __global__ __launch_bounds__(256) void edge_weight_kernel(
 
 // The same weight for the edges of a batch, each with its assembly's S
 // (S <= 16*NC), w_comp and colnorm; unit and raw rows are zero-padded to
-// 16*NC floats (128-byte aligned rows read as float4).  Per edge the FP64
-// sums run in the same order as above, so the results are identical.
+// 16*NC floats (128-byte aligned rows).  Per edge the FP64 sums run in the
+// same order as above, so the results are identical.
+//
+// A wave takes 64 consecutive edges, one per lane.  Edges are sorted by
+// `from`, so the lanes' `from` rows mostly coincide and are read directly; the
+// `to` rows are scattered, so the wave loads them together, 4*NC lanes per
+// row (each load instruction reads whole rows), into an LDS area with an odd
+// row stride (the lanes' reads of their own rows then fall in distinct banks).
+constexpr uint32_t kEwWaves = 4;
 template <int NC>
-__global__ __launch_bounds__(256) void edge_weight_pad_kernel(
+__device__ __forceinline__ void stage_rows(const float* __restrict__ base, uint32_t j, uint32_t ln,
+                                           float* __restrict__ area) {
+  constexpr int P = 4 * NC;          // float4 per row
+  constexpr int RPI = 64 / P;        // rows per load instruction
+  constexpr int STR = 16 * NC + 1;   // staged floats per row
+  const uint32_t g = ln / P, q = ln % P;
+#pragma unroll
+  for (int r0 = 0; r0 < 64; r0 += RPI) {
+    const uint32_t r = r0 + g;
+    const uint32_t jr = (uint32_t)__shfl((int)j, (int)(r < 64 ? r : 0));
+    if (g < (uint32_t)RPI && r < 64) {
+      const float4 v = reinterpret_cast<const float4*>(base + (size_t)jr * 16 * NC)[q];
+      float* dst = area + r * STR + 4 * q;
+      dst[0] = v.x;
+      dst[1] = v.y;
+      dst[2] = v.z;
+      dst[3] = v.w;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();  // rows written by other lanes (LDS keeps a wave's order)
+}
+
+template <int NC>
+__global__ __launch_bounds__(64 * kEwWaves) void edge_weight_pad_kernel(
     const uint32_t* __restrict__ efrom, const uint32_t* __restrict__ eto,
     const float* __restrict__ scomp, uint64_t cap, const uint64_t* __restrict__ ne_dev,
     const float* __restrict__ unitp, const float* __restrict__ rawp,
     const double* __restrict__ colnorm_all, const AsmDesc* __restrict__ desc,
     const uint16_t* __restrict__ asm_of, double min_edge, double neg_thr,
     float* __restrict__ score) {
-  constexpr int V = 4 * NC;  // float4 per row
+  constexpr int V = 4 * NC;          // float4 per row
+  constexpr int STR = 16 * NC + 1;
+  __shared__ float stage[kEwWaves][64 * STR];
+  const uint32_t ln = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  float* __restrict__ area = stage[wv];
+  const float* __restrict__ mine = area + ln * STR;
   const uint64_t cnt = min(*ne_dev, cap);
-  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt;
-       e += (uint64_t)gridDim.x * 256) {
-  const uint32_t i = efrom[e], j = eto[e];
-  const AsmDesc& ad = desc[asm_of[i]];
-  const uint32_t S = ad.S;
-  const double w_comp = ad.w_comp;
-  if (S <= 1) {  // no depth: the composition similarity alone
-    const double sc = (double)scomp[e];
-    score[e] = (float)(S == 0 && sc < min_edge ? 0.0 : sc);
-    continue;
-  }
-  const double* __restrict__ colnorm = colnorm_all + ad.colnorm0;
-  const float4* ua = reinterpret_cast<const float4*>(unitp + (size_t)i * 16 * NC);
-  const float4* ub = reinterpret_cast<const float4*>(unitp + (size_t)j * 16 * NC);
-  float a[4 * V], b[4 * V];
+  const uint64_t wstride = (uint64_t)gridDim.x * 64 * kEwWaves;
+  for (uint64_t e0 = ((uint64_t)blockIdx.x * kEwWaves + wv) * 64; e0 < cnt; e0 += wstride) {  // wave-uniform
+    const uint64_t e = e0 + ln;
+    const bool live = e < cnt;
+    const uint32_t i = live ? efrom[e] : 0, j = live ? eto[e] : 0;
+    const AsmDesc& ad = desc[asm_of[i]];
+    const uint32_t S = ad.S;
+    const double w_comp = ad.w_comp;
+    // unit rows: own `from` row in registers, the `to` rows staged
+    stage_rows<NC>(unitp, j, ln, area);
+    float a[4 * V];
+    {
+      const float4* ua = reinterpret_cast<const float4*>(unitp + (size_t)i * 16 * NC);
 #pragma unroll
-  for (int v = 0; v < V; ++v) {
-    const float4 x = ua[v], y = ub[v];
-    a[4 * v] = x.x, a[4 * v + 1] = x.y, a[4 * v + 2] = x.z, a[4 * v + 3] = x.w;
-    b[4 * v] = y.x, b[4 * v + 1] = y.y, b[4 * v + 2] = y.z, b[4 * v + 3] = y.w;
-  }
-  double corr = 0.0;
-#pragma unroll
-  for (int k = 0; k < 4 * V; ++k)
-    if ((uint32_t)k < S) corr += (double)a[k] * (double)b[k];
-  double w = 0.0;
-  const bool finite = corr == corr && corr != __builtin_huge_val() && corr != -__builtin_huge_val();
-  if (finite && !(neg_thr > -0.99 && corr < neg_thr)) {
-    const double cp = corr < 0.0 ? 0.0 : corr;
-    const float4* ra = reinterpret_cast<const float4*>(rawp + (size_t)i * 16 * NC);
-    const float4* rb = reinterpret_cast<const float4*>(rawp + (size_t)j * 16 * NC);
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const float4 x = ra[v], y = rb[v];
-      a[4 * v] = x.x, a[4 * v + 1] = x.y, a[4 * v + 2] = x.z, a[4 * v + 3] = x.w;
-      b[4 * v] = y.x, b[4 * v + 1] = y.y, b[4 * v + 2] = y.z, b[4 * v + 3] = y.w;
-    }
-    double mn = 0.0, mx = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4 * V; ++k) {
-      if ((uint32_t)k < S) {
-        double x = a[k], y = b[k];
-        if (colnorm) {
-          x *= colnorm[k];
-          y *= colnorm[k];
-        }
-        if (x < y) {
-          mn += x;
-          mx += y;
-        } else {
-          mn += y;
-          mx += x;
-        }
+      for (int v = 0; v < V; ++v) {
+        const float4 x = ua[v];
+        a[4 * v] = x.x, a[4 * v + 1] = x.y, a[4 * v + 2] = x.z, a[4 * v + 3] = x.w;
       }
     }
-    double wj = mx <= 0.0 ? 0.0 : mn / mx;
-    if (!(wj == wj)) wj = 0.0;
-    wj = wj < 0.0 ? 0.0 : (wj > 1.0 ? 1.0 : wj);
-    const double d = cp < wj ? cp : wj;
-    w = w_comp * (double)scomp[e] + (1.0 - w_comp) * d;
-    if (!(w == w) || w < min_edge) w = 0.0;
-  }
-  score[e] = (float)w;
+    double corr = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4 * V; ++k)
+      if ((uint32_t)k < S) corr += (double)a[k] * (double)mine[k];
+    const bool finite = corr == corr && corr != __builtin_huge_val() && corr != -__builtin_huge_val();
+    const bool depth = live && S > 1 && finite && !(neg_thr > -0.99 && corr < neg_thr);
+    __builtin_amdgcn_wave_barrier();  // every lane has read its staged unit row
+    const bool any_depth = __ballot(depth) != 0;
+    if (any_depth) stage_rows<NC>(rawp, j, ln, area);  // wave-uniform
+    double w = 0.0;
+    if (depth) {
+      const double cp = corr < 0.0 ? 0.0 : corr;
+      const double* __restrict__ colnorm = colnorm_all + ad.colnorm0;
+      const float4* ra = reinterpret_cast<const float4*>(rawp + (size_t)i * 16 * NC);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const float4 x = ra[v];
+        a[4 * v] = x.x, a[4 * v + 1] = x.y, a[4 * v + 2] = x.z, a[4 * v + 3] = x.w;
+      }
+      double mn = 0.0, mx = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4 * V; ++k) {
+        if ((uint32_t)k < S) {
+          double x = a[k], y = mine[k];
+          if (colnorm) {
+            x *= colnorm[k];
+            y *= colnorm[k];
+          }
+          if (x < y) {
+            mn += x;
+            mx += y;
+          } else {
+            mn += y;
+            mx += x;
+          }
+        }
+      }
+      double wj = mx <= 0.0 ? 0.0 : mn / mx;
+      if (!(wj == wj)) wj = 0.0;
+      wj = wj < 0.0 ? 0.0 : (wj > 1.0 ? 1.0 : wj);
+      const double d = cp < wj ? cp : wj;
+      w = w_comp * (double)scomp[e] + (1.0 - w_comp) * d;
+      if (!(w == w) || w < min_edge) w = 0.0;
+    } else if (live && S <= 1) {  // no depth: the composition similarity alone
+      const double sc = (double)scomp[e];
+      w = S == 0 && sc < min_edge ? 0.0 : sc;
+    }
+    if (live) score[e] = (float)w;
+    __builtin_amdgcn_wave_barrier();  // the staged rows are read before the next edges' load
   }
 }
 
@@ -1786,6 +1874,7 @@ void dense_work_items(const std::vector<AsmDesc>& desc, uint32_t rows_per_item,
     const uint32_t n = desc[a].n;
     for (uint32_t rb = 0; rb * rows_per_item < n; ++rb)
       for (uint32_t cb = 0; cb * 256 < n; ++cb) {
+        if (std::min(n, cb * 256 + 256) - 1 <= rb * rows_per_item) continue;  // no pair r < j
         items.push_back(a);
         items.push_back(cb);
         items.push_back(rb);
@@ -1846,7 +1935,7 @@ hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long
                                const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st) {
   if (cap == 0) return hipSuccess;
   // a multiple of the XCD count (xcd_segment)
-  const dim3 grid((unsigned)std::max<uint64_t>(kXcds, std::min<uint64_t>((cap * 4 + 255) / 256, 65536) & ~7ull));
+  const dim3 grid((unsigned)std::max<uint64_t>(kXcds, std::min<uint64_t>((cap * 8 + 255) / 256, 65536) & ~7ull));
 #define RBG_GC(NCV, WV)                                                                            \
   hipLaunchKernelGGL((gate_compare_kernel<NCV, WV>), grid, dim3(256), 0, st, cand, cap, ncand, unit, \
                      anynz, desc, asm_of, planes, deg)
@@ -1916,11 +2005,12 @@ hipError_t launch_select(const uint64_t* list, const uint64_t* off, const uint16
 hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
                             const uint32_t* topk_len, const uint32_t* fence, uint32_t FW,
                             uint32_t n, uint32_t K, const BuildScalars* bs,
-                            const AsmDesc* desc, const uint16_t* asm_of, uint64_t* keep,
-                            uint32_t* row_edges, hipStream_t st) {
+                            const AsmDesc* desc, const uint16_t* asm_of, const uint32_t* order,
+                            uint64_t* keep, uint32_t* row_edges, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(emit_mark_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt,
-                     topk_len, fence, FW, n, K, bs, desc, asm_of, (K + 63) / 64, keep, row_edges);
+                     topk_len, fence, FW, n, K, bs, desc, asm_of, order, (K + 63) / 64, keep,
+                     row_edges);
   return hipGetLastError();
 }
 
@@ -1941,10 +2031,10 @@ hipError_t launch_edge_weights_batch(const uint32_t* efrom, const uint32_t* eto,
                                      const uint16_t* asm_of, double min_edge, double neg_thr,
                                      float* score, hipStream_t st) {
   if (cap == 0) return hipSuccess;
-  const dim3 grid((unsigned)std::min<uint64_t>((cap + 255) / 256, 65536));
-#define RBG_EW(NCV)                                                                                \
-  hipLaunchKernelGGL(edge_weight_pad_kernel<NCV>, grid, dim3(256), 0, st, efrom, eto, scomp, cap, ne, \
-                     unit, raw, colnorm, desc, asm_of, min_edge, neg_thr, score)
+  const dim3 grid((unsigned)std::min<uint64_t>((cap + 64 * kEwWaves - 1) / (64 * kEwWaves), 65536));
+#define RBG_EW(NCV)                                                                                   \
+  hipLaunchKernelGGL(edge_weight_pad_kernel<NCV>, grid, dim3(64 * kEwWaves), 0, st, efrom, eto, scomp, \
+                     cap, ne, unit, raw, colnorm, desc, asm_of, min_edge, neg_thr, score)
   switch (NC) {
     case 1: RBG_EW(1); break;
     case 2: RBG_EW(2); break;

@@ -8,6 +8,7 @@
 // HBM budget.  A session's sketches, depth rows, int8 rows and candidate
 // lists stay resident after its build, so an extension uploads and sketches
 // only its new contigs and evaluates only the pairs that involve them.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -148,10 +149,9 @@ void rbg_server::select(std::vector<Request*>& step) {
     x.in = r->in;
     x.src = r->extend ? &s->res : nullptr;
     ins.push_back(x);
-    // growth of the session's state; an extended session's state is rebuilt
-    // beside the old copy, which is freed after the step
-    size_t g = (size_t)r->in.n * resident_row_bytes(r->in.S, P);
-    if (r->extend) g += s->res.device_bytes() + (size_t)s->res.n * resident_row_bytes(r->in.S, P);
+    // growth of the session's state (its buffers grow with 25% headroom)
+    const size_t g = (size_t)n * resident_row_bytes(r->in.S, P) * 5 / 4 -
+                     std::min<size_t>(s->res.device_bytes(), (size_t)n * resident_row_bytes(r->in.S, P) * 5 / 4);
     const size_t need = resident + growth + g + estimate_build_bytes(ins.data(), (uint32_t)ins.size(), &P);
     if (need > cfg.hbm_budget) {
       ins.pop_back();
@@ -201,9 +201,12 @@ void rbg_server::run_step(std::vector<Request*>& step) {
   if (resident + ws.device_bytes() > cfg.hbm_budget) ws.release_buffers();
   const double t_start = now();
   rbg_status st = batch_upload(ctx, &ws, ins.data(), A, P.min_cv);
+  const double t_up = now();
   if (st == RBG_OK) st = run_build(ctx, &ws, &P, false, false, nullptr);
+  const double t_built = now();
   std::vector<rbg_status> rs(A, st);
   std::vector<std::string> msg(A, st == RBG_OK ? std::string() : ctx->err);
+  double t_res = 0, t_save = 0;
   if (st == RBG_OK)
     for (uint32_t k = 0; k < A; ++k) {
       Request* r = step[k];
@@ -211,8 +214,12 @@ void rbg_server::run_step(std::vector<Request*>& step) {
       r->info.incremental = h.incr ? 1 : 0;
       r->info.pass_pairs = h.incr ? (uint64_t)h.n * (h.n - 1) / 2 - (uint64_t)h.n_old * (h.n_old - 1) / 2
                                   : (uint64_t)h.n * (h.n - 1) / 2;
+      const double t0 = now();
       rs[k] = assembly_result(ctx, &ws, k, &P, false, &r->result, nullptr);
+      const double t1 = now();
       if (rs[k] == RBG_OK) rs[k] = save_resident(ctx, &ws, k, &P, &ss[k]->res);
+      t_res += t1 - t0;
+      t_save += now() - t1;
       if (rs[k] != RBG_OK) {
         msg[k] = ctx->err;
         rbg_graph_result_free(&r->result);
@@ -222,6 +229,10 @@ void rbg_server::run_step(std::vector<Request*>& step) {
   std::lock_guard<std::mutex> lk(mu);
   stats.steps += 1;
   stats.busy_ms += 1e3 * (t_end - t_start);
+  stats.upload_ms += 1e3 * (t_up - t_start);
+  stats.build_ms += 1e3 * (t_built - t_up);
+  stats.result_ms += 1e3 * t_res;
+  stats.save_ms += 1e3 * t_save;
   for (uint32_t k = 0; k < A; ++k) {
     Request* r = step[k];
     r->info.step = stats.steps;
