@@ -12,6 +12,8 @@
 #include <numeric>
 #include <thread>
 
+#include <sys/mman.h>
+
 using namespace rbg;
 
 namespace rbg {
@@ -132,6 +134,17 @@ rbg_status d2h_chunks(rbg_ctx* ctx, void* dst, const void* src, size_t bytes, ui
 }
 
 }  // namespace
+
+void* alloc_host_result(size_t bytes) {
+  // Large arrays: 2 MiB aligned with transparent huge pages requested, so the
+  // first write faults in few pages.  Released with free().
+  if (bytes < (8u << 20)) return std::malloc(bytes);
+  void* p = nullptr;
+  const size_t huge = 2u << 20;
+  if (posix_memalign(&p, huge, (bytes + huge - 1) / huge * huge) != 0) return nullptr;
+  (void)madvise(p, (bytes + huge - 1) / huge * huge, MADV_HUGEPAGE);
+  return p;
+}
 
 rbg_status h2d_staged(rbg_ctx* ctx, void* dst, const void* src, size_t bytes) {
   hipStream_t st = ctx->stream;
@@ -484,6 +497,7 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
     // A session whose earlier build was fused keeps its gated pairs: only the
     // pairs with a new row are evaluated.
     h.incr = regime == 2 && h.src && h.src->regime == 2 && h.n_old > 0;
+    h.dense_incr = regime == 1 && h.src && h.src->regime == 1 && h.n_old > 0 && h.src->mat.p;
     d.row0 = h.row0;
     d.n = h.n;
     d.S = h.S;
@@ -502,6 +516,7 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
     d.colnorm0 = h.colnorm0;
     if (regime == 1) {
       d.mat0 = mat_total;
+      d.jmin = h.dense_incr ? h.n_old : 0;
       mat_total += (uint64_t)h.n * h.n;
     } else {
       const uint32_t jmin = h.incr ? h.n_old : 0;
@@ -670,6 +685,14 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                           (double*)b->svd.p, (float*)b->svf.p,
                                           (uint16_t*)b->level.p, st));
   RBG_TRY(mark(2));
+  if (b->has_src)  // served sessions' earlier counts, into this build's row pitch
+    for (uint32_t a = 0; a < A; ++a) {
+      const AsmHost& h = b->asms[a];
+      if (h.dense_incr)
+        RBG_TRY_HIP(ctx, hipMemcpy2DAsync((uint16_t*)b->mat.p + b->desc_h[a].mat0, (size_t)h.n * 2,
+                                          h.src->mat.p, (size_t)h.n_old * 2, (size_t)h.n_old * 2,
+                                          h.n_old, hipMemcpyDeviceToDevice, st));
+    }
   if (b->n_dense_items)
     RBG_TRY_HIP(ctx, launch_dense_counts((const uint32_t*)b->planes.p, W, desc,
                                          (const uint32_t*)b->dense_items.p, b->n_dense_items,
@@ -817,10 +840,10 @@ rbg_status assembly_result(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph
   const uint32_t K = P->max_edges, m = P->m, n = h.n, r0 = h.row0;
   if (!on_device) {
     const size_t nb = std::max<uint64_t>(ne, 1);
-    out->from = (uint32_t*)std::malloc(nb * 4);
-    out->to = (uint32_t*)std::malloc(nb * 4);
-    out->scomp = (float*)std::malloc(nb * 4);
-    out->score = h.S > 0 ? (float*)std::malloc(nb * 4) : nullptr;
+    out->from = (uint32_t*)alloc_host_result(nb * 4);
+    out->to = (uint32_t*)alloc_host_result(nb * 4);
+    out->scomp = (float*)alloc_host_result(nb * 4);
+    out->score = h.S > 0 ? (float*)alloc_host_result(nb * 4) : nullptr;
     if (!out->from || !out->to || !out->scomp || (h.S > 0 && !out->score))
       return ctx->fail(RBG_ERR_NOMEM, "host allocation of the edge list");
     // global rows -> ids local to the assembly on the way
@@ -901,9 +924,14 @@ rbg_status save_resident(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_p
     RBG_TRY_HIP(ctx, fit(r->list, std::max<uint64_t>(nlist, 1) * 8));
     RBG_TRY_HIP(ctx, launch_extract_lists((const uint64_t*)b->off.p, (const uint64_t*)b->list.p, h.row0, n,
                                           (uint64_t*)r->loff.p, (uint64_t*)r->list.p, st));
+    r->mat.release();
   } else {
     r->loff.release();
     r->list.release();
+    const size_t mb = (size_t)n * n * 2;
+    RBG_TRY_HIP(ctx, fit(r->mat, mb));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(r->mat.p, (uint16_t*)b->mat.p + b->desc_h[a].mat0, mb,
+                                    hipMemcpyDeviceToDevice, st));
   }
   RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
   r->nlist = nlist;

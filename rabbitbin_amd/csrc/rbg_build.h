@@ -70,6 +70,8 @@ rbg_status h2d_staged(rbg_ctx* ctx, void* dst, const void* src, size_t bytes);
 rbg_status d2h_staged(rbg_ctx* ctx, void* dst, const void* src, size_t bytes);
 rbg_status d2h_u32_staged(rbg_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t count, uint32_t sub);
 void release_staging(rbg_ctx* ctx);
+// Host memory for result arrays (released with free()).
+void* alloc_host_result(size_t bytes);
 rbg_status get_tables(rbg_ctx* ctx, uint32_t m, uint64_t seed, rbg_ctx::Tables** out);
 double elapsed_ms(hipEvent_t a, hipEvent_t b);
 
@@ -88,12 +90,14 @@ struct Resident {
   DevBuf q8;                  // n x kpad_for(S) i8 in pair-pass order (S > 1)
   DevBuf loff, list;          // regime 2: gated pairs of every row, CSR
   uint64_t nlist = 0;
+  DevBuf mat;                 // regime 1: n x n match counts (the diagonal unused)
   std::vector<uint32_t> perm; // pair-pass position -> local row
   std::vector<uint8_t> anynz;
   std::vector<double> colsum; // column sums of the raw depths, accumulated in row order
   GateQuant gq;               // scale and error norms of q8 (the rows are on the device)
   size_t device_bytes() const {
-    return planes.bytes + unit_pad.bytes + raw_pad.bytes + q8.bytes + loff.bytes + list.bytes;
+    return planes.bytes + unit_pad.bytes + raw_pad.bytes + q8.bytes + loff.bytes + list.bytes +
+           mat.bytes;
   }
 };
 
@@ -109,6 +113,7 @@ struct AsmHost {
   uint32_t n_old = 0;             // rows taken from src
   const Resident* src = nullptr;
   bool incr = false;              // fused pass over pairs with a new row, merged with src's lists
+  bool dense_incr = false;        // small regime: counts of pairs with a new row, src's matrix kept
   GateQuant gq;                   // int8 row statistics (the rows live on the device)
   // the assembly's host state after this build (moved into its Resident by the server)
   std::vector<uint32_t> perm_local;

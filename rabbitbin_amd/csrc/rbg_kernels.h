@@ -37,6 +37,7 @@ struct AsmDesc {
   uint32_t pair0;     // b0 pair draws: [pair0, pair0 + np) of the batch list
   uint32_t np;
   uint32_t colnorm0;  // first of the assembly's S colnorm entries
+  uint32_t jmin;      // regime 1: pairs with j < jmin (local) were counted by an earlier build
   uint64_t mat0;      // regime 1: first entry of the assembly's n x n count matrix
   double w_comp;      // composition weight
 };
@@ -72,7 +73,7 @@ hipError_t launch_calibrate(const uint32_t* rowmax, const AsmDesc* desc, uint32_
 // ---- stage (b), regime 1: n x n counts per assembly (each unordered pair
 // computed once and written at both places; the diagonal is not written)
 // Work items (assembly, column block of 256, row block, rows per block), only
-// blocks holding some pair r < j.
+// blocks holding some pair r < j with j >= the assembly's jmin.
 void dense_work_items(const std::vector<AsmDesc>& desc, uint32_t rows_per_item,
                       std::vector<uint32_t>& items);
 hipError_t launch_dense_counts(const uint32_t* planes, uint32_t W, const AsmDesc* desc,

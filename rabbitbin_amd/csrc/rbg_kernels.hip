@@ -408,14 +408,15 @@ __global__ __launch_bounds__(256) void win16_counts_kernel(const uint16_t* __res
 // ---------------------------------------------------------------------------
 // Work item (assembly a, column block cb, row block rb, rows): thread = local
 // column 256*cb + t of assembly a against its rows [rb*rows, +rows), pairs
-// r < j only; items without such a pair are not generated.
+// r < j with j >= jmin only (jmin > 0: a served session's earlier pairs are in
+// the matrix already); items without such a pair are not generated.
 template <int W>
 __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __restrict__ planes,
                                                            const AsmDesc* __restrict__ desc,
                                                            const uint4* __restrict__ items,
                                                            uint16_t* __restrict__ mat) {
   const uint4 it = items[blockIdx.x];
-  const uint32_t n = desc[it.x].n;
+  const uint32_t n = desc[it.x].n, jmin = desc[it.x].jmin;
   const uint32_t* __restrict__ rows = planes + (size_t)desc[it.x].row0 * 8 * W;
   uint16_t* __restrict__ out = mat + desc[it.x].mat0;
   const uint32_t j = it.y * 256 + threadIdx.x;
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
     uint32_t mism = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) mism += __popc(word_mismatch(a + 8 * w, b + 8 * w));
-    if (j < n && j > r) {
+    if (j < n && j > r && j >= jmin) {
       const uint16_t c = (uint16_t)(32u * W - mism);
       out[(size_t)r * n + j] = c;
       out[(size_t)j * n + r] = c;
@@ -658,21 +659,21 @@ __device__ __forceinline__ void unpack_pair(uint64_t e, uint32_t& i, uint32_t& j
 // of both rows; lanes q and q + 4 compute the same values); the first
 // reduction level pairs lanes q and q^2, the second q and q^1, the last runs
 // in every lane (IEEE addition is commutative, so which lane holds which
-// operand does not change a sum).  For the register compare, lane q holds the
-// 16-byte pieces q, q + 8, ... of both rows' planes, so each load instruction
-// reads one whole 128-byte line per candidate; a piece is half a word group
-// (planes 0-3 in even lanes, 4-7 in odd lanes), and lanes q and q^1 combine
-// the halves: the AND over all eight planes of b (the unset-register mask)
-// and the OR of the eight mismatch words.  All loads are issued before the
-// gate is known: nearly every candidate survives.  Survivors add 1 to deg[i]
-// and deg[j].
+// operand does not change a sum).  For the register compare, lane q holds
+// word groups q, q + 8, ... of both rows' planes (32 bytes each: one group's
+// 8 planes), so the eight lanes of a candidate read 256 contiguous bytes of a
+// row per pair of loads.  (Pieces of 16 bytes per lane, one whole line per
+// load with the two halves of a group in neighbouring lanes, measured slower:
+// the lane exchanges cost more than the fewer lines saved.)  All loads are
+// issued before the gate is known: nearly every candidate survives.
+// Survivors add 1 to deg[i] and deg[j].
 template <int NC, int W>
 __device__ __forceinline__ void gate_compare_one(
     uint64_t* __restrict__ cand, uint64_t t, uint64_t nc, uint32_t q,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
     const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
-  constexpr int PPL = (2 * W + 7) / 8;  // 16-byte plane pieces per lane and row
+  constexpr int GPL = (W + 7) / 8;  // word groups per lane
   const bool live = t < nc;
   const uint64_t e = live ? cand[t] : 0;
   // Row indices are taken as 64-bit fields and masked in 64 bits: with the
@@ -685,7 +686,7 @@ __device__ __forceinline__ void gate_compare_one(
   const float thr = ad.thr;
   const uint32_t acc1_end = ad.acc1_end;
   float4 ua[NC], ub[NC];
-  uint4 pa[PPL], pb[PPL];
+  uint4 pa[2 * GPL], pb[2 * GPL];
   if (live) {
     const float4* ra = reinterpret_cast<const float4*>(unit + i64 * 16 * NC);
     const float4* rb = reinterpret_cast<const float4*>(unit + j64 * 16 * NC);
@@ -694,17 +695,18 @@ __device__ __forceinline__ void gate_compare_one(
       ua[c] = ra[4 * c + q4];
       ub[c] = rb[4 * c + q4];
     }
-    const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W);
-    const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W);
 #pragma unroll
-    for (int x = 0; x < PPL; ++x) {
-      const uint32_t u = 8u * x + q;
-      pa[x] = u < 2u * W ? wa[u] : make_uint4(0, 0, 0, 0);
-      pb[x] = u < 2u * W ? wb[u] : make_uint4(0, 0, 0, 0);
+    for (int x = 0; x < GPL; ++x) {
+      const uint32_t w = 8u * x + q;
+      if (w < (uint32_t)W) {
+        const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W + 8u * w);
+        const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W + 8u * w);
+        pa[2 * x] = wa[0];
+        pa[2 * x + 1] = wa[1];
+        pb[2 * x] = wb[0];
+        pb[2 * x + 1] = wb[1];
+      }
     }
-  } else {
-#pragma unroll
-    for (int x = 0; x < PPL; ++x) pa[x] = pb[x] = make_uint4(0, 0, 0, 0);
   }
   // gate value
   float acc0[4] = {0.f, 0.f, 0.f, 0.f}, acc1[4] = {0.f, 0.f, 0.f, 0.f};
@@ -737,21 +739,22 @@ __device__ __forceinline__ void gate_compare_one(
   const bool nz = !anynz || (live && (anynz[i] || anynz[j]));
   bool pass = live && nz && !(dot < thr);
   pass = __shfl(pass, (int)(lane_id() & ~7u));  // the group's lane 0 decides
-  // register compare (every lane takes part in the pair exchanges; pieces
-  // past the row are zero and count nothing)
+  // register compare
   uint32_t mism = 0;
-  const bool half0 = (q & 1u) == 0;
+  if (pass) {
 #pragma unroll
-  for (int x = 0; x < PPL; ++x) {
-    const uint32_t all_half = pb[x].x & pb[x].y & pb[x].z & pb[x].w;
-    const uint32_t unset = all_half & __shfl_xor(all_half, 1);  // all eight planes set
-    const uint32_t b0 = half0 ? pb[x].x & ~unset : pb[x].x;       // plane 0 sits in the even lane
-    const uint32_t mm_half = (pa[x].x ^ b0) | (pa[x].y ^ pb[x].y) | (pa[x].z ^ pb[x].z) |
-                             (pa[x].w ^ pb[x].w);
-    const uint32_t mm = mm_half | __shfl_xor(mm_half, 1);
-    if (half0 && 8u * x + q < 2u * W) mism += __popc(mm);
+    for (int x = 0; x < GPL; ++x) {
+      if (8u * x + q < (uint32_t)W) {
+        const uint32_t av[8] = {pa[2 * x].x, pa[2 * x].y, pa[2 * x].z, pa[2 * x].w,
+                                pa[2 * x + 1].x, pa[2 * x + 1].y, pa[2 * x + 1].z, pa[2 * x + 1].w};
+        const uint32_t bs[8] = {pb[2 * x].x, pb[2 * x].y, pb[2 * x].z, pb[2 * x].w,
+                                pb[2 * x + 1].x, pb[2 * x + 1].y, pb[2 * x + 1].z, pb[2 * x + 1].w};
+        uint32_t bv[8];
+        column_word(bs, bv);
+        mism += __popc(word_mismatch(av, bv));
+      }
+    }
   }
-  if (!pass) mism = 0;
   mism += __shfl_xor(mism, 1);
   mism += __shfl_xor(mism, 2);
   mism += __shfl_xor(mism, 4);
@@ -1874,7 +1877,8 @@ void dense_work_items(const std::vector<AsmDesc>& desc, uint32_t rows_per_item,
     const uint32_t n = desc[a].n;
     for (uint32_t rb = 0; rb * rows_per_item < n; ++rb)
       for (uint32_t cb = 0; cb * 256 < n; ++cb) {
-        if (std::min(n, cb * 256 + 256) - 1 <= rb * rows_per_item) continue;  // no pair r < j
+        const uint32_t jl = std::min(n, cb * 256 + 256) - 1;  // the block's last column
+        if (jl <= rb * rows_per_item || jl < desc[a].jmin) continue;  // no pair r < j, j >= jmin
         items.push_back(a);
         items.push_back(cb);
         items.push_back(rb);

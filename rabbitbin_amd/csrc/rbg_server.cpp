@@ -50,8 +50,12 @@ struct Session {
   Resident res;
 };
 
-// Resident bytes a session gains with n_add more rows (planes, padded depth
-// rows, int8 rows, list offsets and an allowance of 2K list entries per row).
+// Resident bytes per row of a session (planes, padded depth rows, int8 rows,
+// list offsets and an allowance of 2K list entries per row); a small-regime
+// session also keeps its n x n counts (resident_matrix_bytes).
+size_t resident_matrix_bytes(uint32_t n, const rbg_graph_params& P) {
+  return regime_for(P.regime, n) == 1 ? (size_t)n * n * 2 : 0;
+}
 size_t resident_row_bytes(uint32_t S, const rbg_graph_params& P) {
   const size_t W = words_for(P.m);
   size_t b = 8 * W * 4 + 8 + (size_t)2 * P.max_edges * 8;
@@ -150,8 +154,8 @@ void rbg_server::select(std::vector<Request*>& step) {
     x.src = r->extend ? &s->res : nullptr;
     ins.push_back(x);
     // growth of the session's state (its buffers grow with 25% headroom)
-    const size_t g = (size_t)n * resident_row_bytes(r->in.S, P) * 5 / 4 -
-                     std::min<size_t>(s->res.device_bytes(), (size_t)n * resident_row_bytes(r->in.S, P) * 5 / 4);
+    const size_t after = ((size_t)n * resident_row_bytes(r->in.S, P) + resident_matrix_bytes(n, P)) * 5 / 4;
+    const size_t g = after - std::min<size_t>(s->res.device_bytes(), after);
     const size_t need = resident + growth + g + estimate_build_bytes(ins.data(), (uint32_t)ins.size(), &P);
     if (need > cfg.hbm_budget) {
       ins.pop_back();
@@ -211,9 +215,10 @@ void rbg_server::run_step(std::vector<Request*>& step) {
     for (uint32_t k = 0; k < A; ++k) {
       Request* r = step[k];
       const AsmHost& h = ws.asms[k];
-      r->info.incremental = h.incr ? 1 : 0;
-      r->info.pass_pairs = h.incr ? (uint64_t)h.n * (h.n - 1) / 2 - (uint64_t)h.n_old * (h.n_old - 1) / 2
-                                  : (uint64_t)h.n * (h.n - 1) / 2;
+      const bool incr = h.incr || h.dense_incr;
+      r->info.incremental = incr ? 1 : 0;
+      r->info.pass_pairs = incr ? (uint64_t)h.n * (h.n - 1) / 2 - (uint64_t)h.n_old * (h.n_old - 1) / 2
+                                : (uint64_t)h.n * (h.n - 1) / 2;
       const double t0 = now();
       rs[k] = assembly_result(ctx, &ws, k, &P, false, &r->result, nullptr);
       const double t1 = now();

@@ -103,9 +103,9 @@ def test_served_mix_equals_plain(workload, plain):
             inf = infos[k][t]
             assert inf["status"] == 0 and inf["n"] == sum(len(p[1]) - 1 for p in parts[:t + 1])
             shared += inf["step_requests"] > 1
-            # extensions of a session whose earlier build was fused are incremental
-            prev_fused = t > 0 and plain[k][t - 1]["regime"] == 2
-            assert inf["incremental"] == int(prev_fused and plain[k][t]["regime"] == 2), (k, t)
+            # an extension is incremental when its session stays in the same regime
+            same = t > 0 and plain[k][t - 1]["regime"] == plain[k][t]["regime"]
+            assert inf["incremental"] == int(same), (k, t)
     assert shared > 0
     assert any(i["incremental"] for row in infos for i in row)
 
