@@ -159,15 +159,19 @@ __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_
   const uint64_t c40 = (uint64_t)codes8(prev) | ((uint64_t)codes8(cur.x) << 8) |
                        ((uint64_t)codes8(cur.y) << 16) | ((uint64_t)codes8(cur.z) << 24) |
                        ((uint64_t)codes8(cur.w) << 32);
+  // The same 20 codes in reverse order (code t at bits 2(19-t)): the forward
+  // code of a window (earliest base highest) is then one shift, and the
+  // reverse complement one shift of the complemented codes.
+  uint64_t r40 = (uint64_t)__builtin_bitreverse64(c40) >> 24;  // each code's two bits swapped
+  r40 = ((r40 >> 1) & 0x5555555555555555ull) | ((r40 & 0x5555555555555555ull) << 1);
+  const uint64_t n40 = ~c40;
   // bit i: the 4-mer ending at chunk byte i (buffer bytes 16j+i-3 .. 16j+i) is valid
   const uint32_t km = ((vm & (vm >> 1) & (vm >> 2) & (vm >> 3)) >> 1) & 0xFFFFu;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     if ((km >> i) & 1u) {
-      const uint32_t win = (uint32_t)(c40 >> (2 * (i + 1))) & 0xFFu;
-      const uint32_t fwd = ((win & 0x03u) << 6) | ((win & 0x0Cu) << 2) | ((win & 0x30u) >> 2) |
-                           ((win & 0xC0u) >> 6);
-      const uint32_t rev = ~win & 0xFFu;
+      const uint32_t fwd = (uint32_t)(r40 >> (2 * (15 - i))) & 0xFFu;
+      const uint32_t rev = (uint32_t)(n40 >> (2 * (i + 1))) & 0xFFu;
       atomicAdd(&hist[fwd <= rev ? fwd : rev], 1u);
     }
   }
@@ -581,6 +585,11 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
 #pragma unroll
         for (int v = (r == 0 ? 1 : 0); v < 4; ++v) mx = max(mx, acc[r][v]);
       if (__ballot(mx >= qcut) == 0) continue;  // no pair of these 1024 is near the gate
+      // The lane's first row within the stripe, made opaque here so that the
+      // compiler derives the 16 row ids inside this rarely taken path instead
+      // of keeping them in registers across the loop (which cost occupancy).
+      uint32_t il0 = 64 * wv + 4 * (ln >> 4);
+      asm volatile("" : "+v"(il0));
       const uint32_t j = j0 + (ln & 15u);
       const uint32_t jl = j - jbase;
       const bool full = !diag && jb + 256 <= n && jb >= jmin;
@@ -591,12 +600,12 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
         if (bn > kGateBuf - 256) flush();
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const uint32_t i = row0 + 16 * r + 4 * (ln >> 4) + v;
+          const uint32_t il = il0 + 16 * r + v;  // row ib + il
           const bool c = full ? acc[r][v] >= qcut
-                              : (acc[r][v] >= qcut && i < j && j < n && j >= jmin);
+                              : (acc[r][v] >= qcut && ib + il < j && j < n && j >= jmin);
           const uint64_t mask = __ballot(c);
           if (mask) {
-            if (c) buf[wv][bn + lanes_below(mask)] = ((i - ib) << 12) | jl;
+            if (c) buf[wv][bn + lanes_below(mask)] = (il << 12) | jl;
             bn += (uint32_t)__popcll(mask);
           }
         }
@@ -777,8 +786,11 @@ __device__ __forceinline__ void xcd_segment(uint64_t total, uint64_t& s0, uint64
   s1 = total * (L + 1) / G;
 }
 
+// Six waves per SIMD for S <= 32: the <2, 16> instance (m = 500) needs 81
+// VGPRs unconstrained (five waves) and fits 80 without spilling; the extra
+// wave of gathers in flight measured 5% faster.  Wider rows would spill.
 template <int NC, int W>
-__global__ __launch_bounds__(256) void gate_compare_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6 : 1))) void gate_compare_kernel(
     uint64_t* __restrict__ cand, uint64_t cap, const unsigned long long* __restrict__ ncand,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
