@@ -454,34 +454,42 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
 // the pairs the reference's gate keeps (src/impl/rb_graph.cpp:1108-1116).
 //
 // gate_mfma_kernel: work item = a 256-row stripe bi and up to 16 consecutive
-// 256-column tiles bj0.. (bi <= bj).  Wave w owns rows bi*256+64w..+64 as four
-// 16-row operands held in registers for the whole item; the tiles' column
+// 256-column tiles bj0.. (bi <= bj).  Wave w owns rows bi*256+64w..+64 as two
+// 32-row operands held in registers for the whole item; the tiles' column
 // operands are double-buffered in LDS (the next tile's loads are in flight
-// while the current tile computes).  v_mfma_i32_16x16x32_i8 gives the integer
-// dot of the quantised rows for 16 x 16 pairs (lane l, register v -> row
-// 4*(l/16)+v, column l%16).  Candidates are staged per wave in LDS as local
-// coordinates and appended to `cand` with one global atomic per flush.
+// while the current tile computes).  v_mfma_i32_32x32x32_i8 (the gfx950
+// double-rate form: 32 x 32 pairs at K = 32 per instruction) gives the integer
+// dot of the quantised rows: lane l supplies row (or column) l%32, bytes
+// 16*(l/32)..+15 of each 32-byte K chunk, and holds in register v the pair
+// (row 8*(v/4) + 4*(l/32) + v%4, column l%32).  Per 32-column step a wave
+// makes 2048 pair dots with 2*KC MFMAs and takes the lane maximum with 16
+// max3; only a step whose maximum reaches qcut looks at single pairs.
+// Candidates are staged per wave in LDS as local coordinates and appended to
+// `cand` with one global atomic per flush.
 // ---------------------------------------------------------------------------
 typedef int v4i32 __attribute__((ext_vector_type(4)));
-constexpr uint32_t kGateBuf = 512;  // per-wave candidate buffer (a 16-row block adds <= 256)
+typedef int v16i32 __attribute__((ext_vector_type(16)));
+constexpr uint32_t kGateBuf = 512;  // per-wave candidate buffer, flushed when a ballot may not fit
 
+// Lane slice of a q8 row (zero beyond n): bytes 32c + 16h .. +15 per K chunk c.
 template <int KC>
-__device__ __forceinline__ void load_q8(const int8_t* __restrict__ q, uint32_t row, uint32_t n,
-                                        uint32_t kpad, uint32_t ln, long* out) {
+__device__ __forceinline__ void load_q8_slice(const int8_t* __restrict__ q, uint32_t row, uint32_t n,
+                                              uint32_t h, v4i32* out) {
 #pragma unroll
   for (int c = 0; c < KC; ++c)
-    out[c] = row < n ? *reinterpret_cast<const long*>(q + (size_t)row * kpad + 32 * c + 8 * (ln >> 4)) : 0L;
+    out[c] = row < n ? *reinterpret_cast<const v4i32*>(q + (size_t)row * (32 * KC) + 32 * c + 16 * h)
+                     : v4i32{0, 0, 0, 0};
 }
 
-// This thread's 4 entries of a tile's column operands: entry e = tid + 256k
-// is (step e/64, lane e%64).
+// This thread's 2 entries of a tile's column operands: entry e = tid + 256k
+// is (step e/64, lane e%64), column 32*(e/64) + e%32, half (e%64)/32.
 template <int KC>
 __device__ __forceinline__ void load_tile_ops(const int8_t* __restrict__ q, uint32_t jb, uint32_t n,
-                                              long (*dst)[KC]) {
+                                              v4i32 (*dst)[KC]) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 2; ++k) {
     const uint32_t e = threadIdx.x + 256u * k;
-    load_q8<KC>(q, jb + 16u * (e >> 6) + (e & 15u), n, 32 * KC, e & 63u, dst[k]);
+    load_q8_slice<KC>(q, jb + 32u * (e >> 6) + (e & 31u), n, (e >> 5) & 1u, dst[k]);
   }
 }
 
@@ -494,7 +502,7 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
                                                         unsigned long long* __restrict__ ncand,
                                                         uint64_t cap) {
   __shared__ uint32_t buf[4][kGateBuf];
-  __shared__ long bops[2][16][KC][64];
+  __shared__ v4i32 bops[2][8][KC][64];
   const uint32_t wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
   const uint4 item = items[blockIdx.x];
   // the item's assembly: its rows, in pair-pass order, start at row0
@@ -505,30 +513,14 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
   const uint32_t ib = (item.y >> 16) * 256u, jbase = (item.y & 0xFFFFu) * 256u;
   const uint32_t ntile = item.z;
   const uint32_t jmin = item.w;  // pairs with j < jmin were evaluated by an earlier build
-  const uint32_t row0 = ib + 64 * wv;
+  // the wave's first row, in a scalar register: the step skips below are branches
+  const uint32_t row0 = __builtin_amdgcn_readfirstlane(ib + 64 * wv);
 
-  // Row operands go through LDS (buffer 1 is free until tile 1), so inside
-  // the loop the only outstanding vector-memory loads are the prefetch of
-  // the next tile.
-  {
-    long ra[4][KC];
-    load_tile_ops<KC>(q, ib, n, ra);
+  v4i32 A[2][KC];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t e = threadIdx.x + 256u * k;
-#pragma unroll
-      for (int c = 0; c < KC; ++c) bops[1][e >> 6][c][e & 63u] = ra[k][c];
-    }
-  }
-  long nb[4][KC];
+  for (int rb = 0; rb < 2; ++rb) load_q8_slice<KC>(q, row0 + 32 * rb + (ln & 31u), n, ln >> 5, A[rb]);
+  v4i32 nb[2][KC];
   load_tile_ops<KC>(q, jbase, n, nb);
-  __syncthreads();
-  long A[4][KC];
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < KC; ++c) A[r][c] = bops[1][4 * wv + r][c][ln];
-  __syncthreads();
   uint32_t bn = 0;  // wave-uniform count of staged candidates
 
   auto flush = [&]() {
@@ -547,9 +539,10 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
 
 #pragma unroll 1
   for (uint32_t t = 0; t < ntile; ++t) {
-    // tile t's operands (prefetched) go to LDS; then prefetch tile t+1
+    // tile t's operands (prefetched) go to LDS (the buffer of tile t-2: every
+    // wave passed the barrier of tile t-1 after reading it); then prefetch t+1
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 2; ++k) {
       const uint32_t e = threadIdx.x + 256u * k;
 #pragma unroll
       for (int c = 0; c < KC; ++c) bops[t & 1][e >> 6][c][e & 63u] = nb[k][c];
@@ -558,53 +551,49 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
     if (t + 1 < ntile) load_tile_ops<KC>(q, jbase + 256u * (t + 1), n, nb);
     const uint32_t jb = jbase + 256u * t;
     const bool diag = ib == jb;
-    long Bn[KC];
-#pragma unroll
-    for (int c = 0; c < KC; ++c) Bn[c] = bops[t & 1][0][c][ln];
 #pragma unroll 1
-    for (uint32_t s = 0; s < 16; ++s) {
-      long B[KC];
+    for (uint32_t s = 0; s < 8; ++s) {
+      const uint32_t j0 = jb + 32u * s;
+      if (j0 >= n) break;
+      if (diag && j0 + 31 <= row0) continue;  // every pair of the step has j <= i
+      v4i32 B[KC];
+#pragma unroll
+      for (int c = 0; c < KC; ++c) B[c] = bops[t & 1][s][c][ln];
+      v16i32 acc0 = {}, acc1 = {};
 #pragma unroll
       for (int c = 0; c < KC; ++c) {
-        B[c] = Bn[c];
-        if (s + 1 < 16) Bn[c] = bops[t & 1][s + 1][c][ln];  // the next step's operand in flight
+        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][c], B[c], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][c], B[c], acc1, 0, 0, 0);
       }
-      const uint32_t j0 = jb + 16u * s;
-      if (j0 >= n || (diag && j0 + 15 <= row0)) continue;  // blocks with every j <= i
-      v4i32 acc[4];
+      int32_t mx = acc0[0];  // the largest of the lane's 32 dots (a chain of max3)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        acc[r] = v4i32{0, 0, 0, 0};
+      for (int v = 1; v < 16; ++v) mx = max(mx, acc0[v]);
 #pragma unroll
-        for (int c = 0; c < KC; ++c)
-          acc[r] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A[r][c], B[c], acc[r], 0, 0, 0);
-      }
-      int32_t mx = acc[0][0];  // the largest of the lane's 16 dots (a chain of max3)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int v = (r == 0 ? 1 : 0); v < 4; ++v) mx = max(mx, acc[r][v]);
-      if (__ballot(mx >= qcut) == 0) continue;  // no pair of these 1024 is near the gate
+      for (int v = 0; v < 16; ++v) mx = max(mx, acc1[v]);
+      if (__ballot(mx >= qcut) == 0) continue;  // no pair of these 2048 is near the gate
       // The lane's first row within the stripe, made opaque here so that the
-      // compiler derives the 16 row ids inside this rarely taken path instead
-      // of keeping them in registers across the loop (which cost occupancy).
-      uint32_t il0 = 64 * wv + 4 * (ln >> 4);
+      // compiler derives the row ids inside this rarely taken path instead of
+      // keeping them in registers across the loop (which costs occupancy).
+      uint32_t il0 = 64 * wv + 4 * (ln >> 5);
       asm volatile("" : "+v"(il0));
-      const uint32_t j = j0 + (ln & 15u);
+      const uint32_t j = j0 + (ln & 31u);
       const uint32_t jl = j - jbase;
       const bool full = !diag && jb + 256 <= n && jb >= jmin;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int32_t mr = max(max(acc[r][0], acc[r][1]), max(acc[r][2], acc[r][3]));
-        if (__ballot(mr >= qcut) == 0) continue;  // only the 16-row blocks with candidates
-        if (bn > kGateBuf - 256) flush();
+      for (int rb = 0; rb < 2; ++rb) {
+        const v16i32& acc = rb ? acc1 : acc0;
+        int32_t mr = acc[0];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const uint32_t il = il0 + 16 * r + v;  // row ib + il
-          const bool c = full ? acc[r][v] >= qcut
-                              : (acc[r][v] >= qcut && ib + il < j && j < n && j >= jmin);
+        for (int v = 1; v < 16; ++v) mr = max(mr, acc[v]);
+        if (__ballot(mr >= qcut) == 0) continue;  // only the 32-row blocks with candidates
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const uint32_t il = il0 + 32 * rb + 8 * (v / 4) + (v % 4);  // row ib + il
+          const bool c = full ? acc[v] >= qcut
+                              : (acc[v] >= qcut && ib + il < j && j < n && j >= jmin);
           const uint64_t mask = __ballot(c);
           if (mask) {
+            if (bn > kGateBuf - 64) flush();
             if (c) buf[wv][bn + lanes_below(mask)] = (il << 12) | jl;
             bn += (uint32_t)__popcll(mask);
           }
