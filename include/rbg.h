@@ -174,8 +174,8 @@ typedef struct rbg_assembly_in {
 /* Work schedule of the pair pass: 256-column tiles per work item of the
  * abundance gate, and rows per work item of the all-pairs counts. */
 typedef struct rbg_schedule {
-  uint32_t gate_tiles_per_item;   /* default 16 */
-  uint32_t dense_rows_per_item;   /* default 64 */
+  uint32_t gate_tiles_per_item;   /* default 16, 1..64 */
+  uint32_t dense_rows_per_item;   /* default 64, 1..4096 */
 } rbg_schedule;
 
 typedef struct rbg_batch_stats {

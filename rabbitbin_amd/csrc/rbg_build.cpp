@@ -510,6 +510,7 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
                  : -__builtin_huge_valf();
     // prefilter cut: with the gate off every pair is a candidate
     d.qcut = gate ? gate_qcut(h.gq, d.thr) : INT32_MIN;
+    d.qcut_hi = gate ? std::max(gate_qcut_hi(h.gq, d.thr), d.qcut) : INT32_MAX;
     d.acc1_end = gate ? 2 * (h.S / 32) : 0;
     d.pair0 = h.pair0;
     d.np = h.np;

@@ -385,8 +385,8 @@ void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32
   std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
 }
 
-int32_t gate_qcut(const GateQuant& g, float thr) {
-  if (g.scale <= 0) return INT32_MIN;
+// Bound on |fl(u_i . u_j) - q_i . q_j / s^2| for every pair of rows of g.
+static double gate_slack(const GateQuant& g) {
   // FP32 dot: each product passes through at most ceil(S/16) lane FMAs, the
   // acc0 + acc1 add and four reduction levels.
   const double u = std::ldexp(1.0, -24);
@@ -394,8 +394,20 @@ int32_t gate_qcut(const GateQuant& g, float thr) {
   const double gamma = depth * u / (1.0 - depth * u);
   // |dot(u_i,u_j) - dot(q_i,q_j)/s^2| <= 2 X E + E^2 (Cauchy-Schwarz), plus
   // a margin for the double arithmetic of this bound.
-  const double slack = 2.0 * g.max_qnorm * g.max_err + g.max_err * g.max_err +
-                       gamma * g.max_unorm * g.max_unorm + 1e-6;
+  return 2.0 * g.max_qnorm * g.max_err + g.max_err * g.max_err + gamma * g.max_unorm * g.max_unorm +
+         1e-6;
+}
+
+int32_t gate_qcut_hi(const GateQuant& g, float thr) {
+  if (g.scale <= 0) return INT32_MAX;
+  const double cut = g.scale * g.scale * ((double)thr + gate_slack(g));
+  if (cut <= 0.0 || cut >= (double)INT32_MAX) return INT32_MAX;
+  return (int32_t)std::ceil(cut);
+}
+
+int32_t gate_qcut(const GateQuant& g, float thr) {
+  if (g.scale <= 0) return INT32_MIN;
+  const double slack = gate_slack(g);
   const double cut = g.scale * g.scale * ((double)thr - slack);
   if (cut <= (double)INT32_MIN + 1) return INT32_MIN;
   if (cut > (double)INT32_MAX) return INT32_MAX;

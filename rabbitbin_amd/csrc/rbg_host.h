@@ -76,6 +76,10 @@ void gate_quantize_more(const float* unit, size_t n, uint32_t S, GateQuant& g);
 void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32_t>& perm);
 // Smallest integer dot a kept pair can have; INT32_MIN disables the prefilter.
 int32_t gate_qcut(const GateQuant& g, float thr);
+// Smallest integer dot that proves a pair kept: the same error terms taken the
+// other way, so qdot >= gate_qcut_hi(thr) implies fl(u_i . u_j) >= thr (and,
+// the cut being positive, both rows nonzero).  INT32_MAX: no pair is proven.
+int32_t gate_qcut_hi(const GateQuant& g, float thr);
 
 // glibc rand() stream with private state (srand(seed) then rand()).
 class RandStream {

@@ -33,6 +33,7 @@ struct AsmDesc {
   uint32_t regime;    // 1: all pairs (n x n counts); 2: abundance-gated candidates
   float thr;          // exact abundance-gate threshold (-inf: gate off)
   int32_t qcut;       // int8 prefilter cut
+  int32_t qcut_hi;    // int8 dots from here on pass the exact gate (INT32_MAX: none)
   uint32_t acc1_end;  // odd chunks below this accumulate into the second register
   uint32_t pair0;     // b0 pair draws: [pair0, pair0 + np) of the batch list
   uint32_t np;

@@ -464,20 +464,24 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
 // (row 8*(v/4) + 4*(l/32) + v%4, column l%32).  Per 32-column step a wave
 // makes 2048 pair dots with 2*KC MFMAs and takes the lane maximum with 16
 // max3; only a step whose maximum reaches qcut looks at single pairs.
-// Candidates are staged per wave in LDS as local coordinates and appended to
-// `cand` with one global atomic per flush.
+// Candidates are staged per wave in LDS as local coordinates (row in the
+// stripe << 16 | column from the item's first tile: items of up to 256 tiles;
+// bit 31: the dot reaches qcut_hi, so the pair passes the exact gate) and
+// appended to `cand` with one global atomic per flush (the proof as bit 15).
 // ---------------------------------------------------------------------------
 typedef int v4i32 __attribute__((ext_vector_type(4)));
 typedef int v16i32 __attribute__((ext_vector_type(16)));
 constexpr uint32_t kGateBuf = 512;  // per-wave candidate buffer, flushed when a ballot may not fit
 
 // Lane slice of a q8 row (zero beyond n): bytes 32c + 16h .. +15 per K chunk c.
+// The offset is 32-bit (an assembly's q8 rows span < 2^30 bytes), so the load
+// takes the assembly's base in scalar registers.
 template <int KC>
 __device__ __forceinline__ void load_q8_slice(const int8_t* __restrict__ q, uint32_t row, uint32_t n,
                                               uint32_t h, v4i32* out) {
 #pragma unroll
   for (int c = 0; c < KC; ++c)
-    out[c] = row < n ? *reinterpret_cast<const v4i32*>(q + (size_t)row * (32 * KC) + 32 * c + 16 * h)
+    out[c] = row < n ? *reinterpret_cast<const v4i32*>(q + (row * (32u * KC) + 32u * c + 16u * h))
                      : v4i32{0, 0, 0, 0};
 }
 
@@ -493,8 +497,10 @@ __device__ __forceinline__ void load_tile_ops(const int8_t* __restrict__ q, uint
   }
 }
 
+// Six waves per SIMD for S <= 32 (81 VGPRs unconstrained: five); the one
+// spilled value is reloaded per tile before the tile's loads are issued.
 template <int KC>
-__global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict__ q_all,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KC == 1 ? 6 : 1))) void gate_mfma_kernel(const int8_t* __restrict__ q_all,
                                                         const AsmDesc* __restrict__ desc,
                                                         const uint32_t* __restrict__ perm_all,
                                                         const uint4* __restrict__ items,
@@ -507,7 +513,7 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
   const uint4 item = items[blockIdx.x];
   // the item's assembly: its rows, in pair-pass order, start at row0
   const uint32_t n = desc[item.x].n;
-  const int32_t qcut = desc[item.x].qcut;
+  const int32_t qcut = desc[item.x].qcut, qcut_hi = desc[item.x].qcut_hi;
   const int8_t* __restrict__ q = q_all + (size_t)desc[item.x].row0 * 32 * KC;
   const uint32_t* __restrict__ perm = perm_all + desc[item.x].row0;
   const uint32_t ib = (item.y >> 16) * 256u, jbase = (item.y & 0xFFFFu) * 256u;
@@ -530,9 +536,9 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
     base = __shfl(base, 0);
     for (uint32_t t = ln; t < bn; t += 64) {
       const uint32_t e = buf[wv][t];
-      const uint64_t a = perm[ib + (e >> 12)], b = perm[jbase + (e & 0xFFFu)];
+      const uint64_t a = perm[ib + ((e >> 16) & 0x7FFFu)], b = perm[jbase + (e & 0xFFFFu)];
       const uint64_t i = a < b ? a : b, j = a < b ? b : a;
-      if (base + t < cap) cand[base + t] = (i << 40) | (j << 16);
+      if (base + t < cap) cand[base + t] = (i << 40) | (j << 16) | (uint64_t)((e >> 31) << 15);
     }
     bn = 0;
   };
@@ -551,7 +557,7 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
     if (t + 1 < ntile) load_tile_ops<KC>(q, jbase + 256u * (t + 1), n, nb);
     const uint32_t jb = jbase + 256u * t;
     const bool diag = ib == jb;
-#pragma unroll 1
+#pragma unroll 2
     for (uint32_t s = 0; s < 8; ++s) {
       const uint32_t j0 = jb + 32u * s;
       if (j0 >= n) break;
@@ -594,7 +600,8 @@ __global__ __launch_bounds__(256) void gate_mfma_kernel(const int8_t* __restrict
           const uint64_t mask = __ballot(c);
           if (mask) {
             if (bn > kGateBuf - 64) flush();
-            if (c) buf[wv][bn + lanes_below(mask)] = (il << 12) | jl;
+            const uint32_t sure = acc[v] >= qcut_hi ? 1u : 0u;  // proven to pass the exact gate
+            if (c) buf[wv][bn + lanes_below(mask)] = (sure << 31) | (il << 16) | jl;
             bn += (uint32_t)__popcll(mask);
           }
         }
@@ -679,13 +686,17 @@ __device__ __forceinline__ void gate_compare_one(
   // address of some instantiations while keeping it elsewhere.
   const uint64_t i64 = e >> 40, j64 = (e >> 16) & 0xFFFFFFull;
   const uint32_t i = (uint32_t)i64, j = (uint32_t)j64;
+  // proven by the prefilter (its dot reached qcut_hi): no FP32 gate needed
+  const bool sure = live && ((e >> 15) & 1u);
   const uint32_t q4 = q & 3u;
   const AsmDesc& ad = desc[live ? asm_of[i] : 0];
   const float thr = ad.thr;
   const uint32_t acc1_end = ad.acc1_end;
   float4 ua[NC], ub[NC];
   uint4 pa[2 * GPL], pb[2 * GPL];
-  if (live) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) ua[c] = ub[c] = float4{0.f, 0.f, 0.f, 0.f};
+  if (live && !sure) {
     const float4* ra = reinterpret_cast<const float4*>(unit + i64 * 16 * NC);
     const float4* rb = reinterpret_cast<const float4*>(unit + j64 * 16 * NC);
 #pragma unroll
@@ -693,6 +704,8 @@ __device__ __forceinline__ void gate_compare_one(
       ua[c] = ra[4 * c + q4];
       ub[c] = rb[4 * c + q4];
     }
+  }
+  if (live) {
 #pragma unroll
     for (int x = 0; x < GPL; ++x) {
       const uint32_t w = 8u * x + q;
@@ -734,8 +747,8 @@ __device__ __forceinline__ void gate_compare_one(
 #pragma unroll
   for (int l = 0; l < 4; ++l) t4[l] = __shfl_xor(t8[l], 1) + t8[l];  // l+4 with l
   const float dot = (t4[0] + t4[2]) + (t4[1] + t4[3]);
-  const bool nz = !anynz || (live && (anynz[i] || anynz[j]));
-  bool pass = live && nz && !(dot < thr);
+  const bool nz = sure || !anynz || (live && (anynz[i] || anynz[j]));
+  bool pass = live && (sure || (nz && !(dot < thr)));
   pass = __shfl(pass, (int)(lane_id() & ~7u));  // the group's lane 0 decides
   // register compare
   uint32_t mism = 0;

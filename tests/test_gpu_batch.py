@@ -42,7 +42,7 @@ def plain(ctx, mixed):
     return [ctx.build_graph(s, o, d, p) for s, o, d in mixed]
 
 
-@pytest.mark.parametrize("tiles,rows", [(16, 64), (3, 17)])
+@pytest.mark.parametrize("tiles,rows", [(16, 64), (3, 17), (32, 128), (64, 4096)])
 def test_batch_equals_plain(ctx, mixed, plain, tiles, rows):
     b = api.Batch(ctx, mixed)
     try:
