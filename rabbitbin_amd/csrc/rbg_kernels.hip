@@ -117,21 +117,21 @@ constexpr int kRankGroup = 8;  // proposal ranks loaded together in phase 2
 __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
   return (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
 }
-// Bit b (b < 4) set iff byte b of w is A, C, G or T in either case.
-__device__ __forceinline__ uint32_t base_bits4(uint32_t w) {
+// One word of sequence: the 2-bit codes of its bytes (A, C, G, T = 0..3 in
+// either case; byte b in bits 2b..2b+1) and bit b set iff byte b is one of
+// A, C, G, T.  The code is read from bits 1-2 of the byte; v_perm_b32 maps
+// each code back to its upper-case letter, which must equal the byte.
+__device__ __forceinline__ void decode_word(uint32_t w, uint32_t& code8, uint32_t& valid4) {
   const uint32_t u = w & 0xDFDFDFDFu;
-  const uint32_t other = nonzero_bytes(u ^ 0x41414141u) & nonzero_bytes(u ^ 0x43434343u) &
-                         nonzero_bytes(u ^ 0x47474747u) & nonzero_bytes(u ^ 0x54545454u);
-  const uint32_t bb = ~other & 0x80808080u;
-  return ((bb >> 7) | (bb >> 14) | (bb >> 21) | (bb >> 28)) & 0xFu;
+  const uint32_t c = ((u >> 1) ^ (u >> 2)) & 0x03030303u;
+  const uint32_t letters = 0x54474341u;  // 'T' 'G' 'C' 'A'
+  const uint32_t canon = __builtin_amdgcn_perm(letters, letters, c);
+  const uint32_t ok = ~nonzero_bytes(canon ^ u) & 0x80808080u;
+  valid4 = ((ok >> 7) | (ok >> 14) | (ok >> 21) | (ok >> 28)) & 0xFu;
+  const uint32_t t = (c | (c >> 6)) & 0x000F000Fu;
+  code8 = (t | (t >> 12)) & 0xFFu;
 }
-// 2-bit codes of the four bytes of w, byte b in bits 2b..2b+1.
-__device__ __forceinline__ uint32_t codes8(uint32_t w) {
-  const uint32_t u = w & 0xDFDFDFDFu;
-  uint32_t t = ((u >> 1) ^ (u >> 2)) & 0x03030303u;
-  t = (t | (t >> 6)) & 0x000F000Fu;
-  return (t | (t >> 12)) & 0xFFu;
-}
+
 // Bit i of a 16-bit mask moved to bit 2i.
 __device__ __forceinline__ uint32_t spread16(uint32_t x) {
   x = (x | (x << 8)) & 0x00FF00FFu;
@@ -150,15 +150,19 @@ struct KmerTally {
 __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_t prev, uint4 cur,
                                             int64_t first, int64_t beg, int64_t end,
                                             KmerTally& tl) {
-  uint32_t vm = base_bits4(prev) | (base_bits4(cur.x) << 4) | (base_bits4(cur.y) << 8) |
-                (base_bits4(cur.z) << 12) | (base_bits4(cur.w) << 16);
+  uint32_t c8[5], v4[5];
+  decode_word(prev, c8[0], v4[0]);
+  decode_word(cur.x, c8[1], v4[1]);
+  decode_word(cur.y, c8[2], v4[2]);
+  decode_word(cur.z, c8[3], v4[3]);
+  decode_word(cur.w, c8[4], v4[4]);
+  uint32_t vm = v4[0] | (v4[1] << 4) | (v4[2] << 8) | (v4[3] << 12) | (v4[4] << 16);
   const int64_t lo = beg - first, hi = end - first;
   const uint32_t lo_c = (uint32_t)(lo < 0 ? 0 : (lo > 20 ? 20 : lo));
   const uint32_t hi_c = (uint32_t)(hi < 0 ? 0 : (hi > 20 ? 20 : hi));
   vm &= ((1u << hi_c) - 1u) & ~((1u << lo_c) - 1u);
-  const uint64_t c40 = (uint64_t)codes8(prev) | ((uint64_t)codes8(cur.x) << 8) |
-                       ((uint64_t)codes8(cur.y) << 16) | ((uint64_t)codes8(cur.z) << 24) |
-                       ((uint64_t)codes8(cur.w) << 32);
+  const uint64_t c40 = (uint64_t)c8[0] | ((uint64_t)c8[1] << 8) | ((uint64_t)c8[2] << 16) |
+                       ((uint64_t)c8[3] << 24) | ((uint64_t)c8[4] << 32);
   // The same 20 codes in reverse order (code t at bits 2(19-t)): the forward
   // code of a window (earliest base highest) is then one shift, and the
   // reverse complement one shift of the complemented codes.
