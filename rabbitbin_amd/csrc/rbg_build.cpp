@@ -10,6 +10,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 
 #include <sys/mman.h>
@@ -73,6 +76,70 @@ namespace {
 constexpr size_t kStageHalf = 32u << 20;   // bytes per staging half
 constexpr size_t kStageMin = 4u << 20;     // smaller copies go direct
 
+// Seven workers kept for the life of the process: a parallel section's parts
+// run on them and on the calling thread (creating threads per copy cost more
+// than small copies).  One section at a time.
+class CopyPool {
+ public:
+  static CopyPool& get() {
+    static CopyPool* pool = new CopyPool;  // never destroyed: workers sleep until exit
+    return *pool;
+  }
+  void run(size_t n, const std::function<void(size_t)>& f) {
+    if (n <= 1) {
+      if (n) f(0);
+      return;
+    }
+    std::lock_guard<std::mutex> section(call_mu_);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fn_ = &f;
+      n_ = n;
+      next_ = 0;
+      left_ = n;
+    }
+    cv_.notify_all();
+    drain();
+    std::unique_lock<std::mutex> g(mu_);
+    done_.wait(g, [&] { return left_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  CopyPool() {
+    for (int i = 0; i < 7; ++i) std::thread([this] { work(); }).detach();
+  }
+  // Runs parts until none is left to take.
+  void drain() {
+    for (;;) {
+      size_t t;
+      const std::function<void(size_t)>* fn;
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        if (next_ >= n_) return;
+        t = next_++;
+        fn = fn_;
+      }
+      (*fn)(t);
+      std::lock_guard<std::mutex> g(mu_);
+      if (--left_ == 0) done_.notify_all();
+    }
+  }
+  void work() {
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return next_ < n_; });
+      }
+      drain();
+    }
+  }
+  std::mutex call_mu_, mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(size_t)>* fn_ = nullptr;
+  size_t n_ = 0, next_ = 0, left_ = 0;
+};
+
 // dst = src (sub == 0) or dst[i] = src[i] - sub over 32-bit words, on up to
 // eight threads.
 void parallel_copy(void* dst, const void* src, size_t bytes, uint32_t sub) {
@@ -90,10 +157,7 @@ void parallel_copy(void* dst, const void* src, size_t bytes, uint32_t sub) {
     if (t + 1 == nt && (bytes & 3))  // a tail of bytes (sub is only used on whole words)
       std::memcpy((char*)dst + (bytes & ~(size_t)3), (const char*)src + (bytes & ~(size_t)3), bytes & 3);
   };
-  std::vector<std::thread> pool;
-  for (size_t t = 1; t < nt; ++t) pool.emplace_back(part, t);
-  part(0);
-  for (auto& th : pool) th.join();
+  CopyPool::get().run(nt, part);
 }
 
 rbg_status ensure_staging(rbg_ctx* ctx) {
@@ -129,6 +193,93 @@ rbg_status d2h_chunks(rbg_ctx* ctx, void* dst, const void* src, size_t bytes, ui
       RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[pc & 1]));
       parallel_copy((char*)dst + off, ctx->pin.p + (pc & 1) * H, len, sub);
     }
+  }
+  return RBG_OK;
+}
+
+// Several device -> host copies through the staging halves at once: the
+// parts are laid end to end, a half is filled with whole parts or pieces of
+// them, and the host drains half h - 1 while half h is in flight.  Each part's
+// length is a multiple of 4 bytes.
+struct D2HPart {
+  void* dst;
+  const void* src;
+  size_t bytes;
+  uint32_t sub;  // subtracted from every 32-bit word when nonzero
+};
+rbg_status d2h_parts(rbg_ctx* ctx, const D2HPart* parts, size_t np) {
+  hipStream_t st = ctx->stream;
+  RBG_TRY(ensure_staging(ctx));
+  const size_t H = ctx->pin.half;
+  struct Piece {
+    size_t part, off, len, at;  // bytes [off, off + len) of the part, at `at` in the half
+  };
+  std::vector<std::vector<Piece>> halves(1);
+  size_t used = 0;
+  for (size_t k = 0; k < np; ++k)
+    for (size_t off = 0; off < parts[k].bytes;) {
+      if (used == H) {
+        halves.emplace_back();
+        used = 0;
+      }
+      const size_t len = std::min(parts[k].bytes - off, H - used);
+      halves.back().push_back(Piece{k, off, len, used});
+      used += len;
+      off += len;
+    }
+  for (size_t h = 0; h <= halves.size(); ++h) {
+    if (h < halves.size()) {
+      for (const Piece& pc : halves[h])
+        RBG_TRY_HIP(ctx, hipMemcpyAsync(ctx->pin.p + (h & 1) * H + pc.at, (const char*)parts[pc.part].src + pc.off,
+                                        pc.len, hipMemcpyDeviceToHost, st));
+      RBG_TRY_HIP(ctx, hipEventRecord(ctx->pin.done[h & 1], st));
+    }
+    if (h > 0) {
+      RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[(h - 1) & 1]));
+      for (const Piece& pc : halves[h - 1])
+        parallel_copy((char*)parts[pc.part].dst + pc.off, ctx->pin.p + ((h - 1) & 1) * H + pc.at, pc.len,
+                      parts[pc.part].sub);
+    }
+  }
+  return RBG_OK;
+}
+
+// Several host -> device copies through the staging halves at once (parts
+// laid end to end as in d2h_parts): the host fills half h while half h - 1 is
+// in flight.  Returns once the staging is free again.
+struct H2DPart {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+rbg_status h2d_parts(rbg_ctx* ctx, const H2DPart* parts, size_t np) {
+  hipStream_t st = ctx->stream;
+  RBG_TRY(ensure_staging(ctx));
+  const size_t H = ctx->pin.half;
+  size_t used = 0, h = 0;
+  bool any = false;
+  auto flush_half = [&]() -> rbg_status {  // the half's pieces are queued; mark it
+    RBG_TRY_HIP(ctx, hipEventRecord(ctx->pin.done[h & 1], st));
+    ++h;
+    used = 0;
+    return RBG_OK;
+  };
+  for (size_t k = 0; k < np; ++k)
+    for (size_t off = 0; off < parts[k].bytes;) {
+      if (used == 0 && h >= 2) RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[h & 1]));  // its last DMA
+      const size_t len = std::min(parts[k].bytes - off, H - used);
+      char* stage = ctx->pin.p + (h & 1) * H + used;
+      parallel_copy(stage, (const char*)parts[k].src + off, len, 0);
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)parts[k].dst + off, stage, len, hipMemcpyHostToDevice, st));
+      any = true;
+      used += len;
+      off += len;
+      if (used == H) RBG_TRY(flush_half());
+    }
+  if (used) RBG_TRY(flush_half());
+  if (any) {
+    RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[0]));
+    RBG_TRY_HIP(ctx, hipEventSynchronize(ctx->pin.done[1]));
   }
   return RBG_OK;
 }
@@ -390,6 +541,18 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     work();
     for (auto& th : pool) th.join();
   }
+  // every assembly's new contigs in one pass through the staging
+  {
+    std::vector<H2DPart> seq_parts;
+    uint64_t at = 0;
+    for (uint32_t a = 0; a < count; ++a) {
+      const rbg_assembly_in& x = in[a].in;
+      const uint64_t nb = x.n ? x.offsets[x.n] - x.offsets[0] : 0;
+      if (nb) seq_parts.push_back(H2DPart{(char*)b->seqs.p + at, x.seqs + x.offsets[0], nb});
+      at += nb;
+    }
+    if (!seq_parts.empty()) RBG_TRY(h2d_parts(ctx, seq_parts.data(), seq_parts.size()));
+  }
   uint64_t seq0 = 0, sk = 0;
   for (uint32_t a = 0; a < count; ++a) {
     const rbg_assembly_in& x = in[a].in;
@@ -399,7 +562,6 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     const uint32_t n_old = h.n_old, n_new = x.n, S = x.S;
     // contigs to sketch
     const uint64_t nb = n_new ? x.offsets[n_new] - x.offsets[0] : 0;
-    if (nb) RBG_TRY(h2d_staged(ctx, (char*)b->seqs.p + seq0, x.seqs + x.offsets[0], nb));
     for (uint32_t r = 0; r < n_new; ++r) {
       offs[sk + r] = seq0 + (x.offsets[r] - x.offsets[0]);
       sk_rows[sk + r] = h.row0 + n_old + r;
@@ -848,10 +1010,11 @@ rbg_status assembly_result(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph
     if (!out->from || !out->to || !out->scomp || (h.S > 0 && !out->score))
       return ctx->fail(RBG_ERR_NOMEM, "host allocation of the edge list");
     // global rows -> ids local to the assembly on the way
-    RBG_TRY(d2h_u32_staged(ctx, out->from, (const uint32_t*)b->efrom.p + e0, ne, r0));
-    RBG_TRY(d2h_u32_staged(ctx, out->to, (const uint32_t*)b->eto.p + e0, ne, r0));
-    RBG_TRY(d2h_staged(ctx, out->scomp, (const float*)b->escomp.p + e0, ne * 4));
-    if (h.S > 0) RBG_TRY(d2h_staged(ctx, out->score, (const float*)b->escore.p + e0, ne * 4));
+    const D2HPart parts[4] = {{out->from, (const uint32_t*)b->efrom.p + e0, ne * 4, r0},
+                              {out->to, (const uint32_t*)b->eto.p + e0, ne * 4, r0},
+                              {out->scomp, (const float*)b->escomp.p + e0, ne * 4, 0},
+                              {out->score, (const float*)b->escore.p + e0, ne * 4, 0}};
+    if (ne) RBG_TRY(d2h_parts(ctx, parts, h.S > 0 ? 4 : 3));
   }
   if (diag) {
     if (diag->topk_id)
@@ -934,7 +1097,9 @@ rbg_status save_resident(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_p
     RBG_TRY_HIP(ctx, hipMemcpyAsync(r->mat.p, (uint16_t*)b->mat.p + b->desc_h[a].mat0, mb,
                                     hipMemcpyDeviceToDevice, st));
   }
-  RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+  // No wait: every later use of these buffers (the session's next build, or
+  // their release, which synchronises) comes after the copies on the stream.
+  RBG_TRY_HIP(ctx, hipGetLastError());
   r->nlist = nlist;
   r->n = n;
   r->S = S;
