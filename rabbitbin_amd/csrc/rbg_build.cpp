@@ -76,9 +76,12 @@ namespace {
 constexpr size_t kStageHalf = 32u << 20;   // bytes per staging half
 constexpr size_t kStageMin = 4u << 20;     // smaller copies go direct
 
-// Seven workers kept for the life of the process: a parallel section's parts
+// Workers kept for the life of the process (kCopyThreads - 1 of them, the
+// calling thread makes up the rest): a parallel section's parts
 // run on them and on the calling thread (creating threads per copy cost more
 // than small copies).  One section at a time.
+constexpr size_t kCopyThreads = 16;
+
 class CopyPool {
  public:
   static CopyPool& get() {
@@ -107,7 +110,7 @@ class CopyPool {
 
  private:
   CopyPool() {
-    for (int i = 0; i < 7; ++i) std::thread([this] { work(); }).detach();
+    for (size_t i = 0; i + 1 < kCopyThreads; ++i) std::thread([this] { work(); }).detach();
   }
   // Runs parts until none is left to take.
   void drain() {
@@ -141,9 +144,9 @@ class CopyPool {
 };
 
 // dst = src (sub == 0) or dst[i] = src[i] - sub over 32-bit words, on up to
-// eight threads.
+// kCopyThreads threads (one per 2 MiB).
 void parallel_copy(void* dst, const void* src, size_t bytes, uint32_t sub) {
-  const size_t nt = std::min<size_t>(8, std::max<size_t>(1, bytes >> 22));
+  const size_t nt = std::min<size_t>(kCopyThreads, std::max<size_t>(1, bytes >> 21));
   auto part = [&](size_t t) {
     const size_t words = bytes / 4;
     const size_t w0 = words * t / nt, w1 = words * (t + 1) / nt;
