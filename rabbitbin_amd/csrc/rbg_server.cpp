@@ -8,6 +8,11 @@
 // HBM budget.  A session's sketches, depth rows, int8 rows and candidate
 // lists stay resident after its build, so an extension uploads and sketches
 // only its new contigs and evaluates only the pairs that involve them.
+// Steps are pipelined over two workspaces: the worker uploads and builds a
+// step and saves its sessions' state (device copies on its own stream, so a
+// session's next request can join the very next step), then a finisher
+// thread copies the step's graphs to the host on a second stream while the
+// worker goes on with the next step.  Steps finish in order.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -71,17 +76,32 @@ void set_err(char* dst, size_t cap, const std::string& s) {
 }  // namespace
 
 struct rbg_server {
-  rbg_ctx* ctx = nullptr;
+  rbg_ctx* ctx = nullptr;   // the worker's stream: uploads and builds
+  rbg_ctx* ctx2 = nullptr;  // the finisher's stream: results and saved state
   rbg_server_config cfg{};
-  std::thread worker;
+  std::thread worker, finisher;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_ws, cv_jobs;
   std::deque<Request*> pending;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   std::unordered_map<uint64_t, std::unique_ptr<Session>> sessions;
   uint64_t next_req = 1, next_session = 1;
   bool stop = false;
-  rbg_asm ws;  // the step workspace (buffers reused across steps)
+  rbg_asm ws[2];                // step workspaces (buffers reused across steps)
+  hipEvent_t built[2] = {};     // a workspace's build is complete
+  bool ws_busy[2] = {false, false};
+  struct Job {                  // a built step for the finisher
+    std::vector<Request*> step;
+    int w = 0;
+    rbg_status st = RBG_OK;
+    std::string msg;
+    std::vector<rbg_status> saved;  // per request: its state was kept
+    std::vector<std::string> save_msg;
+    std::vector<uint32_t> n;        // per request: the session's contigs as of it
+    double t_start = 0, t_up = 0, t_built = 0, t_save = 0;
+  };
+  std::deque<Job> jobs;
+  bool worker_done = false;
   Clock::time_point t0 = Clock::now();
   rbg_server_stats stats{};
   std::string err;
@@ -92,10 +112,15 @@ struct rbg_server {
     for (auto& kv : sessions) s += kv.second->res.device_bytes();
     return s;
   }
+  size_t busy_ws_bytes() const {  // under mu
+    return (ws_busy[0] ? ws[0].device_bytes() : 0) + (ws_busy[1] ? ws[1].device_bytes() : 0);
+  }
   // Requests of the next step, FIFO, at most one per session, within the budget.
   void select(std::vector<Request*>& step);
-  void run_step(std::vector<Request*>& step);
+  void build_step(std::vector<Request*>& step, int w);
+  void finish_step(Job& job);
   void loop();
+  void finish_loop();
   void finish(Request* r, rbg_status st, const std::string& msg);
   void drop_session_if_idle(Session* s);
 };
@@ -130,7 +155,7 @@ void rbg_server::select(std::vector<Request*>& step) {
   std::vector<AsmInput> ins;
   size_t growth = 0;
   uint64_t rows = 0;
-  const size_t resident = resident_bytes();
+  const size_t resident = resident_bytes() + busy_ws_bytes();
   for (auto it = pending.begin(); it != pending.end();) {
     Request* r = *it;
     if (step.size() >= cfg.max_step_requests) break;
@@ -182,67 +207,100 @@ void rbg_server::select(std::vector<Request*>& step) {
   }
 }
 
-void rbg_server::run_step(std::vector<Request*>& step) {
+void rbg_server::build_step(std::vector<Request*>& step, int w) {
   const rbg_graph_params& P = cfg.params;
   const uint32_t A = (uint32_t)step.size();
   std::vector<AsmInput> ins(A);
-  std::vector<Session*> ss(A);
+  size_t others;
   {
     std::lock_guard<std::mutex> lk(mu);
-    for (uint32_t k = 0; k < A; ++k) ss[k] = sessions.at(step[k]->session).get();
-  }
-  for (uint32_t k = 0; k < A; ++k) {
-    ins[k].in = step[k]->in;
-    ins[k].src = step[k]->extend ? &ss[k]->res : nullptr;
+    for (uint32_t k = 0; k < A; ++k) {
+      Session* se = sessions.at(step[k]->session).get();
+      ins[k].in = step[k]->in;
+      ins[k].src = step[k]->extend ? &se->res : nullptr;
+    }
+    others = resident_bytes() + (ws_busy[1 - w] ? ws[1 - w].device_bytes() : 0);
   }
   // The workspace holds the largest step so far: give it back when the
   // budget needs the room.
-  size_t resident;
+  if (others + ws[w].device_bytes() > cfg.hbm_budget) ws[w].release_buffers();
+  Job job;
+  job.step = step;
+  job.w = w;
+  job.t_start = now();
+  rbg_status st = batch_upload(ctx, &ws[w], ins.data(), A, P.min_cv);
+  job.t_up = now();
+  if (st == RBG_OK) st = run_build(ctx, &ws[w], &P, false, false, nullptr);
+  if (st == RBG_OK && hipEventRecord(built[w], ctx->stream) != hipSuccess) st = ctx->fail(RBG_ERR_HIP, "event");
+  job.t_built = now();
+  job.st = st;
+  if (st != RBG_OK) job.msg = ctx->err;
+  // the sessions' state, on this stream: the next step that extends a session
+  // reads it after these copies
+  job.saved.assign(A, st);
+  job.save_msg.assign(A, std::string());
+  job.n.assign(A, 0);
+  if (st == RBG_OK) {
+    std::lock_guard<std::mutex> lk(mu);  // resident sizes are read under mu
+    for (uint32_t k = 0; k < A; ++k) {
+      Session* se = sessions.at(step[k]->session).get();
+      job.saved[k] = save_resident(ctx, &ws[w], k, &P, &se->res);
+      if (job.saved[k] != RBG_OK) job.save_msg[k] = ctx->err;
+      job.n[k] = ws[w].asms[k].n;
+    }
+  }
+  job.t_save = now();
   {
     std::lock_guard<std::mutex> lk(mu);
-    resident = resident_bytes();
+    jobs.push_back(std::move(job));
   }
-  if (resident + ws.device_bytes() > cfg.hbm_budget) ws.release_buffers();
-  const double t_start = now();
-  rbg_status st = batch_upload(ctx, &ws, ins.data(), A, P.min_cv);
-  const double t_up = now();
-  if (st == RBG_OK) st = run_build(ctx, &ws, &P, false, false, nullptr);
-  const double t_built = now();
+  cv_jobs.notify_one();
+}
+
+void rbg_server::finish_step(Job& job) {
+  const rbg_graph_params& P = cfg.params;
+  const uint32_t A = (uint32_t)job.step.size();
+  rbg_asm& b = ws[job.w];
+  rbg_status st = job.st;
+  if (st == RBG_OK && hipEventSynchronize(built[job.w]) != hipSuccess) st = RBG_ERR_HIP;
   std::vector<rbg_status> rs(A, st);
-  std::vector<std::string> msg(A, st == RBG_OK ? std::string() : ctx->err);
+  std::vector<std::string> msg(A, st == RBG_OK ? std::string() : (job.msg.empty() ? "build failed" : job.msg));
   double t_res = 0, t_save = 0;
   if (st == RBG_OK)
     for (uint32_t k = 0; k < A; ++k) {
-      Request* r = step[k];
-      const AsmHost& h = ws.asms[k];
+      Request* r = job.step[k];
+      const AsmHost& h = b.asms[k];
       const bool incr = h.incr || h.dense_incr;
       r->info.incremental = incr ? 1 : 0;
       r->info.pass_pairs = incr ? (uint64_t)h.n * (h.n - 1) / 2 - (uint64_t)h.n_old * (h.n_old - 1) / 2
                                 : (uint64_t)h.n * (h.n - 1) / 2;
-      const double t0 = now();
-      rs[k] = assembly_result(ctx, &ws, k, &P, false, &r->result, nullptr);
-      const double t1 = now();
-      if (rs[k] == RBG_OK) rs[k] = save_resident(ctx, &ws, k, &P, &ss[k]->res);
-      t_res += t1 - t0;
-      t_save += now() - t1;
-      if (rs[k] != RBG_OK) {
-        msg[k] = ctx->err;
-        rbg_graph_result_free(&r->result);
+      const double t0s = now();
+      rs[k] = assembly_result(ctx2, &b, k, &P, false, &r->result, nullptr);
+      t_res += now() - t0s;
+      if (rs[k] != RBG_OK) msg[k] = ctx2->err;
+      if (rs[k] == RBG_OK && job.saved[k] != RBG_OK) {
+        rs[k] = job.saved[k];
+        msg[k] = job.save_msg[k];
       }
+      if (rs[k] != RBG_OK) rbg_graph_result_free(&r->result);
     }
+  t_save = job.t_save - job.t_built;
+  // the copies read the workspace: done before it takes the next step
+  (void)hipStreamSynchronize(ctx2->stream);
   const double t_end = now();
   std::lock_guard<std::mutex> lk(mu);
   stats.steps += 1;
-  stats.busy_ms += 1e3 * (t_end - t_start);
-  stats.upload_ms += 1e3 * (t_up - t_start);
-  stats.build_ms += 1e3 * (t_built - t_up);
+  stats.busy_ms += 1e3 * ((job.t_save - job.t_start) + t_res);
+  stats.upload_ms += 1e3 * (job.t_up - job.t_start);
+  stats.build_ms += 1e3 * (job.t_built - job.t_up);
   stats.result_ms += 1e3 * t_res;
   stats.save_ms += 1e3 * t_save;
+  (void)t_end;
   for (uint32_t k = 0; k < A; ++k) {
-    Request* r = step[k];
+    Request* r = job.step[k];
     r->info.step = stats.steps;
     r->info.step_requests = A;
-    r->info.n = ss[k]->res.n ? ss[k]->res.n : r->info.n;
+    if (job.n.size() == A && job.n[k]) r->info.n = job.n[k];  // later steps may have grown the session
     if (rs[k] == RBG_OK) {
       stats.requests += 1;
       stats.graph_pairs += (uint64_t)r->info.n * (r->info.n - 1) / 2;
@@ -250,7 +308,10 @@ void rbg_server::run_step(std::vector<Request*>& step) {
     }
     finish(r, rs[k], msg[k]);
   }
-  stats.running = 0;
+  stats.running -= std::min<uint32_t>(stats.running, A);
+  ws_busy[job.w] = false;
+  cv_ws.notify_all();
+  cv_work.notify_all();
   cv_done.notify_all();
 }
 
@@ -258,27 +319,49 @@ void rbg_server::loop() {
   (void)hipSetDevice(ctx->device);
   for (;;) {
     std::vector<Request*> step;
+    int w = 0;
     {
       std::unique_lock<std::mutex> lk(mu);
       cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+      if (!stop) cv_ws.wait(lk, [&] { return stop || !ws_busy[0] || !ws_busy[1]; });
       if (stop) {
         while (!pending.empty()) {
           Request* r = pending.front();
           pending.pop_front();
           finish(r, RBG_ERR_ARG, "server stopped");
         }
+        worker_done = true;
+        cv_jobs.notify_all();
         cv_done.notify_all();
         return;
       }
+      w = ws_busy[0] ? 1 : 0;
       select(step);
       if (step.empty()) {
         cv_done.notify_all();  // requests may have been failed by admission
+        // wait for a finished step (its sessions rejoin) or for new requests
         if (!pending.empty()) cv_work.wait_for(lk, std::chrono::milliseconds(5));
         continue;
       }
-      stats.running = (uint32_t)step.size();
+      ws_busy[w] = true;
+      stats.running += (uint32_t)step.size();
     }
-    run_step(step);
+    build_step(step, w);
+  }
+}
+
+void rbg_server::finish_loop() {
+  (void)hipSetDevice(ctx->device);
+  for (;;) {
+    Job job;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_jobs.wait(lk, [&] { return !jobs.empty() || worker_done; });
+      if (jobs.empty()) return;
+      job = std::move(jobs.front());
+      jobs.pop_front();
+    }
+    finish_step(job);
   }
 }
 
@@ -308,27 +391,40 @@ rbg_status rbg_server_start(int device, const rbg_server_config* cfg, rbg_server
     return RBG_ERR_ARG;
   auto s = std::make_unique<rbg_server>();
   RBG_TRY(rbg_open(device, &s->ctx));
+  if (rbg_open(device, &s->ctx2) != RBG_OK) {
+    rbg_close(s->ctx);
+    return RBG_ERR_HIP;
+  }
+  for (auto& e : s->built)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      rbg_close(s->ctx2);
+      rbg_close(s->ctx);
+      return RBG_ERR_HIP;
+    }
   s->cfg = *cfg;
   s->cfg.params.flags = 0;
   if (s->cfg.hbm_budget == 0) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      rbg_close(s->ctx2);
       rbg_close(s->ctx);
       return RBG_ERR_HIP;
     }
     s->cfg.hbm_budget = free_b / 10 * 9;
   }
-  s->ws.sched = s->cfg.schedule;
+  s->ws[0].sched = s->ws[1].sched = s->cfg.schedule;
   s->stats.budget = s->cfg.hbm_budget;
   // the sketch tables of the configuration, before any request is timed
   rbg_ctx::Tables* t;
   const rbg_status st = get_tables(s->ctx, P.m, P.pmh_seed, &t);
   if (st != RBG_OK) {
+    rbg_close(s->ctx2);
     rbg_close(s->ctx);
     return st;
   }
   rbg_server* raw = s.release();
   raw->worker = std::thread([raw] { raw->loop(); });
+  raw->finisher = std::thread([raw] { raw->finish_loop(); });
   *out = raw;
   return RBG_OK;
 }
@@ -340,12 +436,17 @@ void rbg_server_stop(rbg_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
-  if (s->worker.joinable()) s->worker.join();
+  s->cv_ws.notify_all();
+  if (s->worker.joinable()) s->worker.join();  // then the finisher drains the built steps
+  if (s->finisher.joinable()) s->finisher.join();
   (void)hipSetDevice(s->ctx->device);
   for (auto& kv : s->reqs) rbg_graph_result_free(&kv.second->result);
   s->reqs.clear();
   s->sessions.clear();
-  s->ws.release_buffers();
+  for (auto& w : s->ws) w.release_buffers();
+  for (auto& e : s->built)
+    if (e) (void)hipEventDestroy(e);
+  rbg_close(s->ctx2);
   rbg_close(s->ctx);
   delete s;
 }
@@ -490,7 +591,7 @@ rbg_status rbg_server_get_stats(rbg_server* s, rbg_server_stats* out) {
   out->sessions = s->sessions.size();
   out->pending = s->pending.size();
   out->resident_bytes = s->resident_bytes();
-  out->workspace_bytes = s->ws.device_bytes();
+  out->workspace_bytes = s->ws[0].device_bytes() + s->ws[1].device_bytes();
   return RBG_OK;
 }
 
