@@ -3,6 +3,8 @@ session's whole assembly as of that request (contigs of earlier requests
 first), whether the request shared its step with others (the mix) or ran
 alone; extensions of fused sessions evaluate only the pairs with a new contig
 and merge the session's resident candidate lists."""
+import time
+
 import numpy as np
 import pytest
 
@@ -160,3 +162,17 @@ def test_server_budget_and_errors():
         assert srv.stats()["sessions"] == 1  # the failed session stays until released
         srv.release(sb)
         assert srv.stats()["sessions"] == 0
+
+
+def test_server_stop_while_busy():
+    """Stopping with requests queued, on the device and being copied out
+    returns promptly (the worker and the finisher both drain), after a request
+    that finished before the stop came out whole."""
+    parts = [_part(3000 + 700 * k, 30, 10, 700 + k) for k in range(8)]
+    srv = api.Server(0, max_step_requests=2)
+    reqs = [srv.submit(*p)[1] for p in parts]
+    g, info = srv.wait(reqs[0], timeout=120)
+    assert info["status"] == 0 and g["n_edges"] > 0 and info["n"] == 3000
+    t0 = time.time()
+    srv.stop()
+    assert time.time() - t0 < 60
