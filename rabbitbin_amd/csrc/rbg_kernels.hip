@@ -642,6 +642,35 @@ __device__ __forceinline__ uint32_t wave_atomic_slot(uint32_t* __restrict__ ctr,
   return slot;
 }
 
+// Like wave_atomic_slot, for returning slots: the lanes are first grouped by
+// key with ballots (no memory traffic), then every group's leader adds its
+// group's size in one atomic instruction, so the wave waits for one round
+// trip instead of one per group.  Lanes with equal keys get consecutive slots.
+// All lanes of the wave must call it.
+struct WaveSlot {
+  uint32_t leader = 0, rank = 0, count = 0;  // count: on leader lanes
+};
+__device__ __forceinline__ WaveSlot wave_group(uint32_t key, bool active) {
+  const uint32_t ln = lane_id();
+  WaveSlot g;
+  uint64_t pending = __ballot(active);
+  while (pending != 0) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t k = __shfl(key, leader);
+    const uint64_t grp = __ballot(active && key == k);
+    if (active && key == k) {
+      g.leader = (uint32_t)leader;
+      g.rank = lanes_below(grp);
+      if ((int)ln == leader) g.count = (uint32_t)__popcll(grp);
+    }
+    pending &= ~grp;
+  }
+  return g;
+}
+__device__ __forceinline__ uint32_t wave_slot_finish(uint32_t base, const WaveSlot& g) {
+  return (uint32_t)__shfl(base, (int)g.leader) + g.rank;
+}
+
 // Gated pairs are packed as (i << 40) | (j << 16) | count; a candidate that
 // the exact gate rejects is overwritten with kDeadPair.
 constexpr uint64_t kDeadPair = ~0ull;
@@ -814,20 +843,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6
 // Scatter each gated pair into both rows' candidate lists: entry = (cnt<<32)|id.
 // Candidates come in abundance-ordered clusters, so a wave often holds the
 // same row several times: the slots are claimed with wave-aggregated atomics.
-__device__ __forceinline__ void scatter_one(const uint64_t* __restrict__ surv, uint64_t t, uint64_t ns,
-                                            const uint64_t* __restrict__ off,
+// One survivor per lane (e = kDeadPair: none); `e` was loaded by the caller
+// one iteration ahead.  The partners' list offsets are loaded before the
+// slot atomics, so the two round trips overlap.
+__device__ __forceinline__ void scatter_one(uint64_t e, const uint64_t* __restrict__ off,
                                             uint32_t* __restrict__ cursor,
                                             uint64_t* __restrict__ list) {
-  const uint64_t e = t < ns ? surv[t] : kDeadPair;
   const bool live = e != kDeadPair;
   uint32_t i = 0, j = 0, c = 0;
   if (live) unpack_pair(e, i, j, c);
-  const uint32_t si = wave_atomic_slot(cursor, i, live);
-  const uint32_t sj = wave_atomic_slot(cursor, j, live);
+  uint64_t oi = 0, oj = 0;
+  if (live) {
+    oi = off[i];
+    oj = off[j];
+  }
+  // both keys' groups first, then both atomics in flight together
+  const WaveSlot gi = wave_group(i, live), gj = wave_group(j, live);
+  uint32_t bi = 0, bj = 0;
+  if (gi.count) bi = atomicAdd(&cursor[i], gi.count);
+  if (gj.count) bj = atomicAdd(&cursor[j], gj.count);
+  const uint32_t si = wave_slot_finish(bi, gi), sj = wave_slot_finish(bj, gj);
   if (live) {
     const uint64_t hi = (uint64_t)c << 32;
-    list[off[i] + si] = hi | j;
-    list[off[j] + sj] = hi | i;
+    list[oi + si] = hi | j;
+    list[oj + sj] = hi | i;
   }
 }
 
@@ -839,8 +878,15 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict
   const uint64_t ns = min((uint64_t)*ncand, cap);
   uint64_t s0, s1;
   xcd_segment(ns, s0, s1);
-  for (uint64_t tw = s0 + (threadIdx.x & ~63u); tw < s1; tw += 256)  // wave-uniform loop
-    scatter_one(surv, tw + (threadIdx.x & 63u), s1, off, cursor, list);
+  const uint64_t ln = threadIdx.x & 63u;
+  uint64_t tw = s0 + (threadIdx.x & ~63u);
+  uint64_t e = tw + ln < s1 ? surv[tw + ln] : kDeadPair;
+  for (; tw < s1; tw += 256) {  // wave-uniform loop; the next survivor is loaded ahead
+    const uint64_t tn = tw + 256 + ln;
+    const uint64_t en = tn < s1 ? surv[tn] : kDeadPair;
+    scatter_one(e, off, cursor, list);
+    e = en;
+  }
 }
 
 // ---------------------------------------------------------------------------
