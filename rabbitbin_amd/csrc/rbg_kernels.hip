@@ -4,7 +4,11 @@
 //   seqs     contig bytes back to back, offsets[n+1]
 //   wx       n x mpad  u8   winning element (0..135) per register, 0xFF unset
 //   planes   n x 8W    u32  register classes bit-sliced: word (w, p) holds bit
-//                           p of the class byte of registers 32w..32w+31.
+//                           p of the class byte of registers 32w..32w+31, at
+//                           plane_word(w, p): planes 0-3 of every group w,
+//                           then planes 4-7 (a 16-byte quarter group per
+//                           (w, half), so eight lanes reading eight groups'
+//                           same half read one contiguous 128-byte line).
 //                           Unset registers and the pad registers >= m hold
 //                           0xFF; a class is < 136, so 0xFF never equals one.
 //   unit     n x 16NC  f32  centred, L2-normalised rank rows, zero padded
@@ -41,15 +45,25 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Column-side transform of one 32-register word group: clear plane 0 where
-// the column register is unset (all eight bits set).
-__device__ __forceinline__ void column_word(const uint32_t* __restrict__ src, uint32_t* dst) {
-  uint32_t all = src[0];
+// Position of word (group w, plane p) in a row of 8W plane words.
+__host__ __device__ constexpr uint32_t plane_word(uint32_t w, uint32_t p, uint32_t W) {
+  return (p >> 2) * 4u * W + 4u * w + (p & 3u);
+}
+
+// The eight plane words of group w of a row.
+__device__ __forceinline__ void load_word(const uint32_t* __restrict__ row, uint32_t w, uint32_t W,
+                                          uint32_t* dst) {
 #pragma unroll
-  for (int p = 1; p < 8; ++p) all &= src[p];
+  for (int p = 0; p < 8; ++p) dst[p] = row[plane_word(w, p, W)];
+}
+
+// Column-side transform of one 32-register word group, in place: clear plane
+// 0 where the column register is unset (all eight bits set).
+__device__ __forceinline__ void column_fix(uint32_t* b) {
+  uint32_t all = b[0];
 #pragma unroll
-  for (int p = 0; p < 8; ++p) dst[p] = src[p];
-  dst[0] &= ~all;
+  for (int p = 1; p < 8; ++p) all &= b[p];
+  b[0] &= ~all;
 }
 
 __device__ __forceinline__ uint32_t word_mismatch(const uint32_t* a, const uint32_t* b) {
@@ -65,9 +79,9 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
   uint32_t mism = 0;
   for (uint32_t w = 0; w < W; ++w) {
     uint32_t a[8], b[8];
-#pragma unroll
-    for (int p = 0; p < 8; ++p) a[p] = ra[8 * w + p];
-    column_word(rb + 8 * w, b);
+    load_word(ra, w, W, a);
+    load_word(rb, w, W, b);
+    column_fix(b);
     mism += __popc(word_mismatch(a, b));
   }
   return 32u * W - mism;
@@ -375,7 +389,7 @@ __global__ __launch_bounds__(256) void sketch_kernel(
       for (int p = 0; p < 8; ++p) {
         const uint64_t bits = __ballot((code >> p) & 1u);
         if (c < n && (ln == 0 || ln == 32) && w0 < W)
-          planes[orow * 8u * W + 8u * w0 + p] = ln == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
+          planes[orow * 8u * W + plane_word(w0, p, W)] = ln == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
       }
     }
   }
@@ -434,12 +448,19 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
   uint32_t b[8 * W];
   const uint32_t jj = j < n ? j : n - 1;
 #pragma unroll
-  for (int w = 0; w < W; ++w) column_word(rows + (size_t)jj * 8 * W + 8 * w, b + 8 * w);
+  for (int w = 0; w < W; ++w) {
+    load_word(rows + (size_t)jj * 8 * W, w, W, b + 8 * w);
+    column_fix(b + 8 * w);
+  }
   for (uint32_t r = r0; r < r1; ++r) {
-    const uint32_t* __restrict__ a = rows + (size_t)r * 8 * W;
+    const uint32_t* __restrict__ ra = rows + (size_t)r * 8 * W;
     uint32_t mism = 0;
 #pragma unroll
-    for (int w = 0; w < W; ++w) mism += __popc(word_mismatch(a + 8 * w, b + 8 * w));
+    for (int w = 0; w < W; ++w) {
+      uint32_t a[8];
+      load_word(ra, w, W, a);
+      mism += __popc(word_mismatch(a, b + 8 * w));
+    }
     if (j < n && j > r && j >= jmin) {
       const uint16_t c = (uint16_t)(32u * W - mism);
       out[(size_t)r * n + j] = c;
@@ -699,18 +720,17 @@ __device__ __forceinline__ void unpack_pair(uint64_t e, uint32_t& i, uint32_t& j
 // in every lane (IEEE addition is commutative, so which lane holds which
 // operand does not change a sum).  For the register compare, lane q holds
 // word groups q, q + 8, ... of both rows' planes (32 bytes each: one group's
-// 8 planes), so the eight lanes of a candidate read 256 contiguous bytes of a
-// row per pair of loads.  (Pieces of 16 bytes per lane, one whole line per
-// load with the two halves of a group in neighbouring lanes, measured slower:
-// the lane exchanges cost more than the fewer lines saved.)  All loads are
-// issued before the gate is known: nearly every candidate survives.
-// Survivors add 1 to deg[i] and deg[j].
+// 8 planes, in two 16-byte halves), so each load instruction of the eight
+// lanes of a candidate reads one contiguous 128-byte line of a row (the
+// same half of eight groups).  All loads are issued before the gate is
+// known: nearly every candidate survives.
+// Returns true on lane 0 of a survivor's eight, with its rows in ri, rj.
 template <int NC, int W>
-__device__ __forceinline__ void gate_compare_one(
+__device__ __forceinline__ bool gate_compare_one(
     uint64_t* __restrict__ cand, uint64_t t, uint64_t nc, uint32_t q,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
-    const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
+    const uint32_t* __restrict__ planes, uint32_t& ri, uint32_t& rj) {
   constexpr int GPL = (W + 7) / 8;  // word groups per lane
   const bool live = t < nc;
   const uint64_t e = live ? cand[t] : 0;
@@ -743,12 +763,12 @@ __device__ __forceinline__ void gate_compare_one(
     for (int x = 0; x < GPL; ++x) {
       const uint32_t w = 8u * x + q;
       if (w < (uint32_t)W) {
-        const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W + 8u * w);
-        const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W + 8u * w);
-        pa[2 * x] = wa[0];
-        pa[2 * x + 1] = wa[1];
+        const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W + 4u * w);
+        const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W + 4u * w);
+        pa[2 * x] = wa[0];             // planes 0-3 of group w
+        pa[2 * x + 1] = wa[W];         // planes 4-7
         pb[2 * x] = wb[0];
-        pb[2 * x + 1] = wb[1];
+        pb[2 * x + 1] = wb[W];
       }
     }
   }
@@ -791,10 +811,9 @@ __device__ __forceinline__ void gate_compare_one(
       if (8u * x + q < (uint32_t)W) {
         const uint32_t av[8] = {pa[2 * x].x, pa[2 * x].y, pa[2 * x].z, pa[2 * x].w,
                                 pa[2 * x + 1].x, pa[2 * x + 1].y, pa[2 * x + 1].z, pa[2 * x + 1].w};
-        const uint32_t bs[8] = {pb[2 * x].x, pb[2 * x].y, pb[2 * x].z, pb[2 * x].w,
-                                pb[2 * x + 1].x, pb[2 * x + 1].y, pb[2 * x + 1].z, pb[2 * x + 1].w};
-        uint32_t bv[8];
-        column_word(bs, bv);
+        uint32_t bv[8] = {pb[2 * x].x, pb[2 * x].y, pb[2 * x].z, pb[2 * x].w,
+                          pb[2 * x + 1].x, pb[2 * x + 1].y, pb[2 * x + 1].z, pb[2 * x + 1].w};
+        column_fix(bv);
         mism += __popc(word_mismatch(av, bv));
       }
     }
@@ -802,10 +821,10 @@ __device__ __forceinline__ void gate_compare_one(
   mism += __shfl_xor(mism, 1);
   mism += __shfl_xor(mism, 2);
   mism += __shfl_xor(mism, 4);
-  const bool add = live && q == 0 && pass;
-  wave_atomic_slot(deg, i, add);
-  wave_atomic_slot(deg, j, add);
   if (live && q == 0) cand[t] = pass ? (e & ~0xFFFFull) | (uint64_t)(32u * W - mism) : kDeadPair;
+  ri = i;
+  rj = j;
+  return live && q == 0 && pass;
 }
 
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup b on XCD b %
@@ -824,20 +843,51 @@ __device__ __forceinline__ void xcd_segment(uint64_t total, uint64_t& s0, uint64
 // Six waves per SIMD for S <= 32: the <2, 16> instance (m = 500) needs 81
 // VGPRs unconstrained (five waves) and fits 80 without spilling; the extra
 // wave of gathers in flight measured 5% faster.  Wider rows would spill.
+//
+// Survivors add 1 to deg[i] and deg[j].  Those atomics are executed at the
+// memory side, and on this architecture a wave's later loads cannot be waited
+// for without waiting for its earlier atomics (one vector-memory counter), so
+// atomics issued every step would put a memory-side round trip into every
+// step's gathers.  Each wave queues its survivors' rows in LDS instead and
+// adds them when the queue is nearly full and once at its end, where nothing
+// waits for them.
+constexpr uint32_t kDegQueue = 512;  // rows per wave
 template <int NC, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6 : 1))) void gate_compare_kernel(
     uint64_t* __restrict__ cand, uint64_t cap, const unsigned long long* __restrict__ ncand,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
     const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
+  __shared__ uint32_t dq[4][kDegQueue];
   const uint64_t nc = min((uint64_t)*ncand, cap);
   uint64_t s0, s1;
   xcd_segment(nc, s0, s1);
-  const uint32_t q = threadIdx.x & 7u;
+  const uint32_t q = threadIdx.x & 7u, ln = lane_id(), wv = threadIdx.x >> 6;
+  uint32_t dn = 0;  // queued rows (wave-uniform)
+  auto flush = [&]() {
+    __builtin_amdgcn_wave_barrier();  // rows written by other lanes
+    for (uint32_t t = 0; t < dn; t += 64)
+      wave_atomic_slot(deg, t + ln < dn ? dq[wv][t + ln] : 0u, t + ln < dn);
+    __builtin_amdgcn_wave_barrier();  // read before the queue is refilled
+    dn = 0;
+  };
   // wave-uniform loop over the segment: 8 candidates per wave, 32 per workgroup per step
-  for (uint64_t tw = s0 + ((threadIdx.x & ~63u) >> 3); tw < s1; tw += 32)
-    gate_compare_one<NC, W>(cand, tw + ((threadIdx.x & 63u) >> 3), s1, q, unit, anynz, desc,
-                            asm_of, planes, deg);
+  for (uint64_t tw = s0 + (wv << 3); tw < s1; tw += 32) {
+    uint32_t ri, rj;
+    const bool add = gate_compare_one<NC, W>(cand, tw + (ln >> 3), s1, q, unit, anynz, desc,
+                                             asm_of, planes, ri, rj);
+    const uint64_t mask = __ballot(add);
+    if (mask) {
+      if (dn > kDegQueue - 16) flush();  // a step queues at most 16 rows
+      if (add) {
+        const uint32_t at = dn + 2u * lanes_below(mask);
+        dq[wv][at] = ri;
+        dq[wv][at + 1] = rj;
+      }
+      dn += 2u * (uint32_t)__popcll(mask);
+    }
+  }
+  if (dn) flush();
 }
 
 // Scatter each gated pair into both rows' candidate lists: entry = (cnt<<32)|id.
