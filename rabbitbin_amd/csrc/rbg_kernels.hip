@@ -98,17 +98,18 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
 // per contig; the four are summed when the weights are formed.  A thread
 // takes a chunk plus the dword before it (the 3-base overlap of 4-mers that
 // end in the chunk) and decodes them four bytes at a time:
-//   u = byte & 0xDF folds a/c/g/t onto A/C/G/T; the 2-bit code is
-//   ((u >> 1) ^ (u >> 2)) & 3 (A 0, C 1, G 2, T 3); a byte is a base when u
-//   is one of the four letters (any other byte breaks every 4-mer covering
-//   it, as the reference's window reset does); bytes outside the contig are
-//   masked off.
+//   the 2-bit code is ((byte >> 1) ^ (byte >> 2)) & 3 (A 0, C 1, G 2, T 3,
+//   either case); a byte is a base when byte & 0xDF is the upper-case
+//   letter of its code (any other byte breaks every 4-mer covering it, as
+//   the reference's window reset does); bytes outside the contig count as
+//   non-bases.
 // A 4-mer ending at byte r is valid when bytes r-3..r all are; its window
 // holds codes r-3..r in bits 0..7 (earliest base lowest), so the forward code
 // is the window with its 2-bit groups reversed and the reverse complement is
-// ~window & 0xFF.  The histogram bin is the smaller of the two.  Then the
-// GC-normalised weights in FP64, in the reference's operation order
-// (src/rabbitbin.cpp:1491-1516), kept as 1/w per element.
+// ~window & 0xFF.  The canonical code is the smaller of the two (the
+// histograms hold it at slot 255 - code).  Then the GC-normalised weights in
+// FP64, in the reference's operation order (src/rabbitbin.cpp:1491-1516),
+// kept as 1/w per element.
 //
 // Phase 2: thread t owns registers k = t, t+256, ...; for each register the
 // winner is argmin_x fl((1/w_x) * T[x][k]) over present elements, ties to the
@@ -131,19 +132,22 @@ constexpr int kRankGroup = 8;  // proposal ranks loaded together in phase 2
 __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
   return (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
 }
-// One word of sequence: the 2-bit codes of its bytes (A, C, G, T = 0..3 in
-// either case; byte b in bits 2b..2b+1) and bit b set iff byte b is one of
-// A, C, G, T.  The code is read from bits 1-2 of the byte; v_perm_b32 maps
-// each code back to its upper-case letter, which must equal the byte.
-__device__ __forceinline__ void decode_word(uint32_t w, uint32_t& code8, uint32_t& valid4) {
-  const uint32_t u = w & 0xDFDFDFDFu;
-  const uint32_t c = ((u >> 1) ^ (u >> 2)) & 0x03030303u;
+// One word of sequence: returns the 2-bit codes of its bytes (A, C, G, T =
+// 0..3 in either case, read from bits 1-3 of the byte; byte b in bits
+// 2b..2b+1) and sets x = the word XOR the upper-case letters of those codes
+// (v_perm_b32 maps a code back to its letter), so byte b is one of A, C, G, T
+// exactly when byte b of x & 0xDF is zero.
+__device__ __forceinline__ uint32_t word_codes(uint32_t w, uint32_t& x) {
+  const uint32_t c = ((w >> 1) ^ (w >> 2)) & 0x03030303u;
   const uint32_t letters = 0x54474341u;  // 'T' 'G' 'C' 'A'
-  const uint32_t canon = __builtin_amdgcn_perm(letters, letters, c);
-  const uint32_t ok = ~nonzero_bytes(canon ^ u) & 0x80808080u;
-  valid4 = ((ok >> 7) | (ok >> 14) | (ok >> 21) | (ok >> 28)) & 0xFu;
+  x = w ^ __builtin_amdgcn_perm(letters, letters, c);
   const uint32_t t = (c | (c >> 6)) & 0x000F000Fu;
-  code8 = (t | (t >> 12)) & 0xFFu;
+  return (t | (t >> 12)) & 0xFFu;
+}
+// Bit b set iff byte b of the word is a base (from word_codes' x).
+__device__ __forceinline__ uint32_t word_valid4(uint32_t x) {
+  const uint32_t ok = ~nonzero_bytes(x & 0xDFDFDFDFu) & 0x80808080u;
+  return ((ok >> 7) | (ok >> 14) | (ok >> 21) | (ok >> 28)) & 0xFu;
 }
 
 // Bit i of a 16-bit mask moved to bit 2i.
@@ -159,43 +163,70 @@ struct KmerTally {
   uint32_t kmers = 0;
 };
 
-// 4-mers ending in the 16 bytes [16j, 16j+16); prev is the dword at 16j-4
-// and `first` the buffer position 16j-4, [beg, end) the contig.
+// Histogram slot of the 4-mer ending at chunk byte i.  c40 holds the codes
+// of the chunk's 20 stream bytes (the dword before the chunk, then its 16
+// bytes; code t at bits 2t..2t+1) and nr40 the complement of the same codes
+// in reverse order (code t at bits 2(19-t)).  With E the window read from
+// c40 (earliest base lowest) and F the window with the earliest base highest
+// (the reference's forward code), the reverse complement's code is 255 - E,
+// so the canonical code min(F, 255 - E) = 255 - max(255 - F, E): the slot is
+// max(E, 255 - F), bins stored in reverse order.
+__device__ __forceinline__ uint32_t window_slot(uint64_t c40, uint64_t nr40, uint32_t i) {
+  const uint32_t e = (uint32_t)(c40 >> (2 * (i + 1))) & 0xFFu;
+  const uint32_t nf = (uint32_t)(nr40 >> (2 * (15 - i))) & 0xFFu;
+  return e > nf ? e : nf;
+}
+
+// The 4-mers ending in the 16 bytes [16j, 16j+16): every lane counts all 16
+// windows (nearly every chunk is all bases and inside its contig), then a
+// lane whose chunk is not takes the invalid windows off again, from the same
+// codes, so they add nothing.  prev is the dword at 16j-4, `first` the buffer
+// position 16j-4 and [beg, end) the contig.
 __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_t prev, uint4 cur,
                                             int64_t first, int64_t beg, int64_t end,
                                             KmerTally& tl) {
-  uint32_t c8[5], v4[5];
-  decode_word(prev, c8[0], v4[0]);
-  decode_word(cur.x, c8[1], v4[1]);
-  decode_word(cur.y, c8[2], v4[2]);
-  decode_word(cur.z, c8[3], v4[3]);
-  decode_word(cur.w, c8[4], v4[4]);
-  uint32_t vm = v4[0] | (v4[1] << 4) | (v4[2] << 8) | (v4[3] << 12) | (v4[4] << 16);
-  const int64_t lo = beg - first, hi = end - first;
-  const uint32_t lo_c = (uint32_t)(lo < 0 ? 0 : (lo > 20 ? 20 : lo));
-  const uint32_t hi_c = (uint32_t)(hi < 0 ? 0 : (hi > 20 ? 20 : hi));
-  vm &= ((1u << hi_c) - 1u) & ~((1u << lo_c) - 1u);
-  const uint64_t c40 = (uint64_t)c8[0] | ((uint64_t)c8[1] << 8) | ((uint64_t)c8[2] << 16) |
-                       ((uint64_t)c8[3] << 24) | ((uint64_t)c8[4] << 32);
-  // The same 20 codes in reverse order (code t at bits 2(19-t)): the forward
-  // code of a window (earliest base highest) is then one shift, and the
-  // reverse complement one shift of the complemented codes.
+  uint32_t x[5];
+  const uint32_t c0 = word_codes(prev, x[0]), c1 = word_codes(cur.x, x[1]),
+                 c2 = word_codes(cur.y, x[2]), c3 = word_codes(cur.z, x[3]),
+                 c4 = word_codes(cur.w, x[4]);
+  const uint64_t c40 = (uint64_t)(c0 | (c1 << 8) | (c2 << 16) | (c3 << 24)) | ((uint64_t)c4 << 32);
+  // the same 20 codes in reverse order, complemented
   uint64_t r40 = (uint64_t)__builtin_bitreverse64(c40) >> 24;  // each code's two bits swapped
   r40 = ((r40 >> 1) & 0x5555555555555555ull) | ((r40 & 0x5555555555555555ull) << 1);
-  const uint64_t n40 = ~c40;
-  // bit i: the 4-mer ending at chunk byte i (buffer bytes 16j+i-3 .. 16j+i) is valid
-  const uint32_t km = ((vm & (vm >> 1) & (vm >> 2) & (vm >> 3)) >> 1) & 0xFFFFu;
+  const uint64_t nr40 = ~r40;
+  // Window i = 4a + p: E is byte a of c40 >> 2(p+1), 255 - F byte 3-a of
+  // nr40 >> 2(3-p), so each window is one byte-select max.
+  const uint32_t clo = (uint32_t)c40, chi = (uint32_t)(c40 >> 32);
+  const uint32_t nlo = (uint32_t)nr40, nhi = (uint32_t)(nr40 >> 32);
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    if ((km >> i) & 1u) {
-      const uint32_t fwd = (uint32_t)(r40 >> (2 * (15 - i))) & 0xFFu;
-      const uint32_t rev = (uint32_t)(n40 >> (2 * (i + 1))) & 0xFFu;
-      atomicAdd(&hist[fwd <= rev ? fwd : rev], 1u);
+  for (int p = 0; p < 4; ++p) {
+    const uint32_t sv = __builtin_amdgcn_alignbit(chi, clo, 2 * (p + 1));
+    const uint32_t uv = __builtin_amdgcn_alignbit(nhi, nlo, 2 * (3 - p));
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const uint32_t e = (sv >> (8 * a)) & 0xFFu, nf = (uv >> (8 * (3 - a))) & 0xFFu;
+      atomicAdd(&hist[e > nf ? e : nf], 1u);
     }
+  }
+  const bool inside = first >= beg && first + 20 <= end;
+  const bool full = inside && ((x[0] | x[1] | x[2] | x[3] | x[4]) & 0xDFDFDFDFu) == 0;
+  uint32_t valid = 0x55555555u;  // bases among the 16 chunk bytes (2-bit spaced)
+  uint32_t km = 0xFFFFu;         // valid windows
+  if (!full) {
+    uint32_t vm = word_valid4(x[0]) | (word_valid4(x[1]) << 4) | (word_valid4(x[2]) << 8) |
+                  (word_valid4(x[3]) << 12) | (word_valid4(x[4]) << 16);
+    const int64_t lo = beg - first, hi = end - first;
+    const uint32_t lo_c = (uint32_t)(lo < 0 ? 0 : (lo > 20 ? 20 : lo));
+    const uint32_t hi_c = (uint32_t)(hi < 0 ? 0 : (hi > 20 ? 20 : hi));
+    vm &= ((1u << hi_c) - 1u) & ~((1u << lo_c) - 1u);
+    // bit i: the 4-mer ending at chunk byte i (buffer bytes 16j+i-3 .. 16j+i) is valid
+    km = ((vm & (vm >> 1) & (vm >> 2) & (vm >> 3)) >> 1) & 0xFFFFu;
+    for (uint32_t inv = ~km & 0xFFFFu; inv != 0; inv &= inv - 1u)
+      atomicSub(&hist[window_slot(c40, nr40, (uint32_t)__ffs(inv) - 1u)], 1u);
+    valid = spread16((vm >> 4) & 0xFFFFu);
   }
   tl.kmers += __popc(km);
   const uint32_t c32 = (uint32_t)(c40 >> 8);  // codes of the 16 chunk bytes
-  const uint32_t valid = spread16((vm >> 4) & 0xFFFFu);
 #pragma unroll
   for (uint32_t b = 0; b < 4; ++b) {
     const uint32_t eq = ~(c32 ^ (0x55555555u * b));
@@ -204,7 +235,7 @@ __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_
 }
 
 template <int CPB>
-__global__ __launch_bounds__(256) void sketch_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void sketch_kernel(
     const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
     uint32_t W, const double* __restrict__ sT, const uint8_t* __restrict__ sX,
     const uint8_t* __restrict__ cls, const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx,
@@ -295,11 +326,11 @@ __global__ __launch_bounds__(256) void sketch_kernel(
   for (uint32_t e = tid; e < CPB * kElemCount; e += 256) {
     const uint32_t q = e / kElemCount, x = e % kElemCount;
     const uint32_t c = c0 + q;
-    double wi = 0.0;
+    double wi = __builtin_huge_val();  // absent element: fl(inf * T) = inf never wins
     const uint32_t v = codes[x];
     uint32_t cnt = 0;
 #pragma unroll
-    for (int w = 0; w < kSketchWaves; ++w) cnt += hist[w][q][v];
+    for (int w = 0; w < kSketchWaves; ++w) cnt += hist[w][q][255u - v];  // reversed bins
     const uint32_t tot = total[q];
     if (c < n && cnt != 0 && tot != 0) {
       const uint64_t vb = (uint64_t)bases[q][0] + bases[q][1] + bases[q][2] + bases[q][3];
@@ -326,7 +357,8 @@ __global__ __launch_bounds__(256) void sketch_kernel(
       if (w > 0.0) wi = 1.0 / w;
     }
     winv[q][x] = wi;
-    if (wi > 0.0) atomicMin(&wmin_bits[q], (unsigned long long)__double_as_longlong(wi));
+    if (wi != __builtin_huge_val())
+      atomicMin(&wmin_bits[q], (unsigned long long)__double_as_longlong(wi));
   }
   __syncthreads();
   double wmin[CPB];
@@ -355,18 +387,21 @@ __global__ __launch_bounds__(256) void sketch_kernel(
           t[g] = sT[(size_t)r * m + k];
           x[g] = sX[(size_t)r * m + k];
         }
+        // Branch-free update, so that the LDS reads of several ranks are in
+        // flight together: an absent element has 1/w = inf, hence hv = inf,
+        // which neither beats nor ties best (best = inf only while bx = -1,
+        // and x > -1).
 #pragma unroll
         for (int g = 0; g < kRankGroup; ++g) {
+          double wq[CPB];
+#pragma unroll
+          for (int q = 0; q < CPB; ++q) wq[q] = winv[q][x[g]];
 #pragma unroll
           for (int q = 0; q < CPB; ++q) {
-            const double wv = winv[q][x[g]];
-            if (wv != 0.0) {
-              const double hv = wv * t[g];
-              if (hv < best[q] || (hv == best[q] && (int)x[g] < bx[q])) {
-                best[q] = hv;
-                bx[q] = (int)x[g];
-              }
-            }
+            const double hv = wq[q] * t[g];
+            const bool take = (hv < best[q]) | ((hv == best[q]) & ((int)x[g] < bx[q]));
+            best[q] = take ? hv : best[q];
+            bx[q] = take ? (int)x[g] : bx[q];
           }
         }
         bool done = true;
