@@ -350,7 +350,7 @@ size_t rbg_asm::device_bytes() const {
   const DevBuf* all[] = {&seqs, &offs, &sk_rows, &asm_of, &unit_pad, &raw_pad, &colnorm, &anynz,
                          &q8, &perm, &pi, &pj, &pcnt, &merge, &desc, &bscal, &svd, &svf, &level,
                          &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
-                         &mat, &cand, &counters, &deg, &cursor, &off, &list, &topk_id, &topk_cnt,
+                         &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt,
                          &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
                          &ecnt, &escomp, &escore, &scan_tmp, &bounds};
   size_t s = 0;
@@ -363,7 +363,7 @@ void rbg_asm::release_buffers() {
   DevBuf* all[] = {&seqs, &offs, &sk_rows, &asm_of, &unit_pad, &raw_pad, &colnorm, &anynz,
                    &q8, &perm, &pi, &pj, &pcnt, &merge, &desc, &bscal, &svd, &svf, &level,
                    &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
-                   &mat, &cand, &counters, &deg, &cursor, &off, &list, &topk_id, &topk_cnt,
+                   &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt,
                    &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
                    &ecnt, &escomp, &escore, &scan_tmp, &bounds};
   for (DevBuf* d : all) d->release();
@@ -749,7 +749,7 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   RBG_TRY_HIP(ctx, b->cand.ensure(std::max<uint64_t>(cap, 1) * 8));
   RBG_TRY_HIP(ctx, b->counters.ensure(16));
   RBG_TRY_HIP(ctx, b->deg.ensure((size_t)N * 4));
-  RBG_TRY_HIP(ctx, b->cursor.ensure((size_t)N * 4));
+  RBG_TRY_HIP(ctx, b->slot.ensure(std::max<uint64_t>(cap, 1) * 8));  // 2 list slots per candidate
   RBG_TRY_HIP(ctx, b->off.ensure((size_t)(N + 1) * 8));
   RBG_TRY_HIP(ctx, b->list.ensure(std::max<uint64_t>(2 * cap + b->list_extra, 1) * 8));
   // +16 ids: the mutual test reads whole 16-id segments of a row's list
@@ -803,8 +803,8 @@ size_t estimate_build_bytes(const AsmInput* in, uint32_t count, const rbg_graph_
   uint64_t s = bytes + 64 + (nsk + 1) * 12;                 // seqs, offs, sk_rows
   s += N * (2 + 1 + 2 * 16 * NC * 4 + kpad + 4);            // asm_of, anynz, unit/raw, q8, perm
   s += N * 8 * W * 4;                                       // planes
-  s += mat * 2 + cap * 8 + (2 * cap + extra) * 8;           // mat, cand, list
-  s += N * (4 + 4 + 8 + (uint64_t)K * 6 + 4 + 4 + FW * 4 + KW * 8 + 4 + 8);  // per-row arrays
+  s += mat * 2 + cap * 16 + (2 * cap + extra) * 8;          // mat, cand + slots, list
+  s += N * (4 + 8 + (uint64_t)K * 6 + 4 + 4 + FW * 4 + KW * 8 + 4 + 8);  // per-row arrays
   s += N * (uint64_t)K * (4 + 4 + 2 + 4 + 4);               // edges
   s += (uint64_t)count * (64 * 4 + (m + 1) * 14 + kCalibRandDev * 4 + 30000 * 12);
   return (size_t)s + (64u << 20);                           // small tables and slack
@@ -873,25 +873,28 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
   RBG_TRY(mark(6));
   RBG_TRY(mark(7));
   if (b->has_gate_items) {
-    // exact FP32 gate and register compare of the candidates, in place
+    // Exact FP32 gate and register compare of the candidates, in place; the
+    // survivors claim their list slots.  A served session's old rows start
+    // at their old list lengths, so the new entries follow the old ones.
+    const MergeDesc* md = (const MergeDesc*)b->merge.p;
     RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->deg.p, N, 0u, st));
-    RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->cursor.p, N, 0u, st));
+    RBG_TRY_HIP(ctx, launch_merge_deg(md, b->n_incr, b->merge_max_old, (uint32_t*)b->deg.p, st));
     RBG_TRY_HIP(ctx, launch_gate_compare((uint64_t*)b->cand.p, b->cand_cap, ncand,
                                          (const float*)b->unit_pad.p, b->NC,
                                          (const uint8_t*)b->anynz.p, desc, asm_of,
-                                         (const uint32_t*)b->planes.p, W, (uint32_t*)b->deg.p, st));
+                                         (const uint32_t*)b->planes.p, W, (uint32_t*)b->deg.p,
+                                         (uint32_t*)b->slot.p, st));
   }
   RBG_TRY(mark(8));
   if (b->has_gate_items) {
     const MergeDesc* md = (const MergeDesc*)b->merge.p;
-    RBG_TRY_HIP(ctx, launch_merge_deg(md, b->n_incr, b->merge_max_old, (uint32_t*)b->deg.p, st));
     size_t tmp = b->scan_tmp.bytes;
     RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)b->deg.p, (uint64_t*)b->off.p, N,
                                                b->scan_tmp.p, tmp, st));
     RBG_TRY_HIP(ctx, launch_merge_lists(md, b->n_incr, b->merge_max_old, (const uint64_t*)b->off.p,
-                                        (uint32_t*)b->cursor.p, (uint64_t*)b->list.p, st));
+                                        (uint64_t*)b->list.p, st));
     RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)b->cand.p, b->cand_cap, ncand,
-                                    (const uint64_t*)b->off.p, (uint32_t*)b->cursor.p,
+                                    (const uint32_t*)b->slot.p, (const uint64_t*)b->off.p,
                                     (uint64_t*)b->list.p, st));
   }
   RBG_TRY(mark(3));

@@ -96,17 +96,21 @@ hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, const AsmDesc* desc,
                             uint64_t* cand, unsigned long long* ncand, uint64_t cap,
                             hipStream_t st);
 // Exact gate + register compare of the min(*ncand, cap) candidates, in place
-// (rejected -> all ones, survivors -> count in the low 16 bits); deg counts
-// survivors per row.  unit rows are zero-padded to 16*NC floats.
+// (rejected -> all ones, survivors -> count in the low 16 bits).  Each
+// survivor t = (i, j) takes the next entry of row i's and of row j's list:
+// deg counts the entries per row and slot[2t], slot[2t+1] receive the
+// survivor's positions in row i's and row j's list (cap < 2^31).  unit rows
+// are zero-padded to 16*NC floats.
 hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
                                const float* unit, uint32_t NC, const uint8_t* anynz,
                                const AsmDesc* desc, const uint16_t* asm_of,
-                               const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st);
+                               const uint32_t* planes, uint32_t W, uint32_t* deg, uint32_t* slot,
+                               hipStream_t st);
 // Incremental assemblies: a served session's candidate lists from its earlier
 // build (CSR over its first n_old rows, local ids), merged into this build's
-// lists: merge_deg adds their lengths to deg (before the offsets are
-// scanned), merge_lists copies them to the head of each row's new list and
-// sets the row's cursor after them (before the scatter).
+// lists: merge_deg sets deg to their lengths (before the exact gate claims
+// the new pairs' slots after them), merge_lists copies them to the head of
+// each row's new list (after the offsets are scanned).
 struct MergeDesc {
   const uint64_t* loff;  // n_old + 1
   const uint64_t* list;  // entries (count << 32 | local id)
@@ -115,13 +119,14 @@ struct MergeDesc {
 hipError_t launch_merge_deg(const MergeDesc* md, uint32_t nmd, uint32_t max_old, uint32_t* deg,
                             hipStream_t st);
 hipError_t launch_merge_lists(const MergeDesc* md, uint32_t nmd, uint32_t max_old, const uint64_t* off,
-                              uint32_t* cursor, uint64_t* list, hipStream_t st);
+                              uint64_t* list, hipStream_t st);
 // Rows [row0, row0 + n) of a build's lists as a local CSR (loff: n + 1 offsets
 // from off[row0]; dst: entries with ids made local).
 hipError_t launch_extract_lists(const uint64_t* off, const uint64_t* list, uint32_t row0, uint32_t n,
                                 uint64_t* loff, uint64_t* dst, hipStream_t st);
+// Survivors into both rows' lists at off[row] + their slots (no atomics).
 hipError_t launch_scatter(const uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
-                          const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st);
+                          const uint32_t* slot, const uint64_t* off, uint64_t* list, hipStream_t st);
 
 // ---- stage (c)
 // Per-row top-K (sv desc, id asc) over the row's partners: regime 1 rows read

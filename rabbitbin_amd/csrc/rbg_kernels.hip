@@ -723,6 +723,28 @@ __device__ __forceinline__ WaveSlot wave_group(uint32_t key, bool active) {
   }
   return g;
 }
+// wave_group with a budget: the first kGroups keys are grouped, the lanes
+// left over form groups of one (their own leader).
+template <int kGroups>
+__device__ __forceinline__ WaveSlot wave_group_budget(uint32_t key, bool active) {
+  const uint32_t ln = lane_id();
+  WaveSlot g;
+  g.leader = ln;
+  g.count = active ? 1u : 0u;
+  uint64_t pending = __ballot(active);
+  for (int it = 0; pending != 0 && it < kGroups; ++it) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t k = __shfl(key, leader);
+    const uint64_t grp = __ballot(active && key == k);
+    if (active && key == k) {
+      g.leader = (uint32_t)leader;
+      g.rank = lanes_below(grp);
+      g.count = (int)ln == leader ? (uint32_t)__popcll(grp) : 0u;
+    }
+    pending &= ~grp;
+  }
+  return g;
+}
 __device__ __forceinline__ uint32_t wave_slot_finish(uint32_t base, const WaveSlot& g) {
   return (uint32_t)__shfl(base, (int)g.leader) + g.rank;
 }
@@ -879,30 +901,40 @@ __device__ __forceinline__ void xcd_segment(uint64_t total, uint64_t& s0, uint64
 // VGPRs unconstrained (five waves) and fits 80 without spilling; the extra
 // wave of gathers in flight measured 5% faster.  Wider rows would spill.
 //
-// Survivors add 1 to deg[i] and deg[j].  Those atomics are executed at the
-// memory side, and on this architecture a wave's later loads cannot be waited
-// for without waiting for its earlier atomics (one vector-memory counter), so
-// atomics issued every step would put a memory-side round trip into every
-// step's gathers.  Each wave queues its survivors' rows in LDS instead and
-// adds them when the queue is nearly full and once at its end, where nothing
-// waits for them.
-constexpr uint32_t kDegQueue = 512;  // rows per wave
+// A survivor (i, j) at candidate t takes the next entry of row i's list and
+// of row j's: deg[i] and deg[j] count the entries, and the values the atomics
+// return are the survivor's slots, slot[2t] in row i and slot[2t+1] in row j
+// (the scatter then needs no atomics).  Those atomics are executed at the
+// memory side, and on this architecture a wave's later loads cannot be
+// waited for without waiting for its earlier atomics (one vector-memory
+// counter), so atomics issued every step would put a memory-side round trip
+// into every step's gathers.  Each wave queues its survivors' (row, slot
+// index) in LDS instead and claims them when the queue is nearly full and
+// once at its end, one round trip per 64 entries.
+constexpr uint32_t kDegQueue = 512;  // entries per wave
 template <int NC, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6 : 1))) void gate_compare_kernel(
     uint64_t* __restrict__ cand, uint64_t cap, const unsigned long long* __restrict__ ncand,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
-    const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg) {
-  __shared__ uint32_t dq[4][kDegQueue];
+    const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg, uint32_t* __restrict__ slot) {
+  __shared__ uint2 dq[4][kDegQueue];
   const uint64_t nc = min((uint64_t)*ncand, cap);
   uint64_t s0, s1;
   xcd_segment(nc, s0, s1);
   const uint32_t q = threadIdx.x & 7u, ln = lane_id(), wv = threadIdx.x >> 6;
-  uint32_t dn = 0;  // queued rows (wave-uniform)
+  uint32_t dn = 0;  // queued entries (wave-uniform)
   auto flush = [&]() {
-    __builtin_amdgcn_wave_barrier();  // rows written by other lanes
-    for (uint32_t t = 0; t < dn; t += 64)
-      wave_atomic_slot(deg, t + ln < dn ? dq[wv][t + ln] : 0u, t + ln < dn);
+    __builtin_amdgcn_wave_barrier();  // entries written by other lanes
+    for (uint32_t t = 0; t < dn; t += 64) {
+      const bool act = t + ln < dn;
+      const uint2 en = act ? dq[wv][t + ln] : make_uint2(0u, 0u);
+      const WaveSlot g = wave_group_budget<8>(en.x, act);
+      uint32_t base = 0;
+      if (g.count) base = atomicAdd(&deg[en.x], g.count);
+      const uint32_t sl = wave_slot_finish(base, g);
+      if (act) slot[en.y] = sl;
+    }
     __builtin_amdgcn_wave_barrier();  // read before the queue is refilled
     dn = 0;
   };
@@ -913,11 +945,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6
                                              asm_of, planes, ri, rj);
     const uint64_t mask = __ballot(add);
     if (mask) {
-      if (dn > kDegQueue - 16) flush();  // a step queues at most 16 rows
+      if (dn > kDegQueue - 16) flush();  // a step queues at most 16 entries
       if (add) {
         const uint32_t at = dn + 2u * lanes_below(mask);
-        dq[wv][at] = ri;
-        dq[wv][at + 1] = rj;
+        const uint32_t t2 = 2u * (uint32_t)(tw + (ln >> 3));  // slot index of row i's entry
+        dq[wv][at] = make_uint2(ri, t2);
+        dq[wv][at + 1] = make_uint2(rj, t2 + 1u);
       }
       dn += 2u * (uint32_t)__popcll(mask);
     }
@@ -925,52 +958,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6
   if (dn) flush();
 }
 
-// Scatter each gated pair into both rows' candidate lists: entry = (cnt<<32)|id.
-// Candidates come in abundance-ordered clusters, so a wave often holds the
-// same row several times: the slots are claimed with wave-aggregated atomics.
-// One survivor per lane (e = kDeadPair: none); `e` was loaded by the caller
-// one iteration ahead.  The partners' list offsets are loaded before the
-// slot atomics, so the two round trips overlap.
-__device__ __forceinline__ void scatter_one(uint64_t e, const uint64_t* __restrict__ off,
-                                            uint32_t* __restrict__ cursor,
-                                            uint64_t* __restrict__ list) {
-  const bool live = e != kDeadPair;
-  uint32_t i = 0, j = 0, c = 0;
-  if (live) unpack_pair(e, i, j, c);
-  uint64_t oi = 0, oj = 0;
-  if (live) {
-    oi = off[i];
-    oj = off[j];
-  }
-  // both keys' groups first, then both atomics in flight together
-  const WaveSlot gi = wave_group(i, live), gj = wave_group(j, live);
-  uint32_t bi = 0, bj = 0;
-  if (gi.count) bi = atomicAdd(&cursor[i], gi.count);
-  if (gj.count) bj = atomicAdd(&cursor[j], gj.count);
-  const uint32_t si = wave_slot_finish(bi, gi), sj = wave_slot_finish(bj, gj);
-  if (live) {
-    const uint64_t hi = (uint64_t)c << 32;
-    list[oi + si] = hi | j;
-    list[oj + sj] = hi | i;
-  }
-}
-
+// Scatter each gated pair into both rows' candidate lists, entry =
+// (cnt << 32) | partner, at the slots the exact gate claimed for it.
 __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict__ surv, uint64_t cap,
                                                       const unsigned long long* __restrict__ ncand,
+                                                      const uint2* __restrict__ slot,
                                                       const uint64_t* __restrict__ off,
-                                                      uint32_t* __restrict__ cursor,
                                                       uint64_t* __restrict__ list) {
   const uint64_t ns = min((uint64_t)*ncand, cap);
-  uint64_t s0, s1;
-  xcd_segment(ns, s0, s1);
-  const uint64_t ln = threadIdx.x & 63u;
-  uint64_t tw = s0 + (threadIdx.x & ~63u);
-  uint64_t e = tw + ln < s1 ? surv[tw + ln] : kDeadPair;
-  for (; tw < s1; tw += 256) {  // wave-uniform loop; the next survivor is loaded ahead
-    const uint64_t tn = tw + 256 + ln;
-    const uint64_t en = tn < s1 ? surv[tn] : kDeadPair;
-    scatter_one(e, off, cursor, list);
-    e = en;
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < ns; t += (uint64_t)gridDim.x * 256) {
+    const uint64_t e = surv[t];
+    if (e == kDeadPair) continue;
+    uint32_t i, j, c;
+    unpack_pair(e, i, j, c);
+    const uint2 sl = slot[t];
+    const uint64_t hi = (uint64_t)c << 32;
+    list[off[i] + sl.x] = hi | j;
+    list[off[j] + sl.y] = hi | i;
   }
 }
 
@@ -978,11 +982,11 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t* __restrict
 // Incremental builds of a served session (fused regime).  The gated pairs of
 // the session's earlier rows are not evaluated again: their candidate lists,
 // kept from the earlier build with local ids (CSR), are merged into this
-// build's lists.  An old row's degree gains its old list length before the
-// offsets are scanned; its old entries then fill the head of its new list
-// (ids made global) and its cursor starts after them, so the scatter of the
-// new pairs appends.  Lists are sets: the selection does not depend on the
-// order of a row's entries.
+// build's lists.  An old row's degree starts at its old list length, before
+// the exact gate claims the slots of the new pairs, so those slots follow the
+// old entries; the old entries then fill the head of the row's new list (ids
+// made global).  Lists are sets: the selection does not depend on the order
+// of a row's entries.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void merge_deg_kernel(const MergeDesc* __restrict__ md,
                                                         uint32_t* __restrict__ deg) {
@@ -994,7 +998,6 @@ __global__ __launch_bounds__(256) void merge_deg_kernel(const MergeDesc* __restr
 // One wave per old row.
 __global__ __launch_bounds__(256) void merge_lists_kernel(const MergeDesc* __restrict__ md,
                                                           const uint64_t* __restrict__ off,
-                                                          uint32_t* __restrict__ cursor,
                                                           uint64_t* __restrict__ list) {
   const MergeDesc d = md[blockIdx.y];
   const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63u;
@@ -1004,7 +1007,6 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(const MergeDesc* __res
   uint64_t* __restrict__ dst = list + off[g];
   // entry = count << 32 | id: adding row0 (id + row0 < 2^24) only changes the id
   for (uint64_t t = s0 + ln; t < s1; t += 64) dst[t - s0] = d.list[t] + d.row0;
-  if (ln == 0) cursor[g] = (uint32_t)(s1 - s0);
 }
 
 // Rows [row0, row0 + n) of a build's lists as a local CSR: loff[r] =
@@ -2085,13 +2087,15 @@ hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, const AsmDesc* desc,
 hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
                                const float* unit, uint32_t NC, const uint8_t* anynz,
                                const AsmDesc* desc, const uint16_t* asm_of,
-                               const uint32_t* planes, uint32_t W, uint32_t* deg, hipStream_t st) {
+                               const uint32_t* planes, uint32_t W, uint32_t* deg, uint32_t* slot,
+                               hipStream_t st) {
   if (cap == 0) return hipSuccess;
+  if (cap > 0x7FFFFFFFull) return hipErrorInvalidValue;  // slot indices 2t are 32-bit
   // a multiple of the XCD count (xcd_segment)
   const dim3 grid((unsigned)std::max<uint64_t>(kXcds, std::min<uint64_t>((cap * 8 + 255) / 256, 65536) & ~7ull));
 #define RBG_GC(NCV, WV)                                                                            \
   hipLaunchKernelGGL((gate_compare_kernel<NCV, WV>), grid, dim3(256), 0, st, cand, cap, ncand, unit, \
-                     anynz, desc, asm_of, planes, deg)
+                     anynz, desc, asm_of, planes, deg, slot)
 #define RBG_GC_W(NCV)                   \
   switch (W) {                          \
     case 16: RBG_GC(NCV, 16); break;    \
@@ -2114,10 +2118,11 @@ hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long
 }
 
 hipError_t launch_scatter(const uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
-                          const uint64_t* off, uint32_t* cursor, uint64_t* list, hipStream_t st) {
+                          const uint32_t* slot, const uint64_t* off, uint64_t* list, hipStream_t st) {
   if (cap == 0) return hipSuccess;
-  const dim3 grid((unsigned)std::max<uint64_t>(kXcds, std::min<uint64_t>((cap + 255) / 256, 65536) & ~7ull));
-  hipLaunchKernelGGL(scatter_kernel, grid, dim3(256), 0, st, cand, cap, ncand, off, cursor, list);
+  const dim3 grid((unsigned)std::min<uint64_t>((cap + 255) / 256, 65536));
+  hipLaunchKernelGGL(scatter_kernel, grid, dim3(256), 0, st, cand, cap, ncand,
+                     reinterpret_cast<const uint2*>(slot), off, list);
   return hipGetLastError();
 }
 
@@ -2129,10 +2134,10 @@ hipError_t launch_merge_deg(const MergeDesc* md, uint32_t nmd, uint32_t max_old,
 }
 
 hipError_t launch_merge_lists(const MergeDesc* md, uint32_t nmd, uint32_t max_old, const uint64_t* off,
-                              uint32_t* cursor, uint64_t* list, hipStream_t st) {
+                              uint64_t* list, hipStream_t st) {
   if (nmd == 0 || max_old == 0) return hipSuccess;
   hipLaunchKernelGGL(merge_lists_kernel, dim3((max_old + 3) / 4, nmd), dim3(256), 0, st, md, off,
-                     cursor, list);
+                     list);
   return hipGetLastError();
 }
 
