@@ -1792,13 +1792,15 @@ __global__ __launch_bounds__(256) void baseline_tables_kernel(const uint32_t* __
 // Block-wide exclusive scan (256 threads; defined with the scan kernels).
 __device__ uint64_t block_scan_excl_u64(uint64_t v, uint64_t* wave_tot, uint64_t* total);
 
-// calib_converge over a table on_at[p] = contigs whose maxsim >= p / 1000.
-__device__ uint32_t calib_converge_dev(const uint32_t* on_at, double total, double coverage,
+// calib_converge over a table cov_at[p] = (contigs whose maxsim >= p / 1000)
+// / total, computed for every p beforehand in parallel, so the serial walk
+// only reads it.
+__device__ uint32_t calib_converge_dev(const double* cov_at, double coverage,
                                        const int32_t* rnd, uint32_t nrnd, uint32_t& ri) {
   uint32_t p = 999, pp = 1000;
   double pcov = 0.0;
   while (p > 50) {
-    const double cov = (double)on_at[p] / total;
+    const double cov = cov_at[p];
     if (cov >= coverage) {
       if (cov - coverage > coverage - pcov) p = pp;
       break;
@@ -1823,19 +1825,21 @@ __global__ __launch_bounds__(256) void calibrate_kernel(
   const uint32_t* __restrict__ rowmax = rowmax_all + desc[a].row0;  // regime 1 rows
   const double* __restrict__ svd = svd_all + (size_t)a * (m + 1);
   const float* __restrict__ svf = svf_all + (size_t)a * (m + 1);
-  const int32_t* __restrict__ rnd = rnd_all + (size_t)a * kCalibRandDev;
   const uint32_t nrnd = kCalibRandDev;
   const uint32_t* __restrict__ round_off = round_off_all + (size_t)a * 11;
   BuildScalars* __restrict__ bs = bs_all + a;
   __shared__ uint32_t hist[514];
-  __shared__ uint32_t on_at[1000];
+  __shared__ double cov_at[1000];
   __shared__ double val[513];
   __shared__ uint64_t wave_tot[4], tot;
-  __shared__ uint32_t ri_s;
+  // the rand() values, read one after another by the serial convergence walk
+  __shared__ int32_t rnd[kCalibRandDev];
+  __shared__ uint32_t ri_s, cmin_s;
   __shared__ double sum_p;
   __shared__ bool stop;
   const uint32_t tid = threadIdx.x;
   for (uint32_t c = tid; c <= m; c += 256) val[c] = regime == 1 ? svd[c] : (double)svf[c];
+  for (uint32_t q = tid; q < kCalibRandDev; q += 256) rnd[q] = rnd_all[(size_t)a * kCalibRandDev + q];
   if (tid == 0) {
     ri_s = 0;
     sum_p = 0.0;
@@ -1873,7 +1877,8 @@ __global__ __launch_bounds__(256) void calibrate_kernel(
       run += loc[k];
     }
     __syncthreads();
-    // on_at[p]: the first count whose maxsim reaches p / 1000, then its suffix
+    // cov_at[p]: the suffix count at the first count whose maxsim reaches
+    // p / 1000, over the contigs sampled
     for (uint32_t p = tid; p < 1000; p += 256) {
       const double cut = (double)p / 1000.;
       uint32_t lo = 0, hi = m + 1;
@@ -1884,13 +1889,12 @@ __global__ __launch_bounds__(256) void calibrate_kernel(
         else
           lo = mid + 1;
       }
-      on_at[p] = hist[lo];
+      cov_at[p] = (double)hist[lo] / (regime == 1 ? (double)n : 2500.0);
     }
     __syncthreads();
     if (tid == 0) {
       uint32_t ri = ri_s;
-      const double total = regime == 1 ? (double)n : 2500.0;
-      const uint32_t p = calib_converge_dev(on_at, total, coverage, rnd, nrnd, ri);
+      const uint32_t p = calib_converge_dev(cov_at, coverage, rnd, nrnd, ri);
       ri_s = ri;
       if (regime == 1) {
         sum_p = (double)p;
@@ -1905,18 +1909,17 @@ __global__ __launch_bounds__(256) void calibrate_kernel(
       }
     }
   }
+  if (tid == 0) cmin_s = m + 1;
+  __syncthreads();
+  const double sim_cutoff = regime == 1 ? sum_p : sum_p / 10.0;
+  const double cutoff = sim_cutoff / 1000.;
+  // cmin: the first count whose float similarity exceeds the cutoff
+  for (uint32_t c = tid; c <= m; c += 256)
+    if ((double)svf[c] > cutoff) atomicMin(&cmin_s, c);
   __syncthreads();
   if (tid == 0) {
-    const double sim_cutoff = regime == 1 ? sum_p : sum_p / 10.0;
-    const double cutoff = sim_cutoff / 1000.;
-    uint32_t cmin = m + 1;
-    for (uint32_t c = 0; c <= m; ++c)
-      if ((double)svf[c] > cutoff) {
-        cmin = c;
-        break;
-      }
     bs->sim_cutoff = sim_cutoff;
-    bs->cmin = cmin;
+    bs->cmin = cmin_s;
     bs->rand_used = ri_s;
   }
 }
