@@ -671,60 +671,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KC == 1 ? 6
   if (bn) flush();
 }
 
-// atomicAdd(&ctr[key], 1) for every active lane, returning each lane's old
-// value, with one atomic per distinct key among the first few groups of the
-// wave (lanes with equal keys get consecutive slots); lanes left after the
-// budget use their own atomics.  All lanes of the wave must call it.
-__device__ __forceinline__ uint32_t wave_atomic_slot(uint32_t* __restrict__ ctr, uint32_t key,
-                                                     bool active) {
-  const uint32_t ln = lane_id();
-  bool todo = active;
-  uint32_t slot = 0;
-  uint64_t pending = __ballot(todo);
-  for (int it = 0; pending != 0 && it < 8; ++it) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t k = __shfl(key, leader);
-    const uint64_t grp = __ballot(todo && key == k);
-    uint32_t base = 0;
-    if ((int)ln == leader) base = atomicAdd(&ctr[k], (uint32_t)__popcll(grp));
-    base = __shfl(base, leader);
-    if (todo && key == k) {
-      slot = base + lanes_below(grp);
-      todo = false;
-    }
-    pending &= ~grp;
-  }
-  if (todo) slot = atomicAdd(&ctr[key], 1u);
-  return slot;
-}
-
-// Like wave_atomic_slot, for returning slots: the lanes are first grouped by
+// Slots from a counter per key, atomicAdd(&ctr[key], 1) for every active
+// lane, without one returning atomic per lane: the lanes are first grouped by
 // key with ballots (no memory traffic), then every group's leader adds its
 // group's size in one atomic instruction, so the wave waits for one round
-// trip instead of one per group.  Lanes with equal keys get consecutive slots.
-// All lanes of the wave must call it.
+// trip.  The first kGroups keys are grouped; the lanes left over form groups
+// of one (their own leader).  Lanes with equal keys get consecutive slots:
+//   g = wave_group_budget<8>(key, active);
+//   if (g.count) base = atomicAdd(&ctr[key], g.count);
+//   slot = wave_slot_finish(base, g);
+// All lanes of the wave must call both.
 struct WaveSlot {
   uint32_t leader = 0, rank = 0, count = 0;  // count: on leader lanes
 };
-__device__ __forceinline__ WaveSlot wave_group(uint32_t key, bool active) {
-  const uint32_t ln = lane_id();
-  WaveSlot g;
-  uint64_t pending = __ballot(active);
-  while (pending != 0) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t k = __shfl(key, leader);
-    const uint64_t grp = __ballot(active && key == k);
-    if (active && key == k) {
-      g.leader = (uint32_t)leader;
-      g.rank = lanes_below(grp);
-      if ((int)ln == leader) g.count = (uint32_t)__popcll(grp);
-    }
-    pending &= ~grp;
-  }
-  return g;
-}
-// wave_group with a budget: the first kGroups keys are grouped, the lanes
-// left over form groups of one (their own leader).
 template <int kGroups>
 __device__ __forceinline__ WaveSlot wave_group_budget(uint32_t key, bool active) {
   const uint32_t ln = lane_id();
