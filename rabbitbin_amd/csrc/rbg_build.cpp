@@ -922,11 +922,10 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
   const uint64_t* ne = (const uint64_t*)b->row_off.p + N;
   RBG_TRY_HIP(ctx, launch_emit_write((const uint32_t*)b->topk_id.p, (const uint16_t*)b->topk_cnt.p,
                                      (const uint32_t*)b->topk_len.p, N, K, (const uint64_t*)b->keep.p,
-                                     (const uint64_t*)b->row_off.p, (uint32_t*)b->efrom.p,
-                                     (uint32_t*)b->eto.p, (uint16_t*)b->ecnt.p, st));
-  RBG_TRY_HIP(ctx, launch_cnt_to_sv((const uint16_t*)b->ecnt.p, b->edge_cap, ne,
-                                    (const uint32_t*)b->efrom.p, asm_of, m, (const float*)b->svf.p,
-                                    (float*)b->escomp.p, st));
+                                     (const uint64_t*)b->row_off.p, asm_of, m,
+                                     (const float*)b->svf.p, (uint32_t*)b->efrom.p,
+                                     (uint32_t*)b->eto.p, (uint16_t*)b->ecnt.p,
+                                     (float*)b->escomp.p, st));
   RBG_TRY(mark(4));
   RBG_TRY_HIP(ctx, launch_edge_weights_batch((const uint32_t*)b->efrom.p, (const uint32_t*)b->eto.p,
                                              (const float*)b->escomp.p, b->edge_cap, ne,

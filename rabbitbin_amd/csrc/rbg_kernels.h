@@ -147,16 +147,14 @@ hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
                             uint32_t nrows, uint32_t K, const BuildScalars* bs,
                             const AsmDesc* desc, const uint16_t* asm_of, const uint32_t* order,
                             uint64_t* keep, uint32_t* row_edges, hipStream_t st);
+// Pass 1 also writes each edge's sComp from its count (the row's assembly's svf).
 hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
                              const uint32_t* topk_len, uint32_t nrows, uint32_t K,
-                             const uint64_t* keep, const uint64_t* row_off, uint32_t* efrom,
-                             uint32_t* eto, uint16_t* ecnt, hipStream_t st);
+                             const uint64_t* keep, const uint64_t* row_off, const uint16_t* asm_of,
+                             uint32_t m, const float* svf, uint32_t* efrom, uint32_t* eto,
+                             uint16_t* ecnt, float* escomp, hipStream_t st);
 
 // ---- stage (d)
-// sComp of the min(*ne, cap) edges from their counts (the edge's assembly's svf).
-hipError_t launch_cnt_to_sv(const uint16_t* ecnt, uint64_t cap, const uint64_t* ne,
-                            const uint32_t* efrom, const uint16_t* asm_of, uint32_t m,
-                            const float* svf, float* scomp, hipStream_t st);
 // Edge weights of the min(*ne, cap) edges of a batch: unit/raw rows padded to
 // 16*NC floats, each edge with its assembly's S, w_comp and colnorm.
 hipError_t launch_edge_weights_batch(const uint32_t* efrom, const uint32_t* eto,

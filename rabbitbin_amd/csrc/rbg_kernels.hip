@@ -1449,25 +1449,32 @@ __global__ __launch_bounds__(256) void emit_mark_kernel(
   if (ln == 0) row_edges[i] = total;
 }
 
+// It also writes each edge's composition similarity sComp = svf[count] of the
+// row's assembly.
 __global__ __launch_bounds__(256) void emit_write_kernel(
     const uint32_t* __restrict__ topk_id, const uint16_t* __restrict__ topk_cnt,
     const uint32_t* __restrict__ topk_len, uint32_t n, uint32_t K, uint32_t KW,
     const uint64_t* __restrict__ keep, const uint64_t* __restrict__ row_off,
-    uint32_t* __restrict__ efrom, uint32_t* __restrict__ eto, uint16_t* __restrict__ ecnt) {
+    const uint16_t* __restrict__ asm_of, uint32_t m, const float* __restrict__ svf_all,
+    uint32_t* __restrict__ efrom, uint32_t* __restrict__ eto, uint16_t* __restrict__ ecnt,
+    float* __restrict__ escomp) {
   const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t ln = threadIdx.x & 63u;
   if (i >= n) return;
   const uint32_t len = topk_len[i];
   const uint64_t off = row_off[i];
+  const float* __restrict__ svf = svf_all + (size_t)asm_of[i] * (m + 1);
   uint64_t written = 0;
   for (uint32_t s = 0; s * 64 < len; ++s) {
     const uint64_t m = keep[(size_t)i * KW + s];
     if ((m >> ln) & 1ull) {
       const uint32_t t = s * 64 + ln;
       const uint64_t pos = off + written + lanes_below(m);
+      const uint16_t c = topk_cnt[(size_t)i * K + t];
       efrom[pos] = i;
       eto[pos] = topk_id[(size_t)i * K + t];
-      ecnt[pos] = topk_cnt[(size_t)i * K + t];
+      ecnt[pos] = c;
+      escomp[pos] = svf[c];
     }
     written += (uint32_t)__popcll(m);
   }
@@ -1650,17 +1657,6 @@ __global__ __launch_bounds__(64 * kEwWaves) void edge_weight_pad_kernel(
     __builtin_amdgcn_wave_barrier();  // the staged rows are read before the next edges' load
   }
 }
-
-__global__ void u16_to_sv_kernel(const uint16_t* __restrict__ c, uint64_t cap,
-                                 const uint64_t* __restrict__ ne_dev,
-                                 const uint32_t* __restrict__ efrom,
-                                 const uint16_t* __restrict__ asm_of, uint32_t m,
-                                 const float* __restrict__ svf, float* __restrict__ out) {
-  const uint64_t cnt = min(*ne_dev, cap);
-  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (uint64_t)gridDim.x * 256)
-    out[e] = svf[(size_t)asm_of[efrom[e]] * (m + 1) + c[e]];
-}
-
 
 // ---------------------------------------------------------------------------
 // Build scalars on the device, so a graph build needs no host round trip
@@ -2136,11 +2132,13 @@ hipError_t launch_emit_mark(const uint32_t* topk_id, const uint16_t* topk_cnt,
 
 hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
                              const uint32_t* topk_len, uint32_t n, uint32_t K, const uint64_t* keep,
-                             const uint64_t* row_off, uint32_t* efrom, uint32_t* eto,
-                             uint16_t* ecnt, hipStream_t st) {
+                             const uint64_t* row_off, const uint16_t* asm_of, uint32_t m,
+                             const float* svf, uint32_t* efrom, uint32_t* eto, uint16_t* ecnt,
+                             float* escomp, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(emit_write_kernel, dim3((n + 3) / 4), dim3(256), 0, st, topk_id, topk_cnt,
-                     topk_len, n, K, (K + 63) / 64, keep, row_off, efrom, eto, ecnt);
+                     topk_len, n, K, (K + 63) / 64, keep, row_off, asm_of, m, svf, efrom, eto, ecnt,
+                     escomp);
   return hipGetLastError();
 }
 
@@ -2176,15 +2174,6 @@ hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const
   return hipGetLastError();
 }
 
-hipError_t launch_cnt_to_sv(const uint16_t* ecnt, uint64_t cap, const uint64_t* ne,
-                            const uint32_t* efrom, const uint16_t* asm_of, uint32_t m,
-                            const float* svf, float* scomp, hipStream_t st) {
-  if (cap == 0) return hipSuccess;
-  const dim3 grid((unsigned)std::min<uint64_t>((cap + 255) / 256, 65536));
-  hipLaunchKernelGGL(u16_to_sv_kernel, grid, dim3(256), 0, st, ecnt, cap, ne, efrom, asm_of, m, svf,
-                     scomp);
-  return hipGetLastError();
-}
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n, void* tmp,
                                      size_t& tmp_bytes, hipStream_t st) {
   // n + 1 outputs; blocks cover 1024 inputs each (the last block also writes out[n]).
