@@ -292,7 +292,8 @@ void depth_features(const float* depth, size_t n, size_t S, double min_cv, float
         double ss = 0.0;
         for (size_t k = 0; k < S; ++k) {
           const double d = (double)row[k] - mean;
-          ss = std::fma(d, d, ss);  // fused in the reference build
+          // exact with or without fusion: ranks make d a multiple of 1/2
+          ss = std::fma(d, d, ss);
         }
         if (ss > 0.0) {
           const double inv = 1.0 / std::sqrt(ss);
