@@ -17,6 +17,8 @@
  *   calibrate_sim_cutoff +      src/impl/rb_graph.cpp:1362,
  *   build_similarity_graph      src/impl/rb_graph.cpp:199     -> rbg_build_graph (small)
  *   graph dispatch              src/rabbitbin.cpp:5035-5054   -> rbg_build_graph (auto)
+ *   the same graph seams over the caller's g_win16 / g_depth_unit / g_anynz
+ *                                                             -> rbg_build_graph_from_sketches
  *   edge-weight loop            src/rabbitbin.cpp:5482-5509   -> rbg_edge_weights
  *   depth features              src/rabbitbin.cpp:4784-4988   -> rbg_depth_features
  */
@@ -85,7 +87,53 @@ typedef struct rbg_graph_params {
   uint32_t rand_seed;      /* srand() seed (--seed); calibration consumes rand() */
   int regime;              /* 0 auto (n > 25000 -> fused), 1 small, 2 fused */
   uint32_t flags;          /* RBG_EDGES_ON_DEVICE: leave the edge list in HBM */
+  double near_tol;         /* edges whose FP64 weight (or correlation) lies within
+                              near_tol of the min-edge (or negative) cut are
+                              listed in the result (default 1e-6) */
+  uint32_t semantics;      /* RBG_SEM_* bits of non-default reference semantics
+                              (rbg_env_semantics); any bit set makes every
+                              build return RBG_ERR_UNSUPPORTED */
 } rbg_graph_params;
+
+/* ---------------------------------------------------------------------------
+ * Reference semantics selected by RABBIT_* environment variables.  The
+ * reference reads them at run time (src/rabbitbin.cpp:428-680, 3975-3992,
+ * 4655-4715, 4930-4940, 3480-3495, 3683-3688; src/impl/rb_graph.cpp:693).
+ * This library implements the defaults only.  rbg_env_semantics reads the
+ * process environment the way the reference does (an empty value counts as
+ * unset where the reference uses rb_getenv, and as set where it calls getenv)
+ * and returns the bits of the variables whose value changes this path's
+ * results; `names` (may be NULL) receives them comma-separated.  Variables that
+ * change speed only (RABBIT_TILE, RABBIT_ABD_Q8, RABBIT_NO_COMP_EE,
+ * RABBIT_NO_ABD_PRELOAD, ingest and NUMA knobs) and those acting before or
+ * after this path (depth parsing and prefilter, clustering, splitting,
+ * recruitment) are not reported.  RABBIT_NEG_DEPTH is a parameter here:
+ * rbg_default_params reads it into neg_depth_thr.
+ * ------------------------------------------------------------------------- */
+#define RBG_SEM_NO_PMH          (1u << 0)   /* RABBIT_PMH=0: OPH sketches */
+#define RBG_SEM_PMH_K           (1u << 1)   /* RABBIT_PMHK != 4 */
+#define RBG_SEM_NO_PMH_BASE     (1u << 2)   /* RABBIT_PMH_BASE=0: no b0 rescale */
+#define RBG_SEM_GC_NORM         (1u << 3)   /* RABBIT_GC_NORM != 1 */
+#define RBG_SEM_GC_NORM_CAP     (1u << 4)   /* RABBIT_GC_NORM_CAP != 20 */
+#define RBG_SEM_IDF_NORM        (1u << 5)   /* RABBIT_IDF_NORM=1 */
+#define RBG_SEM_EXACT_COS       (1u << 6)   /* RABBIT_EXACT_COS >= 1 */
+#define RBG_SEM_WIN_BITS        (1u << 7)   /* RABBIT_WIN_BITS != 16 */
+#define RBG_SEM_LSH             (1u << 8)   /* RABBIT_LSH=1 */
+#define RBG_SEM_PMH_INDEX       (1u << 9)   /* RABBIT_PMH_INDEX set */
+#define RBG_SEM_GRAPH_INDEX     (1u << 10)  /* RABBIT_GRAPH_INDEX=1 */
+#define RBG_SEM_NO_ABDFIRST     (1u << 11)  /* RABBIT_NO_ABDFIRST set: no abundance gate */
+#define RBG_SEM_NO_MUTUAL       (1u << 12)  /* RABBIT_MUTUAL_KNN=0 */
+#define RBG_SEM_W_COMP          (1u << 13)  /* RABBIT_W_COMP set */
+#define RBG_SEM_EDGE_CUT        (1u << 14)  /* RABBIT_EDGE_CUT=otsu|gmm */
+#define RBG_SEM_EDGE_POWER      (1u << 15)  /* RABBIT_EDGE_POWER != 1 */
+#define RBG_SEM_SNN             (1u << 16)  /* RABBIT_SNN on */
+#define RBG_SEM_DEPTH_SIM       (1u << 17)  /* RABBIT_DEPTH_SIM=corr|wjac */
+#define RBG_SEM_DEPTH_FUSE      (1u << 18)  /* RABBIT_DEPTH_FUSE=geo|product */
+#define RBG_SEM_WJAC_NORM       (1u << 19)  /* RABBIT_DEPTH_WJAC_NORM=0 */
+#define RBG_SEM_WJAC_CUT        (1u << 20)  /* RABBIT_DEPTH_WJAC_CUT > 0 */
+#define RBG_SEM_DUAL_CONJ       (1u << 21)  /* RABBIT_DUAL_CONJ != 0: conjunctive dual-depth corr */
+#define RBG_SEM_SNV_EDGE        (1u << 22)  /* RABBIT_SNV_EDGE != 0 */
+uint32_t rbg_env_semantics(char* names, size_t names_cap);
 
 /* With this flag rbg_asm_build_graph returns the counts, cutoffs and timings
  * only (the result's array pointers are NULL); the edge list stays in the
@@ -98,7 +146,9 @@ typedef struct rbg_graph_params {
  * host round trip inside a build).  Per-stage timings are then not reported. */
 #define RBG_BUILD_CAPTURE 2u
 
-/* Fills `p` with the reference defaults listed above (rand_seed = 1). */
+/* Fills `p` with the reference defaults listed above (rand_seed = 1,
+ * near_tol = 1e-6), neg_depth_thr from RABBIT_NEG_DEPTH when that is set (as
+ * src/rabbitbin.cpp:446-449 reads it) and semantics = rbg_env_semantics(). */
 void rbg_default_params(rbg_graph_params* p);
 
 typedef struct rbg_graph_result {
@@ -117,6 +167,15 @@ typedef struct rbg_graph_result {
    * abundance gate in the fused regime; 0 in the small regime), the exact
    * gate + register compare of its candidates, and the top-N selection */
   double ms_k_pair, ms_k_compare, ms_k_select;
+  /* Edges whose weight the reference's mixed FP32/FP64 arithmetic and FP64
+   * could decide differently: the FP64 correlation within near_tol of the
+   * negative cut, or the FP64 weight within near_tol of the min-edge cut.
+   * near_edge: their indices in the edge list (ascending); near_score64:
+   * their weights computed in FP64 throughout (0 when FP64 cuts the edge).
+   * score itself is the reference's arithmetic.  NULL when n_near == 0. */
+  uint64_t n_near;
+  uint64_t* near_edge;
+  double* near_score64;
 } rbg_graph_result;
 
 /* Optional per-row diagnostics for parity tests (any pointer may be NULL):
@@ -136,6 +195,27 @@ rbg_status rbg_build_graph(rbg_ctx* ctx, const char* seqs, const uint64_t* offse
                            const rbg_graph_params* params, rbg_graph_result* out,
                            rbg_graph_diag* diag);
 void rbg_graph_result_free(rbg_graph_result* r);
+
+/* ---------------------------------------------------------------------------
+ * Graph from the caller's sketches and depth rows: the seams of
+ * gen_fused_calib_graph (src/impl/rb_graph.cpp:501) and calibrate_sim_cutoff +
+ * build_similarity_graph (src/impl/rb_graph.cpp:1362, 199), dispatched as
+ * src/rabbitbin.cpp:5035-5054, with the globals they read passed explicitly:
+ *   win16    n x m  g_win16: 16-bit folded winners, 0 = unset register
+ *   b0       g_pmh_baseline; NULL estimates it (estimate_pmh_baseline)
+ *   unit     n x S  g_depth_unit (S > 1)     raw  n x S  g_depth_raw (S > 1)
+ *   colnorm  S      g_depth_colnorm (NULL: 1) anynz n    g_anynz (NULL: all 1)
+ * m must equal params->m.  Per register the distinct non-zero winners are
+ * renumbered to byte classes, at most 255 per register (PMH sketches of
+ * 4-mers have at most 136); the match counts are those of
+ * pmh_match_count16 (src/rabbitbin.cpp:1261-1293).  Edge weights as
+ * rbg_build_graph.  diag->win16 must be NULL (the winners are the input).
+ * ------------------------------------------------------------------------- */
+rbg_status rbg_build_graph_from_sketches(rbg_ctx* ctx, const uint16_t* win16, uint32_t n, uint32_t m,
+                                         const double* b0, const float* unit, const float* raw,
+                                         const double* colnorm, const uint8_t* anynz, uint32_t S,
+                                         const rbg_graph_params* params, rbg_graph_result* out,
+                                         rbg_graph_diag* diag);
 
 /* ---------------------------------------------------------------------------
  * Device-resident assembly: the contigs and depth features are uploaded once
@@ -280,8 +360,10 @@ rbg_status rbg_server_get_stats(rbg_server* s, rbg_server_stats* out);
 
 /* ---------------------------------------------------------------------------
  * Stage (d) alone (src/rabbitbin.cpp:5482-5509): edge weights for a given
- * edge list.  unit/raw are n x S, colnorm S.  Weights are computed in FP64
- * and returned as float (score) and, if non-NULL, double (score64).
+ * edge list.  unit/raw are n x S, colnorm S.  score is the reference's
+ * arithmetic (the correlation summed as depth_corr_fast sums it on an AVX-512
+ * build, src/rabbitbin.cpp:705-746; any S); score64, if non-NULL, the same
+ * weight with the correlation summed in FP64 throughout.
  * ------------------------------------------------------------------------- */
 rbg_status rbg_edge_weights(rbg_ctx* ctx, const uint32_t* from, const uint32_t* to,
                             const float* scomp, uint64_t ne, const float* unit,

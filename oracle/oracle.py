@@ -191,6 +191,25 @@ def edge_weights(frm, to, scomp, unit, raw, colnorm, S, mode, min_edge_weight=0.
     return w, w64
 
 
+def edge_near(frm, to, scomp, unit, raw, colnorm, S, tol, min_edge_weight=0.70, neg_thr=-0.3):
+    """Indices of the edges whose FP64 correlation is within tol of the
+    negative cut or whose FP64 weight before the cut is within tol of the
+    min-edge cut (ascending)."""
+    frm = np.ascontiguousarray(frm, np.uint32)
+    to = np.ascontiguousarray(to, np.uint32)
+    sc = np.ascontiguousarray(scomp, np.float32)
+    u = np.ascontiguousarray(unit, np.float32) if unit is not None else None
+    r = np.ascontiguousarray(raw, np.float32) if raw is not None else None
+    cn = np.ascontiguousarray(colnorm, np.float64) if colnorm is not None else None
+    near = np.zeros(len(frm), np.uint8)
+    L = orc()
+    L.orc_edge_near.argtypes = [u32p, u32p, f32p, C.c_uint64, f32p, f32p, f64p, C.c_uint32, C.c_double,
+                                C.c_double, C.c_double, C.c_double, u8p]
+    L.orc_edge_near(_p(frm, u32p), _p(to, u32p), _p(sc, f32p), len(frm), _p(u, f32p), _p(r, f32p),
+                    _p(cn, f64p), S, w_comp_for(S), min_edge_weight, neg_thr, tol, _p(near, u8p))
+    return np.flatnonzero(near).astype(np.uint64)
+
+
 def pair_pass_rows(win16, unit, anynz, S, b0, row0, row1, K=200, threads=0):
     """Timed fused pair pass of rows [row0, row1) against all partners."""
     L = orc()
@@ -208,7 +227,7 @@ def pair_pass_rows(win16, unit, anynz, S, b0, row0, row1, K=200, threads=0):
     return {"seconds": secs, "pairs": pairs.value, "gated": gated.value}
 
 
-def full_pipeline(seqs, offsets, depth, rand_seed=1, m=500, K=200, force_regime=0, threads=0):
+def full_pipeline(seqs, offsets, depth, rand_seed=1, m=500, K=200, force_regime=0, threads=0, near_tol=1e-6):
     """Oracle of the whole path: sketches, b0, graph, edge weights (reference arithmetic)."""
     w32 = sketch_batch(seqs, offsets, m=m, threads=threads)
     w16 = fold16(w32)
@@ -227,6 +246,10 @@ def full_pipeline(seqs, offsets, depth, rand_seed=1, m=500, K=200, force_regime=
                                                 colnorm if S > 1 else None, S, mode=1)
         g["score_ref"], _ = edge_weights(g["from"], g["to"], g["sv"], unit, raw,
                                          colnorm if S > 1 else None, S, mode=0)
+        g["near"] = edge_near(g["from"], g["to"], g["sv"], unit, raw, colnorm if S > 1 else None, S,
+                              near_tol)
+    else:
+        g["near"] = np.zeros(0, np.uint64)
     return g
 
 
@@ -357,6 +380,30 @@ def unit_dot_f_avx512(a, b):
     if L.orc_unit_dot_f_avx512(_p(a, f32p), _p(b, f32p), len(a), C.byref(out)) != 0:
         return None
     return np.float32(out.value)
+
+
+def depth_corr(a, b, mode):
+    """The edge weight's correlation term: mode 0 in the reference's AVX-512
+    order (FP32 chunks, FP64 tail), mode 1 in FP64 throughout."""
+    L = orc()
+    L.orc_depth_corr.restype = C.c_double
+    L.orc_depth_corr.argtypes = [f32p, f32p, C.c_uint32, C.c_int]
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return L.orc_depth_corr(_p(a, f32p), _p(b, f32p), len(a), mode)
+
+
+def depth_corr_avx512(a, b):
+    """depth_corr_fast through AVX-512 intrinsics (None without AVX-512F)."""
+    L = orc()
+    L.orc_depth_corr_avx512.restype = C.c_int
+    L.orc_depth_corr_avx512.argtypes = [f32p, f32p, C.c_uint32, f64p]
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = C.c_double()
+    if L.orc_depth_corr_avx512(_p(a, f32p), _p(b, f32p), len(a), C.byref(out)) != 0:
+        return None
+    return out.value
 
 
 def sv_from_count(cnt, m, b0):

@@ -57,6 +57,11 @@ void orc_depth_features(const float* depth, size_t n, size_t S, double min_cv,
 uint32_t orc_match_count16(const uint16_t* a, const uint16_t* b, uint32_t m);
 double orc_pmh_baseline(const uint16_t* win16, size_t n, uint32_t m);
 float orc_unit_dot_f(const float* a, const float* b, uint32_t S);
+// depth_corr_fast (src/rabbitbin.cpp:705-746): mode 0 in the reference's
+// AVX-512 order, mode 1 in FP64 throughout; the intrinsics form returns -1 on a
+// host without AVX-512F.
+double orc_depth_corr(const float* a, const float* b, uint32_t S, int mode);
+int orc_depth_corr_avx512(const float* a, const float* b, uint32_t S, double* out);
 double orc_sv_from_count(uint32_t cnt, uint32_t m, double b0);
 
 // glibc rand() stream after srand(seed), through the reentrant interface.
@@ -108,5 +113,11 @@ void orc_edge_weights(const uint32_t* from, const uint32_t* to,
                       double w_comp, double min_edge_weight,
                       double neg_depth_thr, int mode, float* out_w,
                       double* out_w64);
+// Per edge 1 when its FP64 correlation lies within tol of neg_depth_thr or its
+// FP64 weight before the cut within tol of min_edge_weight.
+void orc_edge_near(const uint32_t* from, const uint32_t* to, const float* scomp, uint64_t ne,
+                   const float* unit, const float* raw, const double* colnorm, uint32_t S,
+                   double w_comp, double min_edge_weight, double neg_depth_thr, double tol,
+                   uint8_t* near);
 
 }  // extern "C"

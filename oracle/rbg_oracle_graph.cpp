@@ -422,6 +422,34 @@ void orc_edge_weights(const uint32_t* from, const uint32_t* to, const float* sco
   }
 }
 
+// Edges whose FP64 arithmetic lies within tol of a cut (the edges the
+// reference's mixed FP32/FP64 correlation could decide differently from FP64):
+// the FP64 correlation within tol of the negative cut, or the FP64 weight
+// before the min-edge cut within tol of that cut.  S <= 1 weights are exact.
+void orc_edge_near(const uint32_t* from, const uint32_t* to, const float* scomp, uint64_t ne,
+                   const float* unit, const float* raw, const double* colnorm, uint32_t S,
+                   double w_comp, double min_edge_weight, double neg_thr, double tol,
+                   uint8_t* near) {
+#pragma omp parallel for schedule(static)
+  for (uint64_t e = 0; e < ne; ++e) {
+    uint8_t f = 0;
+    if (S > 1) {
+      const size_t i = from[e], j = to[e];
+      const double corr = corr_fp64(unit + i * S, unit + j * S, S);
+      if (neg_thr > -0.99 && std::fabs(corr - neg_thr) <= tol) f = 1;
+      if (std::isfinite(corr) && !(neg_thr > -0.99 && corr < neg_thr)) {
+        const double cp = corr < 0.0 ? 0.0 : corr;
+        double wj = wjac(raw + i * S, raw + j * S, colnorm, S);
+        if (!std::isfinite(wj)) wj = 0.0;
+        wj = wj < 0.0 ? 0.0 : (wj > 1.0 ? 1.0 : wj);
+        const double w = w_comp * (double)scomp[e] + (1.0 - w_comp) * (cp < wj ? cp : wj);
+        if (std::fabs(w - min_edge_weight) <= tol) f = 1;
+      }
+    }
+    near[e] = f;
+  }
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -452,7 +480,37 @@ __attribute__((target("avx512f"))) static float unit_dot_avx512(const float* a, 
   return r;
 }
 
+// depth_corr_fast with AVX-512 intrinsics, the loop shape of
+// src/rabbitbin.cpp:705-746 (FP32 chunks of 32 and 16, one reduce, FP64 tail).
+// Used only to pin corr_ref on an AVX-512 host.
+__attribute__((target("avx512f"))) static double depth_corr_avx512(const float* a, const float* b,
+                                                                   uint32_t S) {
+  uint32_t k = 0;
+  double s = 0.0;
+  __m512 acc0 = _mm512_setzero_ps(), acc1 = _mm512_setzero_ps();
+  for (; k + 32 <= S; k += 32) {
+    acc0 = _mm512_fmadd_ps(_mm512_loadu_ps(a + k), _mm512_loadu_ps(b + k), acc0);
+    acc1 = _mm512_fmadd_ps(_mm512_loadu_ps(a + k + 16), _mm512_loadu_ps(b + k + 16), acc1);
+  }
+  for (; k + 16 <= S; k += 16) acc0 = _mm512_fmadd_ps(_mm512_loadu_ps(a + k), _mm512_loadu_ps(b + k), acc0);
+  if (k > 0) s = (double)_mm512_reduce_add_ps(_mm512_add_ps(acc0, acc1));
+  for (; k < S; ++k) s += (double)a[k] * (double)b[k];
+  return s;
+}
+
 extern "C" {
+
+int orc_depth_corr_avx512(const float* a, const float* b, uint32_t S, double* out) {
+  if (!__builtin_cpu_supports("avx512f")) return -1;
+  *out = depth_corr_avx512(a, b, S);
+  return 0;
+}
+
+// The two correlation forms of the edge weight: mode 0 the reference's
+// depth_corr_fast order, mode 1 FP64 throughout.
+double orc_depth_corr(const float* a, const float* b, uint32_t S, int mode) {
+  return mode == 0 ? corr_ref(a, b, S) : corr_fp64(a, b, S);
+}
 
 int orc_unit_dot_f_avx512(const float* a, const float* b, uint32_t S, float* out) {
   if (!__builtin_cpu_supports("avx512f")) return -1;

@@ -38,11 +38,27 @@ class GraphParams(C.Structure):
         ("rand_seed", C.c_uint32),
         ("regime", C.c_int),
         ("flags", C.c_uint32),
+        ("near_tol", C.c_double),
+        ("semantics", C.c_uint32),
     ]
 
 
 EDGES_ON_DEVICE = 1
 BUILD_CAPTURE = 2
+
+# RBG_SEM_* bits (include/rbg.h): RABBIT_* variables selecting non-default
+# reference semantics, which every build refuses.
+SEMANTICS = {
+    "RABBIT_PMH": 1 << 0, "RABBIT_PMHK": 1 << 1, "RABBIT_PMH_BASE": 1 << 2, "RABBIT_GC_NORM": 1 << 3,
+    "RABBIT_GC_NORM_CAP": 1 << 4, "RABBIT_IDF_NORM": 1 << 5, "RABBIT_EXACT_COS": 1 << 6,
+    "RABBIT_WIN_BITS": 1 << 7, "RABBIT_LSH": 1 << 8, "RABBIT_PMH_INDEX": 1 << 9,
+    "RABBIT_GRAPH_INDEX": 1 << 10, "RABBIT_NO_ABDFIRST": 1 << 11, "RABBIT_MUTUAL_KNN": 1 << 12,
+    "RABBIT_W_COMP": 1 << 13, "RABBIT_EDGE_CUT": 1 << 14, "RABBIT_EDGE_POWER": 1 << 15,
+    "RABBIT_SNN": 1 << 16, "RABBIT_DEPTH_SIM": 1 << 17, "RABBIT_DEPTH_FUSE": 1 << 18,
+    "RABBIT_DEPTH_WJAC_NORM": 1 << 19, "RABBIT_DEPTH_WJAC_CUT": 1 << 20, "RABBIT_DUAL_CONJ": 1 << 21,
+    "RABBIT_SNV_EDGE": 1 << 22,
+}
+RBG_SEM_NO_MUTUAL = SEMANTICS["RABBIT_MUTUAL_KNN"]
 
 
 class AssemblyIn(C.Structure):
@@ -78,6 +94,9 @@ class GraphResult(C.Structure):
         ("ms_k_pair", C.c_double),
         ("ms_k_compare", C.c_double),
         ("ms_k_select", C.c_double),
+        ("n_near", C.c_uint64),
+        ("near_edge", u64p),
+        ("near_score64", f64p),
     ]
 
 
@@ -133,6 +152,9 @@ def lib():
     L.rbg_depth_features.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_double, f32p, f32p, f32p, f64p, u8p]
     L.rbg_build_graph.argtypes = [vp, C.c_char_p, u64p, C.c_uint32, f32p, C.c_uint32,
                                   C.POINTER(GraphParams), C.POINTER(GraphResult), C.POINTER(GraphDiag)]
+    L.rbg_build_graph_from_sketches.argtypes = [vp, u16p, C.c_uint32, C.c_uint32, f64p, f32p, f32p, f64p, u8p,
+                                                C.c_uint32, C.POINTER(GraphParams), C.POINTER(GraphResult),
+                                                C.POINTER(GraphDiag)]
     L.rbg_asm_create.argtypes = [vp, C.c_char_p, u64p, C.c_uint32, f32p, C.c_uint32, C.c_double, C.POINTER(vp)]
     L.rbg_asm_free.argtypes = [vp]
     L.rbg_asm_copy_edges.argtypes = [vp, vp, u32p, u32p, f32p, f32p]
@@ -143,6 +165,8 @@ def lib():
     L.rbg_edge_weights.argtypes = [vp, u32p, u32p, f32p, C.c_uint64, f32p, f32p, f64p, C.c_uint32,
                                    C.c_uint32, C.c_double, C.c_double, C.c_double, f32p, f64p]
     L.rbg_pmh_baseline.argtypes = [vp, u16p, C.c_uint32, C.c_uint32, f64p]
+    L.rbg_env_semantics.argtypes = [C.c_char_p, C.c_size_t]
+    L.rbg_env_semantics.restype = C.c_uint32
     L.rbg_batch_create.argtypes = [vp, C.POINTER(AssemblyIn), C.c_uint32, C.c_double, C.POINTER(vp)]
     L.rbg_batch_set_schedule.argtypes = [vp, C.POINTER(Schedule)]
     L.rbg_batch_build.argtypes = [vp, vp, C.POINTER(GraphParams), C.POINTER(BatchStats)]
@@ -165,6 +189,14 @@ def lib():
 
 def _p(a, t):
     return a.ctypes.data_as(t) if a is not None else None
+
+
+def env_semantics():
+    """rbg_env_semantics: (mask, comma list of the RABBIT_* variables that
+    select non-default reference semantics in this process's environment)."""
+    buf = C.create_string_buffer(1024)
+    mask = lib().rbg_env_semantics(buf, len(buf))
+    return mask, buf.value.decode()
 
 
 def default_params(**kw):
@@ -252,6 +284,10 @@ class Context:
             "score": (np.ctypeslib.as_array(res.score, (ne,)).copy() if (ne and res.score) else None),
             "sim_cutoff": res.sim_cutoff, "b0": res.b0, "regime": res.regime,
             "gated_pairs": res.gated_pairs,
+            "near_edge": (np.ctypeslib.as_array(res.near_edge, (res.n_near,)).copy() if res.n_near
+                          else np.zeros(0, np.uint64)),
+            "near_score64": (np.ctypeslib.as_array(res.near_score64, (res.n_near,)).copy() if res.n_near
+                             else np.zeros(0, np.float64)),
             "ms": {"sketch": res.ms_sketch, "pairs": res.ms_pairs, "select": res.ms_select,
                    "edges": res.ms_edges, "total": res.ms_total, "k_pair": res.ms_k_pair,
                    "k_compare": res.ms_k_compare, "k_select": res.ms_k_select},
@@ -284,6 +320,31 @@ class Context:
         res = GraphResult()
         self._check(self._L.rbg_build_graph(self._h, buf, _p(off, u64p), n, _p(dep, f32p), S, C.byref(p),
                                             C.byref(res), C.byref(d) if d else None), "rbg_build_graph")
+        return self._result(res, n, p.max_edges, arrs)
+
+    def build_graph_from_sketches(self, win16, unit=None, raw=None, colnorm=None, anynz=None, b0=None,
+                                  params=None, diag=False):
+        """rbg_build_graph_from_sketches: the graph from the caller's g_win16,
+        depth rows, g_anynz and (optionally) b0."""
+        w = np.ascontiguousarray(win16, np.uint16)
+        n, m = w.shape
+        p = params or default_params()
+        u = None if unit is None else np.ascontiguousarray(unit, np.float32)
+        r = None if raw is None else np.ascontiguousarray(raw, np.float32)
+        cn = None if colnorm is None else np.ascontiguousarray(colnorm, np.float64)
+        nz = None if anynz is None else np.ascontiguousarray(anynz, np.uint8)
+        S = 0 if u is None else u.shape[1]
+        b0v = None if b0 is None else C.c_double(b0)
+        d, arrs = self._diag(n, p.max_edges, p.m, diag)
+        if d is not None:
+            d.win16 = None
+            arrs.pop("win16")
+        res = GraphResult()
+        self._check(self._L.rbg_build_graph_from_sketches(self._h, _p(w, u16p), n, m,
+                                                          C.byref(b0v) if b0v is not None else None,
+                                                          _p(u, f32p), _p(r, f32p), _p(cn, f64p), _p(nz, u8p), S,
+                                                          C.byref(p), C.byref(res), C.byref(d) if d else None),
+                    "rbg_build_graph_from_sketches")
         return self._result(res, n, p.max_edges, arrs)
 
     def edge_weights(self, frm, to, scomp, unit, raw, colnorm, S, w_comp, min_edge_weight=0.7,
@@ -483,7 +544,11 @@ class Server:
 
         out = {"n_edges": ne, "from": arr(res.from_), "to": arr(res.to), "scomp": arr(res.scomp),
                "score": arr(res.score), "sim_cutoff": res.sim_cutoff, "b0": res.b0, "regime": res.regime,
-               "gated_pairs": res.gated_pairs}
+               "gated_pairs": res.gated_pairs,
+               "near_edge": (np.ctypeslib.as_array(res.near_edge, (res.n_near,)).copy() if res.n_near
+                             else np.zeros(0, np.uint64)),
+               "near_score64": (np.ctypeslib.as_array(res.near_score64, (res.n_near,)).copy() if res.n_near
+                                else np.zeros(0, np.float64))}
         for k, dt in (("from", np.uint32), ("to", np.uint32), ("scomp", np.float32)):
             if out[k] is None:
                 out[k] = np.zeros(0, dt)

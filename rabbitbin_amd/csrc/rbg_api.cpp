@@ -46,18 +46,130 @@ const char* rbg_last_error(const rbg_ctx* ctx) {
   return copy.c_str();
 }
 
+}  // extern "C"
+
+namespace {
+
+// The reference's two ways of reading a variable: rb_getenv treats an empty
+// value as unset (src/rabbitbin.cpp:428-431), getenv does not.
+const char* env_nonempty(const char* key) {
+  const char* e = std::getenv(key);
+  return (e && e[0]) ? e : nullptr;
+}
+
+struct Knob {
+  const char* name;
+  uint32_t bit;
+  bool (*changed)(const char* key);  // value differs from the default in effect
+};
+
+// Each predicate restates how the reference parses the variable and when the
+// parsed value differs from the default (file:line of the parse).
+const Knob kKnobs[] = {
+    {"RABBIT_PMH", RBG_SEM_NO_PMH,  // rabbitbin.cpp:433-436
+     [](const char* k) { const char* e = env_nonempty(k); return e && e[0] == '0'; }},
+    {"RABBIT_PMHK", RBG_SEM_PMH_K,  // rabbitbin.cpp:4668
+     [](const char* k) { const char* e = env_nonempty(k); return e && std::atoi(e) != 4; }},
+    {"RABBIT_PMH_BASE", RBG_SEM_NO_PMH_BASE,  // rabbitbin.cpp:4669
+     [](const char* k) { const char* e = env_nonempty(k); return e && e[0] == '0'; }},
+    {"RABBIT_GC_NORM", RBG_SEM_GC_NORM,  // rabbitbin.cpp:441-444
+     [](const char* k) { const char* e = env_nonempty(k); return e && std::atoi(e) != 1; }},
+    {"RABBIT_GC_NORM_CAP", RBG_SEM_GC_NORM_CAP,  // rabbitbin.cpp:3987-3988
+     [](const char* k) {
+       const char* e = env_nonempty(k);
+       if (!e) return false;
+       const double v = std::atof(e);
+       return (v < 1.0 ? 20.0 : v) != 20.0;
+     }},
+    {"RABBIT_IDF_NORM", RBG_SEM_IDF_NORM,  // rabbitbin.cpp:3989
+     [](const char* k) { const char* e = env_nonempty(k); return e && e[0] == '1'; }},
+    {"RABBIT_EXACT_COS", RBG_SEM_EXACT_COS,  // rabbitbin.cpp:3990-3991
+     [](const char* k) { const char* e = env_nonempty(k); return e && std::atoi(e) >= 1; }},
+    {"RABBIT_WIN_BITS", RBG_SEM_WIN_BITS,  // rabbitbin.cpp:4935-4936
+     [](const char* k) { const char* e = std::getenv(k); return e && std::atoi(e) != 16; }},
+    {"RABBIT_LSH", RBG_SEM_LSH,  // rabbitbin.cpp:4937-4938
+     [](const char* k) { const char* e = std::getenv(k); return e && e[0] == '1'; }},
+    {"RABBIT_PMH_INDEX", RBG_SEM_PMH_INDEX,  // rabbitbin.cpp:4711
+     [](const char* k) { return std::getenv(k) != nullptr; }},
+    {"RABBIT_GRAPH_INDEX", RBG_SEM_GRAPH_INDEX,  // rabbitbin.cpp:4657-4660
+     [](const char* k) { const char* e = env_nonempty(k); return e && e[0] == '1'; }},
+    {"RABBIT_NO_ABDFIRST", RBG_SEM_NO_ABDFIRST,  // impl/rb_graph.cpp:693
+     [](const char* k) { return std::getenv(k) != nullptr; }},
+    {"RABBIT_MUTUAL_KNN", RBG_SEM_NO_MUTUAL,  // rabbitbin.cpp:437-440
+     [](const char* k) { const char* e = env_nonempty(k); return e && e[0] == '0'; }},
+    {"RABBIT_W_COMP", RBG_SEM_W_COMP,  // rabbitbin.cpp:451-456, 4671-4672
+     [](const char* k) { return env_nonempty(k) != nullptr; }},
+    {"RABBIT_EDGE_CUT", RBG_SEM_EDGE_CUT,  // rabbitbin.cpp:505-509
+     [](const char* k) {
+       const char* e = env_nonempty(k);
+       return e && (!std::strcmp(e, "otsu") || !std::strcmp(e, "gmm"));
+     }},
+    {"RABBIT_EDGE_POWER", RBG_SEM_EDGE_POWER,  // rabbitbin.cpp:4701
+     [](const char* k) { const char* e = env_nonempty(k); return e && std::atof(e) != 1.0; }},
+    {"RABBIT_SNN", RBG_SEM_SNN,  // rabbitbin.cpp:484-485
+     [](const char* k) { const char* e = env_nonempty(k); return e && e[0] != '0'; }},
+    {"RABBIT_DEPTH_SIM", RBG_SEM_DEPTH_SIM,  // rabbitbin.cpp:654-660
+     [](const char* k) {
+       const char* e = env_nonempty(k);
+       if (!e) return false;
+       const char c = e[0];
+       return c == 'c' || c == 'C' || c == '0' || c == 'w' || c == 'W' || c == '1';
+     }},
+    {"RABBIT_DEPTH_FUSE", RBG_SEM_DEPTH_FUSE,  // rabbitbin.cpp:661-667
+     [](const char* k) {
+       const char* e = env_nonempty(k);
+       if (!e) return false;
+       const char c = e[0];
+       return c == 'g' || c == 'G' || c == '0' || c == 'p' || c == 'P' || c == '2';
+     }},
+    {"RABBIT_DEPTH_WJAC_NORM", RBG_SEM_WJAC_NORM,  // rabbitbin.cpp:668-671
+     [](const char* k) { const char* e = env_nonempty(k); return e && e[0] == '0'; }},
+    {"RABBIT_DEPTH_WJAC_CUT", RBG_SEM_WJAC_CUT,  // rabbitbin.cpp:672-675, 942
+     [](const char* k) { const char* e = env_nonempty(k); return e && std::atof(e) > 0.0; }},
+    {"RABBIT_DUAL_CONJ", RBG_SEM_DUAL_CONJ,  // rabbitbin.cpp:3686-3688, 951
+     [](const char* k) { const char* e = std::getenv(k); return e && std::atoi(e) != 0; }},
+    {"RABBIT_SNV_EDGE", RBG_SEM_SNV_EDGE,  // rabbitbin.cpp:3488-3490
+     [](const char* k) { const char* e = std::getenv(k); return e && std::atoi(e) != 0; }},
+};
+
+}  // namespace
+
+extern "C" {
+
+uint32_t rbg_env_semantics(char* names, size_t names_cap) {
+  uint32_t mask = 0;
+  std::string list;
+  for (const Knob& k : kKnobs)
+    if (k.changed(k.name)) {
+      mask |= k.bit;
+      if (!list.empty()) list += ",";
+      list += k.name;
+    }
+  if (names && names_cap) {
+    const size_t n = std::min(list.size(), names_cap - 1);
+    std::memcpy(names, list.data(), n);
+    names[n] = 0;
+  }
+  return mask;
+}
+
 void rbg_default_params(rbg_graph_params* p) {
   if (!p) return;
+  std::memset(p, 0, sizeof(*p));
   p->m = 500;
   p->max_edges = 200;
   p->min_edge_weight = 70 / 100.;
   p->coverage = 95 / 100.;
   p->min_cv = 1.0;
-  p->neg_depth_thr = -0.3;
+  // RABBIT_NEG_DEPTH (src/rabbitbin.cpp:445-449): an explicit parameter here
+  const char* e = env_nonempty("RABBIT_NEG_DEPTH");
+  p->neg_depth_thr = e ? std::atof(e) : -0.3;
   p->pmh_seed = 42;
   p->rand_seed = 1;
   p->regime = 0;
   p->flags = 0;
+  p->near_tol = 1e-6;
+  p->semantics = rbg_env_semantics(nullptr, 0);
 }
 
 rbg_status rbg_sketch_winners(rbg_ctx* ctx, const char* seqs, const uint64_t* offsets, uint32_t n,
@@ -159,6 +271,68 @@ rbg_status rbg_build_graph(rbg_ctx* ctx, const char* seqs, const uint64_t* offse
   const rbg_status s = rbg_asm_build_graph(ctx, a, params, out, diag);
   rbg_asm_free(a);
   return s;
+}
+
+rbg_status rbg_build_graph_from_sketches(rbg_ctx* ctx, const uint16_t* win16, uint32_t n, uint32_t m,
+                                         const double* b0, const float* unit, const float* raw,
+                                         const double* colnorm, const uint8_t* anynz, uint32_t S,
+                                         const rbg_graph_params* params, rbg_graph_result* out,
+                                         rbg_graph_diag* diag) {
+  if (!ctx || !params || !out || (n && !win16)) return RBG_ERR_ARG;
+  std::memset(out, 0, sizeof(*out));
+  if (n == 0) return ctx->fail(RBG_ERR_ARG, "no contigs");
+  if (m != params->m) return ctx->fail(RBG_ERR_ARG, "the sketches' m differs from params->m");
+  if (m < 2 || m > 512) return ctx->fail(RBG_ERR_UNSUPPORTED, "sketch-m must be in [2, 512]");
+  if (S > 1 && (!unit || !raw)) return ctx->fail(RBG_ERR_ARG, "S > 1 needs unit and raw rows");
+  if (b0 && !(*b0 >= 0.0 && *b0 < 1.0)) return ctx->fail(RBG_ERR_ARG, "b0 must be in [0, 1)");
+  if (diag && diag->win16) return ctx->fail(RBG_ERR_ARG, "the winners are the caller's input");
+  RBG_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  // Per register, the distinct non-zero 16-bit winners are numbered in
+  // ascending order; two registers match exactly when their numbers do.
+  const uint32_t mpad = 32 * words_for(m);
+  std::vector<uint8_t> classes((size_t)n * mpad, 0xFF);
+  std::vector<uint16_t> present((size_t)m * 4096, 0);  // m x 65536 bits
+  for (size_t r = 0; r < n; ++r)
+    for (uint32_t k = 0; k < m; ++k) {
+      const uint16_t v = win16[r * m + k];
+      present[(size_t)k * 4096 + (v >> 4)] |= (uint16_t)(1u << (v & 15));
+    }
+  std::vector<uint8_t> rank((size_t)m * 65536, 0xFF);
+  for (uint32_t k = 0; k < m; ++k) {
+    uint32_t c = 0;
+    for (uint32_t v = 1; v < 65536; ++v)
+      if (present[(size_t)k * 4096 + (v >> 4)] >> (v & 15) & 1u) {
+        if (c == 255)
+          return ctx->fail(RBG_ERR_UNSUPPORTED, "register " + std::to_string(k) +
+                                                    " holds more than 255 distinct winners");
+        rank[(size_t)k * 65536 + v] = (uint8_t)c++;
+      }
+  }
+  for (size_t r = 0; r < n; ++r)
+    for (uint32_t k = 0; k < m; ++k) classes[r * mpad + k] = rank[(size_t)k * 65536 + win16[r * m + k]];
+  SketchInput sk;
+  sk.classes = classes.data();
+  sk.mpad = mpad;
+  sk.unit = unit;
+  sk.raw = raw;
+  sk.colnorm = colnorm;
+  sk.anynz = anynz;
+  if (b0) sk.b0 = *b0;
+  AsmInput in;
+  in.in = rbg_assembly_in{nullptr, nullptr, n, nullptr, S};
+  in.sk = &sk;
+  rbg_asm a;
+  RBG_TRY(batch_upload(ctx, &a, &in, 1, params->min_cv));
+  rbg_graph_params P = *params;
+  P.flags &= ~(uint32_t)RBG_BUILD_CAPTURE;
+  RBG_TRY(run_build(ctx, &a, &P, false, false, nullptr));
+  const rbg_status st = assembly_result(ctx, &a, 0, &P, (P.flags & RBG_EDGES_ON_DEVICE) != 0, out, diag);
+  if (st != RBG_OK) {
+    rbg_graph_result_free(out);
+    return st;
+  }
+  out->ms_total = elapsed_ms(ctx->ev[0], ctx->ev[5]);
+  return RBG_OK;
 }
 
 rbg_status rbg_batch_create(rbg_ctx* ctx, const rbg_assembly_in* asms, uint32_t count,
@@ -280,9 +454,14 @@ void rbg_graph_result_free(rbg_graph_result* r) {
   std::free(r->to);
   std::free(r->scomp);
   std::free(r->score);
+  std::free(r->near_edge);
+  std::free(r->near_score64);
   r->from = r->to = nullptr;
   r->scomp = r->score = nullptr;
+  r->near_edge = nullptr;
+  r->near_score64 = nullptr;
   r->n_edges = 0;
+  r->n_near = 0;
 }
 
 rbg_status rbg_edge_weights(rbg_ctx* ctx, const uint32_t* from, const uint32_t* to,

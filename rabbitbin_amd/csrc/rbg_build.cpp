@@ -352,7 +352,7 @@ size_t rbg_asm::device_bytes() const {
                          &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
                          &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt,
                          &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
-                         &ecnt, &escomp, &escore, &scan_tmp, &bounds};
+                         &ecnt, &escomp, &escore, &scan_tmp, &bounds, &near_e, &near_w, &cls_in};
   size_t s = 0;
   for (const DevBuf* d : all) s += d->bytes;
   return s;
@@ -365,7 +365,7 @@ void rbg_asm::release_buffers() {
                    &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
                    &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt,
                    &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
-                   &ecnt, &escomp, &escore, &scan_tmp, &bounds};
+                   &ecnt, &escomp, &escore, &scan_tmp, &bounds, &near_e, &near_w, &cls_in};
   for (DevBuf* d : all) d->release();
   planned = false;
   built = false;
@@ -400,11 +400,15 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
   b->has_src = false;
   b->n_incr = 0;
   b->list_extra = 0;
+  b->cls_rows = 0;
   for (uint32_t a = 0; a < count; ++a) {
     const rbg_assembly_in& x = in[a].in;
     const Resident* src = in[a].src;
+    const SketchInput* sk = in[a].sk;
     const uint32_t n_old = src ? src->n : 0;
-    if ((x.n > 0 && (!x.seqs || !x.offsets)) || (x.S > 0 && x.n > 0 && !x.depth))
+    if (sk && (count != 1 || src || !sk->classes || (x.S > 1 && (!sk->unit || !sk->raw))))
+      return ctx->fail(RBG_ERR_ARG, "sketch input: one assembly with class bytes (and unit/raw rows when S > 1)");
+    if (!sk && ((x.n > 0 && (!x.seqs || !x.offsets)) || (x.S > 0 && x.n > 0 && !x.depth)))
       return ctx->fail(RBG_ERR_ARG, "assembly " + std::to_string(a) + ": missing input");
     if (n_old + (uint64_t)x.n == 0)
       return ctx->fail(RBG_ERR_ARG, "assembly " + std::to_string(a) + ": no contigs");
@@ -420,10 +424,13 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     h.S = x.S;
     h.row0 = (uint32_t)rows;
     h.colnorm0 = (uint32_t)ncol;
+    if (sk) h.b0_in = sk->b0;
     rows += h.n;
     ncol += x.S;
-    nsk += x.n;
-    if (x.n) bytes += x.offsets[x.n] - x.offsets[0];
+    if (!sk) {
+      nsk += x.n;
+      if (x.n) bytes += x.offsets[x.n] - x.offsets[0];
+    }
     if (x.S > 1) {
       nc_max = std::max(nc_max, nc_for(x.S));
       kc_max = std::max(kc_max, kpad_for(x.S) / 32);
@@ -473,13 +480,21 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     AsmHost& h = b->asms[a];
     Prep& pr = prep[a];
     const Resident* src = h.src;
+    const SketchInput* sk = in[a].sk;
     const uint32_t n_old = h.n_old, n_new = x.n, S = x.S;
     // depth features of the new rows (row-local), colnorm from running column
     // sums (the same additions in the same order as over all rows at once)
     h.anynz_local.assign(h.n, 1);
     if (src) std::copy(src->anynz.begin(), src->anynz.end(), h.anynz_local.begin());
     std::vector<float> unit, raw;
-    if (S > 0) {
+    if (sk) {  // the caller's rows (g_depth_unit, g_depth_raw, g_depth_colnorm, g_anynz)
+      if (sk->anynz) std::copy(sk->anynz, sk->anynz + n_new, h.anynz_local.begin());
+      if (S > 1) {
+        unit.assign(sk->unit, sk->unit + (size_t)n_new * S);
+        raw.assign(sk->raw, sk->raw + (size_t)n_new * S);
+        for (uint32_t c = 0; c < S; ++c) colnorm[h.colnorm0 + c] = sk->colnorm ? sk->colnorm[c] : 1.0;
+      }
+    } else if (S > 0) {
       unit.assign((size_t)n_new * S, 0.f);
       raw.assign((size_t)n_new * S, 0.f);
       depth_features(x.depth, n_new, S, min_cv, nullptr, unit.data(), raw.data(), nullptr,
@@ -529,7 +544,7 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
       for (uint32_t r = 0; r < n_new; ++r) h.perm_local[n_old + r] = n_old + r;
       if (src) h.gq = src->gq;
     }
-    baseline_pairs(h.n, pr.ai, pr.aj);  // b0 pair draws (they depend on n only)
+    if (!(sk && sk->b0 == sk->b0)) baseline_pairs(h.n, pr.ai, pr.aj);  // b0 pair draws (they depend on n only)
   };
   if (count == 1) {
     prepare(0, true);
@@ -550,7 +565,7 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     uint64_t at = 0;
     for (uint32_t a = 0; a < count; ++a) {
       const rbg_assembly_in& x = in[a].in;
-      const uint64_t nb = x.n ? x.offsets[x.n] - x.offsets[0] : 0;
+      const uint64_t nb = (x.n && !in[a].sk) ? x.offsets[x.n] - x.offsets[0] : 0;
       if (nb) seq_parts.push_back(H2DPart{(char*)b->seqs.p + at, x.seqs + x.offsets[0], nb});
       at += nb;
     }
@@ -562,18 +577,26 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     AsmHost& h = b->asms[a];
     Prep& pr = prep[a];
     const Resident* src = h.src;
-    const uint32_t n_old = h.n_old, n_new = x.n, S = x.S;
-    // contigs to sketch
+    const uint32_t n_old = h.n_old, S = x.S;
+    const uint32_t n_new = in[a].sk ? 0u : x.n;  // contigs to sketch
     const uint64_t nb = n_new ? x.offsets[n_new] - x.offsets[0] : 0;
     for (uint32_t r = 0; r < n_new; ++r) {
       offs[sk + r] = seq0 + (x.offsets[r] - x.offsets[0]);
       sk_rows[sk + r] = h.row0 + n_old + r;
+    }
+    if (in[a].sk) {  // class bytes of the caller's sketches, bit-sliced at build time
+      const SketchInput& si = *in[a].sk;
+      RBG_TRY_HIP(ctx, b->cls_in.ensure((size_t)x.n * si.mpad));
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(b->cls_in.p, si.classes, (size_t)x.n * si.mpad, hipMemcpyHostToDevice, st));
+      b->cls_rows = x.n;
+      b->cls_mpad = si.mpad;
     }
     sk += n_new;
     seq0 += nb;
     offs[sk] = seq0;
     for (uint32_t r = 0; r < h.n; ++r) asm_of[h.row0 + r] = (uint16_t)a;
     std::copy(h.anynz_local.begin(), h.anynz_local.end(), anynz.begin() + h.row0);
+    h.n_zero = (uint32_t)std::count(h.anynz_local.begin(), h.anynz_local.end(), (uint8_t)0);
     if (!pr.upad.empty()) {
       const size_t o = (size_t)(h.row0 + n_old) * 16 * NC * 4;
       RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->unit_pad.p + o, pr.upad.data(), pr.upad.size() * 4,
@@ -625,6 +648,19 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   const uint32_t m = P->m, K = P->max_edges;
   if (m < 2 || m > 512) return ctx->fail(RBG_ERR_UNSUPPORTED, "sketch-m must be in [2, 512]");
   if (K == 0 || K > 1024) return ctx->fail(RBG_ERR_UNSUPPORTED, "max-edges must be in [1, 1024]");
+  if (P->semantics) {
+    char names[512];
+    (void)rbg_env_semantics(names, sizeof names);
+    return ctx->fail(RBG_ERR_UNSUPPORTED,
+                     "non-default reference semantics requested (semantics=0x" + [&] {
+                       char h[16];
+                       std::snprintf(h, sizeof h, "%x", P->semantics);
+                       return std::string(h);
+                     }() + (names[0] ? std::string(", environment: ") + names : std::string()) +
+                         "): this library implements the defaults only; use the reference's CPU path");
+  }
+  if (!(P->near_tol >= 0.0 && P->near_tol <= 1.0))
+    return ctx->fail(RBG_ERR_ARG, "near_tol must be in [0, 1]");
   PlanKey key;
   key.m = m;
   key.K = K;
@@ -637,6 +673,7 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   key.min_edge = P->min_edge_weight;
   key.coverage = P->coverage;
   key.neg_thr = P->neg_depth_thr;
+  key.near_tol = P->near_tol;
   if (b->planned && b->key == key) return RBG_OK;
   b->drop_graph();
   b->planned = false;
@@ -659,18 +696,24 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
     if (regime != 1 && regime != 2) return ctx->fail(RBG_ERR_ARG, "regime must be 0, 1 or 2");
     if (regime == 2 && h.n < 2500)
       return ctx->fail(RBG_ERR_ARG, "fused regime needs n >= 2500 (assembly " + std::to_string(a) + ")");
-    // A session whose earlier build was fused keeps its gated pairs: only the
-    // pairs with a new row are evaluated.
-    h.incr = regime == 2 && h.src && h.src->regime == 2 && h.n_old > 0;
-    h.dense_incr = regime == 1 && h.src && h.src->regime == 1 && h.n_old > 0 && h.src->mat.p;
+    const bool dense = dense_counts(regime, h.S, P->min_edge_weight);
+    if (dense && (uint64_t)h.n * h.n * 2 > kDenseMaxBytes)
+      return ctx->fail(RBG_ERR_UNSUPPORTED, "assembly " + std::to_string(a) + ": " + std::to_string(h.n) +
+                                                " contigs without the abundance gate (S <= 2) need an n x n "
+                                                "count matrix above 64 GiB");
+    // A session whose earlier build kept its gated pairs (or its counts)
+    // evaluates only the pairs with a new row.
+    h.incr = !dense && h.src && h.src->regime == 2 && !h.src->dense && h.n_old > 0;
+    h.dense_incr = dense && h.src && h.src->dense && h.n_old > 0 && h.src->mat.p;
+    d.dense = dense ? 1u : 0u;
     d.row0 = h.row0;
     d.n = h.n;
     d.S = h.S;
     d.regime = (uint32_t)regime;
     // composition weight: auto_w_comp(S) (src/rabbitbin.cpp:470-473)
-    d.w_comp = h.S <= 2 ? 1.0 : 0.0;
+    d.w_comp = w_comp_for(h.S);
     // abundance-first gate (src/impl/rb_graph.cpp:693-706)
-    const bool gate = regime == 2 && h.S > 1 && d.w_comp < 1.0 && d.w_comp < P->min_edge_weight;
+    const bool gate = gate_on(regime, h.S, P->min_edge_weight);
     d.thr = gate ? (float)((P->min_edge_weight - d.w_comp) / (1.0 - d.w_comp) - 1e-4)
                  : -__builtin_huge_valf();
     // prefilter cut: with the gate off every pair is a candidate
@@ -680,7 +723,8 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
     d.pair0 = h.pair0;
     d.np = h.np;
     d.colnorm0 = h.colnorm0;
-    if (regime == 1) {
+    d.b0_in = h.b0_in;
+    if (dense) {
       d.mat0 = mat_total;
       d.jmin = h.dense_incr ? h.n_old : 0;
       mat_total += (uint64_t)h.n * h.n;
@@ -747,7 +791,7 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   RBG_TRY_HIP(ctx, b->level.ensure((size_t)A * (m + 1) * 2));
   RBG_TRY_HIP(ctx, b->mat.ensure(std::max<uint64_t>(mat_total, 1) * 2));
   RBG_TRY_HIP(ctx, b->cand.ensure(std::max<uint64_t>(cap, 1) * 8));
-  RBG_TRY_HIP(ctx, b->counters.ensure(16));
+  RBG_TRY_HIP(ctx, b->counters.ensure(32));  // candidates, near-cut edges (u64 each)
   RBG_TRY_HIP(ctx, b->deg.ensure((size_t)N * 4));
   RBG_TRY_HIP(ctx, b->slot.ensure(std::max<uint64_t>(cap, 1) * 8));  // 2 list slots per candidate
   RBG_TRY_HIP(ctx, b->off.ensure((size_t)(N + 1) * 8));
@@ -768,6 +812,9 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   RBG_TRY_HIP(ctx, b->escomp.ensure(ce * 4));
   RBG_TRY_HIP(ctx, b->escore.ensure(ce * 4));
   RBG_TRY_HIP(ctx, b->bounds.ensure((size_t)A * 4 * 8));
+  b->near_cap = std::min<uint64_t>(ce, 1u << 20);
+  RBG_TRY_HIP(ctx, b->near_e.ensure(b->near_cap * 8));
+  RBG_TRY_HIP(ctx, b->near_w.ensure(b->near_cap * 8));
   size_t tmp = 0;
   RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64(nullptr, nullptr, N, nullptr, tmp, st));
   RBG_TRY_HIP(ctx, b->scan_tmp.ensure(tmp));
@@ -791,10 +838,10 @@ size_t estimate_build_bytes(const AsmInput* in, uint32_t count, const rbg_graph_
     NC = std::max(NC, nc_for(x.S));
     kpad = std::max(kpad, kpad_for(x.S));
     const int regime = regime_for(P->regime, n);
-    if (regime == 1) {
+    if (dense_counts(regime, x.S, P->min_edge_weight)) {
       mat += (uint64_t)n * n;
     } else {
-      const bool incr = src && src->regime == 2 && n_old > 0;
+      const bool incr = src && src->regime == 2 && !src->dense && n_old > 0;
       cap += cand_capacity(pass_pairs(n, n_old, incr), n);
       if (incr) extra += src->nlist;
     }
@@ -828,6 +875,7 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
     return RBG_OK;
   };
   RBG_TRY(mark(0));
+  RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->counters.p, 8, 0u, st));
   const uint32_t mpad = 32 * W;
   if (want_wx) RBG_TRY_HIP(ctx, b->wx.ensure((size_t)N * mpad));
   // sketches: rows of resident sessions are copied, the others computed
@@ -841,6 +889,9 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                  (const uint8_t*)t->cls.p, (const uint8_t*)t->codes.p,
                                  want_wx ? (uint8_t*)b->wx.p : nullptr, mpad, (uint32_t*)b->planes.p,
                                  b->has_src ? (const uint32_t*)b->sk_rows.p : nullptr, st));
+  if (b->cls_rows)
+    RBG_TRY_HIP(ctx, launch_planes_from_classes((const uint8_t*)b->cls_in.p, b->cls_rows, m, b->cls_mpad, W, 0,
+                                                (uint32_t*)b->planes.p, st));
   RBG_TRY(mark(1));
   // b0 and the sv tables (the pair draws depend on n only)
   if (b->np_total)
@@ -865,7 +916,6 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                          (uint16_t*)b->mat.p, st));
   unsigned long long* ncand = (unsigned long long*)b->counters.p;
   if (b->has_gate_items) {
-    RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->counters.p, 4, 0u, st));
     RBG_TRY_HIP(ctx, launch_gate_mfma((const int8_t*)b->q8.p, b->kpad, desc,
                                       (const uint32_t*)b->perm.p, (const uint32_t*)b->gate_items.p,
                                       b->n_gate_items, (uint64_t*)b->cand.p, ncand, b->cand_cap, st));
@@ -931,8 +981,9 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                              (const float*)b->escomp.p, b->edge_cap, ne,
                                              (const float*)b->unit_pad.p, (const float*)b->raw_pad.p,
                                              b->NC, (const double*)b->colnorm.p, desc, asm_of,
-                                             P->min_edge_weight, P->neg_depth_thr,
-                                             (float*)b->escore.p, st));
+                                             P->min_edge_weight, P->neg_depth_thr, P->near_tol,
+                                             (float*)b->escore.p, ncand + 1, b->near_cap,
+                                             (uint64_t*)b->near_e.p, (double*)b->near_w.p, st));
   RBG_TRY_HIP(ctx, launch_gather_bounds((const uint64_t*)b->row_off.p,
                                         b->has_gate_items ? (const uint64_t*)b->off.p : nullptr, desc,
                                         A, (uint64_t*)b->bounds.p, st));
@@ -971,16 +1022,26 @@ rbg_status run_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P, bool w
     }
     b->bs_h.resize(A);
     b->bounds_h.resize((size_t)A * 4);
-    unsigned long long nc = 0;
+    unsigned long long cnt[2] = {0, 0};  // candidates, near-cut edges
     RBG_TRY_HIP(ctx, hipMemcpyAsync(b->bs_h.data(), b->bscal.p, A * sizeof(BuildScalars),
                                     hipMemcpyDeviceToHost, st));
     RBG_TRY_HIP(ctx, hipMemcpyAsync(b->bounds_h.data(), b->bounds.p, (size_t)A * 32,
                                     hipMemcpyDeviceToHost, st));
-    if (b->has_gate_items)
-      RBG_TRY_HIP(ctx, hipMemcpyAsync(&nc, b->counters.p, 8, hipMemcpyDeviceToHost, st));
+    RBG_TRY_HIP(ctx, hipMemcpyAsync(cnt, b->counters.p, 16, hipMemcpyDeviceToHost, st));
     RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+    const uint64_t nc = b->has_gate_items ? cnt[0] : 0;
     b->ncand_last = nc;
     if (nc <= b->cand_cap) {
+      if (cnt[1] > b->near_cap)
+        return ctx->fail(RBG_ERR_NOMEM, "more near-cut edges than the list holds (" +
+                                            std::to_string(cnt[1]) + "): lower near_tol");
+      b->near_e_h.resize(cnt[1]);
+      b->near_w_h.resize(cnt[1]);
+      if (cnt[1]) {
+        RBG_TRY_HIP(ctx, hipMemcpyAsync(b->near_e_h.data(), b->near_e.p, cnt[1] * 8, hipMemcpyDeviceToHost, st));
+        RBG_TRY_HIP(ctx, hipMemcpyAsync(b->near_w_h.data(), b->near_w.p, cnt[1] * 8, hipMemcpyDeviceToHost, st));
+        RBG_TRY_HIP(ctx, hipStreamSynchronize(st));
+      }
       b->built = true;
       return RBG_OK;
     }
@@ -1005,7 +1066,25 @@ rbg_status assembly_result(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph
   out->sim_cutoff = s.sim_cutoff;
   out->regime = (int)b->desc_h[a].regime;
   out->gated_pairs = (b->bounds_h[4 * a + 3] - b->bounds_h[4 * a + 2]) / 2;
+  if (b->desc_h[a].dense && out->regime == 2)  // no gate: every pair passing is_nz was evaluated
+    out->gated_pairs = (uint64_t)h.n * (h.n - 1) / 2 - (uint64_t)h.n_zero * (h.n_zero - (h.n_zero > 0)) / 2;
   const uint32_t K = P->max_edges, m = P->m, n = h.n, r0 = h.row0;
+  {  // near-cut edges of this assembly, by edge index
+    std::vector<std::pair<uint64_t, double>> nr;
+    for (size_t t = 0; t < b->near_e_h.size(); ++t)
+      if (b->near_e_h[t] >= e0 && b->near_e_h[t] < e1) nr.emplace_back(b->near_e_h[t] - e0, b->near_w_h[t]);
+    std::sort(nr.begin(), nr.end());
+    out->n_near = nr.size();
+    if (!nr.empty()) {
+      out->near_edge = (uint64_t*)std::malloc(nr.size() * 8);
+      out->near_score64 = (double*)std::malloc(nr.size() * 8);
+      if (!out->near_edge || !out->near_score64) return ctx->fail(RBG_ERR_NOMEM, "host allocation of the near list");
+      for (size_t t = 0; t < nr.size(); ++t) {
+        out->near_edge[t] = nr[t].first;
+        out->near_score64[t] = nr[t].second;
+      }
+    }
+  }
   if (!on_device) {
     const size_t nb = std::max<uint64_t>(ne, 1);
     out->from = (uint32_t*)alloc_host_result(nb * 4);
@@ -1086,8 +1165,9 @@ rbg_status save_resident(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_p
                                       n, hipMemcpyDeviceToDevice, st));
   }
   const int regime = (int)b->desc_h[a].regime;
+  const bool dense = b->desc_h[a].dense != 0;
   uint64_t nlist = 0;
-  if (regime == 2) {
+  if (!dense) {
     nlist = b->bounds_h[4 * a + 3] - b->bounds_h[4 * a + 2];
     RBG_TRY_HIP(ctx, fit(r->loff, ((size_t)n + 1) * 8));
     RBG_TRY_HIP(ctx, fit(r->list, std::max<uint64_t>(nlist, 1) * 8));
@@ -1109,6 +1189,7 @@ rbg_status save_resident(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph_p
   r->n = n;
   r->S = S;
   r->regime = regime;
+  r->dense = dense;
   r->perm = std::move(h.perm_local);
   r->anynz = std::move(h.anynz_local);
   r->colsum = std::move(h.colsum);

@@ -90,7 +90,8 @@ struct Resident {
   DevBuf q8;                  // n x kpad_for(S) i8 in pair-pass order (S > 1)
   DevBuf loff, list;          // regime 2: gated pairs of every row, CSR
   uint64_t nlist = 0;
-  DevBuf mat;                 // regime 1: n x n match counts (the diagonal unused)
+  DevBuf mat;                 // dense counts: n x n match counts (the diagonal unused)
+  bool dense = false;         // the last build counted pairs into mat (no candidate lists)
   std::vector<uint32_t> perm; // pair-pass position -> local row
   std::vector<uint8_t> anynz;
   std::vector<double> colsum; // column sums of the raw depths, accumulated in row order
@@ -101,10 +102,24 @@ struct Resident {
   }
 };
 
+// Sketches and depth rows a caller supplies instead of contigs and depths
+// (rbg_build_graph_from_sketches): per-register class bytes n x mpad (0xFF
+// unset), unit/raw rows n x S, column norms, anynz, and b0 (NaN: estimate).
+struct SketchInput {
+  const uint8_t* classes = nullptr;
+  uint32_t mpad = 0;
+  const float* unit = nullptr;
+  const float* raw = nullptr;
+  const double* colnorm = nullptr;
+  const uint8_t* anynz = nullptr;
+  double b0 = __builtin_nan("");
+};
+
 // One assembly of a build.
 struct AsmInput {
   rbg_assembly_in in{};          // host contigs (and depth): the new rows
   const Resident* src = nullptr; // rows [0, src->n) come from here when set
+  const SketchInput* sk = nullptr;  // set: in.seqs/in.depth are unused
 };
 
 // Host side of one assembly of a batch.
@@ -112,6 +127,8 @@ struct AsmHost {
   uint32_t n = 0, S = 0, row0 = 0, colnorm0 = 0, pair0 = 0, np = 0;
   uint32_t n_old = 0;             // rows taken from src
   const Resident* src = nullptr;
+  double b0_in = __builtin_nan("");  // b0 given by the caller (NaN: estimated on the device)
+  uint32_t n_zero = 0;            // rows with anynz = 0 (they pass is_nz only with a nonzero partner)
   bool incr = false;              // fused pass over pairs with a new row, merged with src's lists
   bool dense_incr = false;        // small regime: counts of pairs with a new row, src's matrix kept
   GateQuant gq;                   // int8 row statistics (the rows live on the device)
@@ -126,17 +143,33 @@ struct PlanKey {
   uint32_t m = 0, K = 0, rand_seed = 0, tiles = 0, rows = 0;
   int regime = -1;
   uint64_t pmh_seed = 0, cand_cap = 0;
-  double min_edge = 0, coverage = 0, neg_thr = 0;
+  double min_edge = 0, coverage = 0, neg_thr = 0, near_tol = 0;
   bool operator==(const PlanKey& o) const {
     return m == o.m && K == o.K && rand_seed == o.rand_seed && tiles == o.tiles && rows == o.rows &&
            regime == o.regime && pmh_seed == o.pmh_seed && cand_cap == o.cand_cap &&
-           min_edge == o.min_edge && coverage == o.coverage && neg_thr == o.neg_thr;
+           min_edge == o.min_edge && coverage == o.coverage && neg_thr == o.neg_thr &&
+           near_tol == o.near_tol;
   }
 };
 
 // Regime of an assembly of n contigs under `regime` (0 = auto: n > 25000 fused,
 // src/rabbitbin.cpp:5035-5036).
 inline int regime_for(int regime, uint32_t n) { return regime ? regime : (n > 25000 ? 2 : 1); }
+// Composition weight auto_w_comp(S) (src/rabbitbin.cpp:470-473).
+inline double w_comp_for(uint32_t S) { return S <= 2 ? 1.0 : 0.0; }
+// Whether the fused pass applies the abundance gate (src/impl/rb_graph.cpp:693-695).
+inline bool gate_on(int regime, uint32_t S, double min_edge) {
+  const double wc = w_comp_for(S);
+  return regime == 2 && S > 1 && wc < 1.0 && wc < min_edge;
+}
+// Whether an assembly's pair counts go through the n x n matrix: the small
+// regime, and the fused regime without the gate (every pair is a candidate,
+// so the matrix, 2 bytes per pair, is cheaper than candidate lists).
+inline bool dense_counts(int regime, uint32_t S, double min_edge) {
+  return regime == 1 || !gate_on(regime, S, min_edge);
+}
+// Largest n x n count matrix of one assembly (n <= 185,363).
+constexpr uint64_t kDenseMaxBytes = 64ull << 30;
 
 }  // namespace rbg
 
@@ -158,8 +191,9 @@ struct rbg_asm {
   uint32_t n_incr = 0;        // incremental assemblies (their list merge descriptors)
   uint32_t merge_max_old = 0; // most earlier rows of an incremental assembly
   uint64_t list_extra = 0;    // candidate-list entries merged from sessions
+  uint32_t cls_rows = 0, cls_mpad = 0;  // rows given as class bytes (graph from sketches)
   rbg::DevBuf seqs, offs, sk_rows, asm_of, unit_pad, raw_pad, colnorm, anynz, q8, perm, pi, pj,
-      pcnt, merge;
+      pcnt, merge, cls_in;
   // plan of the current configuration
   bool planned = false;
   rbg::PlanKey key;
@@ -171,7 +205,9 @@ struct rbg_asm {
   rbg_schedule sched{16, 64};
   rbg::DevBuf desc, bscal, svd, svf, level, cal_rnd, cal_members, cal_roff, dense_items, gate_items;
   rbg::DevBuf wx, planes, mat, cand, counters, deg, slot, off, list, topk_id, topk_cnt, topk_len,
-      rowmax, fence, keep, row_edges, row_off, efrom, eto, ecnt, escomp, escore, scan_tmp, bounds;
+      rowmax, fence, keep, row_edges, row_off, efrom, eto, ecnt, escomp, escore, scan_tmp, bounds,
+      near_e, near_w;
+  uint64_t near_cap = 0;  // near-cut list entries (edge index, FP64 weight)
   // captured build
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -179,6 +215,8 @@ struct rbg_asm {
   // host copies of the last build's per-assembly results
   std::vector<rbg::BuildScalars> bs_h;
   std::vector<uint64_t> bounds_h;
+  std::vector<uint64_t> near_e_h;  // the last build's near-cut edges (global edge index)
+  std::vector<double> near_w_h;
   uint64_t ncand_last = 0;
   bool built = false;
   void drop_graph() {

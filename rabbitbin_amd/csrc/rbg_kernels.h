@@ -41,6 +41,10 @@ struct AsmDesc {
   uint32_t jmin;      // regime 1: pairs with j < jmin (local) were counted by an earlier build
   uint64_t mat0;      // regime 1: first entry of the assembly's n x n count matrix
   double w_comp;      // composition weight
+  double b0_in;       // b0 given by the caller; NaN: the median of the pair draws
+  uint32_t dense;     // 1: every pair counted into the n x n matrix (regime 1, and regime 2
+                      // without the abundance gate); 0: gated candidate lists
+  uint32_t pad0;
 };
 
 // Per-assembly tables: svd/svf/level hold m+1 entries per assembly; the
@@ -55,6 +59,10 @@ hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, 
                          uint32_t W, const double* sT, const uint8_t* sX, const uint8_t* cls,
                          const uint8_t* codes, uint8_t* wx, uint32_t mpad, uint32_t* planes,
                          const uint32_t* out_row, hipStream_t st);
+// Bit planes of rows [row0, row0 + n) from per-register class bytes (n x mpad,
+// registers >= m and unset ones 0xFF).
+hipError_t launch_planes_from_classes(const uint8_t* cls, uint32_t n, uint32_t m, uint32_t mpad,
+                                      uint32_t W, uint32_t row0, uint32_t* planes, hipStream_t st);
 hipError_t launch_list_counts(const uint32_t* planes, uint32_t W, const uint32_t* pi,
                               const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st);
 hipError_t launch_win16_counts(const uint16_t* win16, uint32_t m, const uint32_t* pi,
@@ -156,13 +164,19 @@ hipError_t launch_emit_write(const uint32_t* topk_id, const uint16_t* topk_cnt,
 
 // ---- stage (d)
 // Edge weights of the min(*ne, cap) edges of a batch: unit/raw rows padded to
-// 16*NC floats, each edge with its assembly's S, w_comp and colnorm.
+// 16*NC floats, each edge with its assembly's S, w_comp and colnorm.  The
+// correlation is summed in the reference's order; edges whose FP64 weight or
+// correlation lies within near_tol of a cut are appended (edge index, FP64
+// weight) to near_edge / near_w64 at *near_cnt (entries past near_cap are
+// counted, not written).
 hipError_t launch_edge_weights_batch(const uint32_t* efrom, const uint32_t* eto,
                                      const float* scomp, uint64_t cap, const uint64_t* ne,
                                      const float* unit, const float* raw, uint32_t NC,
                                      const double* colnorm, const AsmDesc* desc,
                                      const uint16_t* asm_of, double min_edge, double neg_thr,
-                                     float* score, hipStream_t st);
+                                     double near_tol, float* score, unsigned long long* near_cnt,
+                                     uint64_t near_cap, uint64_t* near_edge, double* near_w64,
+                                     hipStream_t st);
 // Edge weights of an explicit edge list (unpadded n x S rows, host count).
 hipError_t launch_edge_weights(const uint32_t* efrom, const uint32_t* eto, const float* scomp,
                                uint64_t ne, const float* unit, const float* raw,

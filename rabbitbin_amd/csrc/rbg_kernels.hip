@@ -430,6 +430,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
   }
 }
 
+// Bit planes from class bytes: one wave per (row, 64-register span); lane l
+// holds register 64s + l, and each plane's ballot gives two plane words.
+__global__ __launch_bounds__(256) void planes_from_classes_kernel(const uint8_t* __restrict__ cls,
+                                                                  uint32_t n, uint32_t m, uint32_t mpad,
+                                                                  uint32_t W, uint32_t row0,
+                                                                  uint32_t* __restrict__ planes) {
+  const uint32_t spans = (32u * W + 63u) / 64u;
+  const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const uint32_t ln = threadIdx.x & 63u;
+  if (wave >= (uint64_t)n * spans) return;  // wave-uniform
+  const uint32_t r = (uint32_t)(wave / spans), sp = (uint32_t)(wave % spans);
+  const uint32_t k = 64u * sp + ln;
+  const uint32_t code = k < m ? cls[(size_t)r * mpad + k] : 0xFFu;
+  const uint32_t w0 = 2u * sp + (ln >> 5);
+  uint32_t* row = planes + (size_t)(row0 + r) * 8u * W;
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    const uint64_t bits = __ballot((code >> p) & 1u);
+    if ((ln == 0 || ln == 32) && w0 < W) row[plane_word(w0, p, W)] = ln == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
+  }
+}
+
 // Counts for an explicit list of pairs (b0 sampling).
 __global__ __launch_bounds__(256) void list_counts_kernel(const uint32_t* __restrict__ planes,
                                                           uint32_t W, const uint32_t* __restrict__ pi,
@@ -1000,12 +1022,13 @@ __global__ __launch_bounds__(256) void extract_lists_kernel(const uint64_t* __re
 // count matrix (regime 1; partners are all rows j != i of the assembly that
 // pass is_nz) or its candidate list (regime 2).  Ids are global rows.
 struct RowSource {
-  const uint64_t* list;   // regime 2
+  const uint64_t* list;   // candidate lists
   const uint64_t* off;
-  const uint16_t* mrow;   // regime 1: the row's counts against the assembly's rows
+  const uint16_t* mrow;   // dense: the row's counts against the assembly's rows
   const uint8_t* anynz;
   uint32_t row0, local, n;
   bool dense;
+  bool small;             // regime 1: the row maximum runs over every partner
 };
 
 __device__ __forceinline__ RowSource row_source(const uint64_t* list, const uint64_t* off,
@@ -1018,7 +1041,8 @@ __device__ __forceinline__ RowSource row_source(const uint64_t* list, const uint
   s.row0 = d.row0;
   s.local = i - d.row0;
   s.n = d.n;
-  s.dense = d.regime == 1;
+  s.dense = d.dense != 0;
+  s.small = d.regime == 1;
   s.mrow = s.dense ? mat + d.mat0 + (size_t)s.local * d.n : nullptr;
   return s;
 }
@@ -1032,8 +1056,11 @@ __device__ __forceinline__ bool row_entry(const RowSource& s, uint32_t i, uint32
   if (s.dense) {
     id = s.row0 + t;
     c = s.mrow[t];
-    for_max = t != s.local;
-    return t != s.local && (!s.anynz || s.anynz[i] || s.anynz[id]);
+    const bool nz = !s.anynz || s.anynz[i] || s.anynz[id];
+    // the small regime's calibration takes every partner (calibrate_sim_cutoff,
+    // src/impl/rb_graph.cpp:1362-1384); the fused pass only evaluated ones
+    for_max = t != s.local && (s.small || nz);
+    return t != s.local && nz;
   }
   const uint64_t e = s.list[s.off[i] + t];
   id = (uint32_t)e;
@@ -1480,9 +1507,130 @@ __global__ __launch_bounds__(256) void emit_write_kernel(
   }
 }
 
-// Stage (d): per-edge weight in FP64 (the reference's depth_edge_term with
-// the default fuse = min, src/rabbitbin.cpp:930-970, and the combination and
-// cutoff of src/rabbitbin.cpp:5500-5507).
+// Stage (d): per-edge weight, the reference's depth_edge_term with the
+// default fuse = min (src/rabbitbin.cpp:930-970) and the combination and
+// cutoff of src/rabbitbin.cpp:5500-5507.
+//
+// The correlation term is summed as depth_corr_fast sums it on the
+// reference's AVX-512 build (src/rabbitbin.cpp:705-746): the whole 16-float
+// chunks by per-lane FP32 FMAs (chunk pairs of a 32-wide step go to two
+// accumulators), one reduce in GCC's _mm512_reduce_add_ps order (lanes l+8,
+// then l+4, then (0,2),(1,3), then the last pair), then the remaining S % 16
+// products in FP64.  Below 16 samples that is the plain FP64 sum.  The same
+// sum in FP64 throughout is the FP64 weight the near-cut list reports.
+
+// corr in the reference order over two rows given by pointers (any S).
+__device__ __forceinline__ double corr_ref_ptr(const float* __restrict__ a, const float* __restrict__ b,
+                                               uint32_t S) {
+  float acc0[16], acc1[16];
+#pragma unroll
+  for (int l = 0; l < 16; ++l) acc0[l] = acc1[l] = 0.f;
+  uint32_t k = 0;
+  for (; k + 32 <= S; k += 32) {
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+      acc0[l] = __builtin_fmaf(a[k + l], b[k + l], acc0[l]);
+      acc1[l] = __builtin_fmaf(a[k + 16 + l], b[k + 16 + l], acc1[l]);
+    }
+  }
+  for (; k + 16 <= S; k += 16) {
+#pragma unroll
+    for (int l = 0; l < 16; ++l) acc0[l] = __builtin_fmaf(a[k + l], b[k + l], acc0[l]);
+  }
+  double s = 0.0;
+  if (k > 0) {
+    float t8[8], t4[4];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) t8[l] = (acc0[l + 8] + acc1[l + 8]) + (acc0[l] + acc1[l]);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) t4[l] = t8[l + 4] + t8[l];
+    s = (double)((t4[0] + t4[2]) + (t4[1] + t4[3]));
+  }
+  for (; k < S; ++k) s += (double)a[k] * (double)b[k];
+  return s;
+}
+
+// The same over a row held in registers (N = 16 NC floats) and a staged row;
+// returns the reference-order sum, and the FP64 sum in c64.
+template <int N>
+__device__ __forceinline__ double corr_ref_regs(const float (&a)[N], const float* __restrict__ b, uint32_t S,
+                                                double& c64) {
+  float acc0[16], acc1[16];
+#pragma unroll
+  for (int l = 0; l < 16; ++l) acc0[l] = acc1[l] = 0.f;
+  const uint32_t nfull = S >> 4, n32 = 2u * (S >> 5);
+#pragma unroll
+  for (int c = 0; c < N / 16; ++c) {
+    if ((uint32_t)c < nfull) {
+      if ((c & 1) && (uint32_t)c < n32) {
+#pragma unroll
+        for (int l = 0; l < 16; ++l) acc1[l] = __builtin_fmaf(a[16 * c + l], b[16 * c + l], acc1[l]);
+      } else {
+#pragma unroll
+        for (int l = 0; l < 16; ++l) acc0[l] = __builtin_fmaf(a[16 * c + l], b[16 * c + l], acc0[l]);
+      }
+    }
+  }
+  double s = 0.0;
+  if (nfull > 0) {
+    float t8[8], t4[4];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) t8[l] = (acc0[l + 8] + acc1[l + 8]) + (acc0[l] + acc1[l]);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) t4[l] = t8[l + 4] + t8[l];
+    s = (double)((t4[0] + t4[2]) + (t4[1] + t4[3]));
+  }
+  double f = 0.0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if ((uint32_t)k < S) {
+      const double p = (double)a[k] * (double)b[k];  // exact
+      f += p;
+      if ((uint32_t)k >= 16u * nfull) s += p;
+    }
+  }
+  c64 = f;
+  return s;
+}
+
+__device__ __forceinline__ bool finite64(double x) {
+  return x == x && x != __builtin_huge_val() && x != -__builtin_huge_val();
+}
+
+// Weighted Jaccard of column-normalised raw depths (src/rabbitbin.cpp:685-698),
+// clamped to [0, 1] as depth_edge_term clamps it.
+__device__ __forceinline__ double wjac_clamped(double mn, double mx) {
+  double wj = mx <= 0.0 ? 0.0 : mn / mx;
+  if (!(wj == wj)) wj = 0.0;
+  return wj < 0.0 ? 0.0 : (wj > 1.0 ? 1.0 : wj);
+}
+
+// Edge weight from the correlation and the weighted Jaccard; `pre` is the
+// weight before the min-edge cut (0 when the correlation cuts the edge).
+__device__ __forceinline__ double edge_weight_of(double corr, double wj, double sc, double w_comp,
+                                                 double min_edge, double neg_thr, double& pre) {
+  pre = 0.0;
+  if (!finite64(corr) || (neg_thr > -0.99 && corr < neg_thr)) return 0.0;
+  const double cp = corr < 0.0 ? 0.0 : corr;
+  const double d = cp < wj ? cp : wj;
+  double w = w_comp * sc + (1.0 - w_comp) * d;
+  pre = w;
+  if (!(w == w) || w < min_edge) w = 0.0;
+  return w;
+}
+
+// Whether the FP64 weight of an edge lies within tol of a cut: its
+// correlation within tol of the negative cut, or its weight before the
+// min-edge cut within tol of that cut.
+__device__ __forceinline__ bool near_cut(double corr64, double pre64, double min_edge, double neg_thr,
+                                         double tol) {
+  const bool ncorr = neg_thr > -0.99 && fabs(corr64 - neg_thr) <= tol;
+  const bool nw = fabs(pre64 - min_edge) <= tol;
+  return ncorr || nw;
+}
+
+// rbg_edge_weights: one edge per thread, rows unpadded (any S).  score is
+// the reference's weight, score64 the weight with the correlation in FP64.
 __global__ __launch_bounds__(256) void edge_weight_kernel(
     const uint32_t* __restrict__ efrom, const uint32_t* __restrict__ eto,
     const float* __restrict__ scomp, uint64_t ne, const float* __restrict__ unit,
@@ -1492,22 +1640,18 @@ __global__ __launch_bounds__(256) void edge_weight_kernel(
   const uint64_t cnt = ne;
   for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < cnt;
        e += (uint64_t)gridDim.x * 256) {
-  const uint32_t i = efrom[e], j = eto[e];
-  double w;
-  if (S == 0) {
-    w = (double)scomp[e] < min_edge ? 0.0 : (double)scomp[e];
-  } else if (S == 1) {
-    w = scomp[e];
-  } else {
-    const float* a = unit + (size_t)i * S;
-    const float* b = unit + (size_t)j * S;
-    double corr = 0.0;
-    for (uint32_t k = 0; k < S; ++k) corr += (double)a[k] * (double)b[k];
-    const bool finite = corr == corr && corr != __builtin_huge_val() && corr != -__builtin_huge_val();
-    if (!finite || (neg_thr > -0.99 && corr < neg_thr)) {
-      w = 0.0;
+    const uint32_t i = efrom[e], j = eto[e];
+    double w, w64;
+    if (S == 0) {
+      w = w64 = (double)scomp[e] < min_edge ? 0.0 : (double)scomp[e];
+    } else if (S == 1) {
+      w = w64 = scomp[e];
     } else {
-      const double cp = corr < 0.0 ? 0.0 : corr;
+      const float* a = unit + (size_t)i * S;
+      const float* b = unit + (size_t)j * S;
+      const double corr = corr_ref_ptr(a, b, S);
+      double corr64 = 0.0;
+      for (uint32_t k = 0; k < S; ++k) corr64 += (double)a[k] * (double)b[k];
       const float* ra = raw + (size_t)i * S;
       const float* rb = raw + (size_t)j * S;
       double mn = 0.0, mx = 0.0;
@@ -1525,23 +1669,20 @@ __global__ __launch_bounds__(256) void edge_weight_kernel(
           mx += x;
         }
       }
-      double wj = mx <= 0.0 ? 0.0 : mn / mx;
-      if (!(wj == wj)) wj = 0.0;
-      wj = wj < 0.0 ? 0.0 : (wj > 1.0 ? 1.0 : wj);
-      const double d = cp < wj ? cp : wj;
-      w = w_comp * (double)scomp[e] + (1.0 - w_comp) * d;
-      if (!(w == w) || w < min_edge) w = 0.0;
+      const double wj = wjac_clamped(mn, mx);
+      double pre;
+      w = edge_weight_of(corr, wj, scomp[e], w_comp, min_edge, neg_thr, pre);
+      w64 = edge_weight_of(corr64, wj, scomp[e], w_comp, min_edge, neg_thr, pre);
     }
-  }
-  score[e] = (float)w;
-  if (score64) score64[e] = w;
+    score[e] = (float)w;
+    if (score64) score64[e] = w64;
   }
 }
 
-// The same weight for the edges of a batch, each with its assembly's S
+// The weights of the edges of a batch, each with its assembly's S
 // (S <= 16*NC), w_comp and colnorm; unit and raw rows are zero-padded to
-// 16*NC floats (128-byte aligned rows).  Per edge the FP64 sums run in the
-// same order as above, so the results are identical.
+// 16*NC floats (128-byte aligned rows).  Edges whose FP64 weight lies within
+// near_tol of a cut are appended to the near list (edge index, FP64 weight).
 //
 // A wave takes 64 consecutive edges, one per lane.  Edges are sorted by
 // `from`, so the lanes' `from` rows mostly coincide and are read directly; the
@@ -1578,8 +1719,9 @@ __global__ __launch_bounds__(64 * kEwWaves) void edge_weight_pad_kernel(
     const float* __restrict__ scomp, uint64_t cap, const uint64_t* __restrict__ ne_dev,
     const float* __restrict__ unitp, const float* __restrict__ rawp,
     const double* __restrict__ colnorm_all, const AsmDesc* __restrict__ desc,
-    const uint16_t* __restrict__ asm_of, double min_edge, double neg_thr,
-    float* __restrict__ score) {
+    const uint16_t* __restrict__ asm_of, double min_edge, double neg_thr, double near_tol,
+    float* __restrict__ score, unsigned long long* __restrict__ near_cnt, uint64_t near_cap,
+    uint64_t* __restrict__ near_edge, double* __restrict__ near_w64) {
   constexpr int V = 4 * NC;          // float4 per row
   constexpr int STR = 16 * NC + 1;
   __shared__ float stage[kEwWaves][64 * STR];
@@ -1606,18 +1748,18 @@ __global__ __launch_bounds__(64 * kEwWaves) void edge_weight_pad_kernel(
         a[4 * v] = x.x, a[4 * v + 1] = x.y, a[4 * v + 2] = x.z, a[4 * v + 3] = x.w;
       }
     }
-    double corr = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4 * V; ++k)
-      if ((uint32_t)k < S) corr += (double)a[k] * (double)mine[k];
-    const bool finite = corr == corr && corr != __builtin_huge_val() && corr != -__builtin_huge_val();
-    const bool depth = live && S > 1 && finite && !(neg_thr > -0.99 && corr < neg_thr);
+    double corr64;
+    const double corr = corr_ref_regs<4 * V>(a, mine, S, corr64);
+    const bool cut = !finite64(corr) || (neg_thr > -0.99 && corr < neg_thr);
+    const bool cut64 = !finite64(corr64) || (neg_thr > -0.99 && corr64 < neg_thr);
+    const bool depth = live && S > 1 && !(cut && cut64);  // a weighted Jaccard is needed
     __builtin_amdgcn_wave_barrier();  // every lane has read its staged unit row
     const bool any_depth = __ballot(depth) != 0;
     if (any_depth) stage_rows<NC>(rawp, j, ln, area);  // wave-uniform
     double w = 0.0;
+    bool near = false;
+    double w64 = 0.0;
     if (depth) {
-      const double cp = corr < 0.0 ? 0.0 : corr;
       const double* __restrict__ colnorm = colnorm_all + ad.colnorm0;
       const float4* ra = reinterpret_cast<const float4*>(rawp + (size_t)i * 16 * NC);
 #pragma unroll
@@ -1643,17 +1785,26 @@ __global__ __launch_bounds__(64 * kEwWaves) void edge_weight_pad_kernel(
           }
         }
       }
-      double wj = mx <= 0.0 ? 0.0 : mn / mx;
-      if (!(wj == wj)) wj = 0.0;
-      wj = wj < 0.0 ? 0.0 : (wj > 1.0 ? 1.0 : wj);
-      const double d = cp < wj ? cp : wj;
-      w = w_comp * (double)scomp[e] + (1.0 - w_comp) * d;
-      if (!(w == w) || w < min_edge) w = 0.0;
-    } else if (live && S <= 1) {  // no depth: the composition similarity alone
+      const double wj = wjac_clamped(mn, mx);
+      const double sc = (double)scomp[e];
+      double pre, pre64;
+      w = edge_weight_of(corr, wj, sc, w_comp, min_edge, neg_thr, pre);
+      w64 = edge_weight_of(corr64, wj, sc, w_comp, min_edge, neg_thr, pre64);
+      near = near_cut(corr64, pre64, min_edge, neg_thr, near_tol);
+    } else if (live && S > 1) {  // both correlations cut the edge
+      near = neg_thr > -0.99 && fabs(corr64 - neg_thr) <= near_tol;
+    } else if (live) {  // S <= 1: the composition similarity alone (exact)
       const double sc = (double)scomp[e];
       w = S == 0 && sc < min_edge ? 0.0 : sc;
     }
     if (live) score[e] = (float)w;
+    if (near) {
+      const unsigned long long t = atomicAdd(near_cnt, 1ull);
+      if (t < near_cap) {
+        near_edge[t] = e;
+        near_w64[t] = w64;
+      }
+    }
     __builtin_amdgcn_wave_barrier();  // the staged rows are read before the next edges' load
   }
 }
@@ -1703,6 +1854,8 @@ __global__ __launch_bounds__(256) void baseline_tables_kernel(const uint32_t* __
       }
       b0 = (double)(float)((double)c * inv_m);
     }
+    const double given = desc[a].b0_in;
+    if (given == given) b0 = given;  // the caller's b0
     b0s = b0;
     bs->b0 = b0;
   }
@@ -1962,6 +2115,15 @@ hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, 
   return hipGetLastError();
 }
 
+hipError_t launch_planes_from_classes(const uint8_t* cls, uint32_t n, uint32_t m, uint32_t mpad,
+                                      uint32_t W, uint32_t row0, uint32_t* planes, hipStream_t st) {
+  const uint64_t waves = (uint64_t)n * ((32u * W + 63u) / 64u);
+  if (waves == 0) return hipSuccess;
+  hipLaunchKernelGGL(planes_from_classes_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, cls, n,
+                     m, mpad, W, row0, planes);
+  return hipGetLastError();
+}
+
 hipError_t launch_list_counts(const uint32_t* planes, uint32_t W, const uint32_t* pi,
                               const uint32_t* pj, uint32_t np, uint32_t* out, hipStream_t st) {
   if (np == 0) return hipSuccess;
@@ -1982,7 +2144,7 @@ void dense_work_items(const std::vector<AsmDesc>& desc, uint32_t rows_per_item,
                       std::vector<uint32_t>& items) {
   items.clear();
   for (uint32_t a = 0; a < (uint32_t)desc.size(); ++a) {
-    if (desc[a].regime != 1) continue;
+    if (!desc[a].dense) continue;
     const uint32_t n = desc[a].n;
     for (uint32_t rb = 0; rb * rows_per_item < n; ++rb)
       for (uint32_t cb = 0; cb * 256 < n; ++cb) {
@@ -2147,12 +2309,15 @@ hipError_t launch_edge_weights_batch(const uint32_t* efrom, const uint32_t* eto,
                                      const float* unit, const float* raw, uint32_t NC,
                                      const double* colnorm, const AsmDesc* desc,
                                      const uint16_t* asm_of, double min_edge, double neg_thr,
-                                     float* score, hipStream_t st) {
+                                     double near_tol, float* score, unsigned long long* near_cnt,
+                                     uint64_t near_cap, uint64_t* near_edge, double* near_w64,
+                                     hipStream_t st) {
   if (cap == 0) return hipSuccess;
   const dim3 grid((unsigned)std::min<uint64_t>((cap + 64 * kEwWaves - 1) / (64 * kEwWaves), 65536));
 #define RBG_EW(NCV)                                                                                   \
   hipLaunchKernelGGL(edge_weight_pad_kernel<NCV>, grid, dim3(64 * kEwWaves), 0, st, efrom, eto, scomp, \
-                     cap, ne, unit, raw, colnorm, desc, asm_of, min_edge, neg_thr, score)
+                     cap, ne, unit, raw, colnorm, desc, asm_of, min_edge, neg_thr, near_tol, score,   \
+                     near_cnt, near_cap, near_edge, near_w64)
   switch (NC) {
     case 1: RBG_EW(1); break;
     case 2: RBG_EW(2); break;

@@ -56,10 +56,10 @@ struct Session {
 };
 
 // Resident bytes per row of a session (planes, padded depth rows, int8 rows,
-// list offsets and an allowance of 2K list entries per row); a small-regime
-// session also keeps its n x n counts (resident_matrix_bytes).
-size_t resident_matrix_bytes(uint32_t n, const rbg_graph_params& P) {
-  return regime_for(P.regime, n) == 1 ? (size_t)n * n * 2 : 0;
+// list offsets and an allowance of 2K list entries per row); a session whose
+// pairs are counted densely also keeps its n x n counts (resident_matrix_bytes).
+size_t resident_matrix_bytes(uint32_t n, uint32_t S, const rbg_graph_params& P) {
+  return dense_counts(regime_for(P.regime, n), S, P.min_edge_weight) ? (size_t)n * n * 2 : 0;
 }
 size_t resident_row_bytes(uint32_t S, const rbg_graph_params& P) {
   const size_t W = words_for(P.m);
@@ -179,7 +179,7 @@ void rbg_server::select(std::vector<Request*>& step) {
     x.src = r->extend ? &s->res : nullptr;
     ins.push_back(x);
     // growth of the session's state (its buffers grow with 25% headroom)
-    const size_t after = ((size_t)n * resident_row_bytes(r->in.S, P) + resident_matrix_bytes(n, P)) * 5 / 4;
+    const size_t after = ((size_t)n * resident_row_bytes(r->in.S, P) + resident_matrix_bytes(n, r->in.S, P)) * 5 / 4;
     const size_t g = after - std::min<size_t>(s->res.device_bytes(), after);
     const size_t need = resident + growth + g + estimate_build_bytes(ins.data(), (uint32_t)ins.size(), &P);
     if (need > cfg.hbm_budget) {
@@ -252,6 +252,17 @@ void rbg_server::build_step(std::vector<Request*>& step, int w) {
   job.t_save = now();
   {
     std::lock_guard<std::mutex> lk(mu);
+    // A failed build or save leaves a session without (or with partly
+    // refitted) device state: mark it failed now, before the next select()
+    // could admit one of its queued requests against that state.
+    for (uint32_t k = 0; k < A; ++k) {
+      const rbg_status sk = st != RBG_OK ? st : job.saved[k];
+      if (sk == RBG_OK) continue;
+      auto it = sessions.find(step[k]->session);
+      if (it == sessions.end()) continue;
+      it->second->failed = true;
+      it->second->fail_msg = st != RBG_OK ? (job.msg.empty() ? "build failed" : job.msg) : job.save_msg[k];
+    }
     jobs.push_back(std::move(job));
   }
   cv_jobs.notify_one();
@@ -382,6 +393,8 @@ rbg_status rbg_server_start(int device, const rbg_server_config* cfg, rbg_server
   *out = nullptr;
   const rbg_graph_params& P = cfg->params;
   if (P.m < 2 || P.m > 512 || P.max_edges == 0 || P.max_edges > 1024) return RBG_ERR_UNSUPPORTED;
+  if (P.semantics) return RBG_ERR_UNSUPPORTED;  // non-default reference semantics (rbg_env_semantics)
+  if (!(P.near_tol >= 0.0 && P.near_tol <= 1.0)) return RBG_ERR_ARG;
   if (cfg->max_step_requests == 0 || cfg->max_step_requests > 65535 || cfg->max_step_rows == 0 ||
       cfg->max_step_rows >= (1u << 24))
     return RBG_ERR_ARG;

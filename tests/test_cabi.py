@@ -68,3 +68,40 @@ def test_host_entry_points_reject_bad_arguments():
     assert L.rbg_sketch_tables(1, 42, None, None, None) != 0  # m must be >= 2
     with pytest.raises(api.RbgError):
         api.calibrate(np.zeros(10, np.uint32), 1, 0.0, 1)
+
+
+def test_env_semantics_follow_reference_parsing(monkeypatch):
+    """rbg_env_semantics reports the RABBIT_* variables whose value changes
+    the path's results, parsed as the reference parses them (empty values
+    are unset for rb_getenv variables and set for getenv ones); the default
+    parameters carry the mask, and RABBIT_NEG_DEPTH becomes neg_depth_thr."""
+    from rabbitbin_amd import api
+    for k in api.SEMANTICS:
+        monkeypatch.delenv(k, raising=False)
+    monkeypatch.delenv("RABBIT_NEG_DEPTH", raising=False)
+    assert api.env_semantics() == (0, "")
+    assert api.default_params().semantics == 0
+    # values that keep the defaults
+    for k, v in (("RABBIT_MUTUAL_KNN", "1"), ("RABBIT_WIN_BITS", "16"), ("RABBIT_PMHK", "4"),
+                 ("RABBIT_GC_NORM_CAP", "0.5"), ("RABBIT_EDGE_CUT", "fixed"), ("RABBIT_DEPTH_SIM", "fuse"),
+                 ("RABBIT_DEPTH_FUSE", "min"), ("RABBIT_TILE", "64"), ("RABBIT_NO_COMP_EE", "1"),
+                 ("RABBIT_PMH", "")):
+        monkeypatch.setenv(k, v)
+    assert api.env_semantics() == (0, "")
+    # values that change the results
+    cases = [("RABBIT_MUTUAL_KNN", "0"), ("RABBIT_WIN_BITS", "32"), ("RABBIT_WIN_BITS", ""),
+             ("RABBIT_NO_ABDFIRST", ""), ("RABBIT_PMHK", "5"), ("RABBIT_GC_NORM_CAP", "10"),
+             ("RABBIT_EDGE_CUT", "otsu"), ("RABBIT_DEPTH_SIM", "wjac"), ("RABBIT_DEPTH_FUSE", "geo"),
+             ("RABBIT_W_COMP", "0.5"), ("RABBIT_EDGE_POWER", "2"), ("RABBIT_PMH", "0")]
+    for k, v in cases:
+        old = os.environ.get(k)
+        monkeypatch.setenv(k, v)
+        mask, names = api.env_semantics()
+        assert mask == api.SEMANTICS[k] and names == k, (k, v, mask, names)
+        assert api.default_params().semantics == mask
+        if old is None:
+            monkeypatch.delenv(k)
+        else:
+            monkeypatch.setenv(k, old)
+    monkeypatch.setenv("RABBIT_NEG_DEPTH", "-0.5")
+    assert api.default_params().neg_depth_thr == -0.5

@@ -14,7 +14,7 @@ def _edges(g, key="scomp"):
     return {(int(a), int(b)): float(s) for a, b, s in zip(g["from"], g["to"], g[key])}
 
 
-def _check_graph(g, ref):
+def _check_graph(g, ref, near_tol=1e-6):
     assert np.array_equal(g["win16"], ref["win16"])
     assert g["b0"] == ref["b0"]
     assert np.array_equal(g["rowmax_cnt"], ref["rowmax_cnt"])
@@ -27,9 +27,22 @@ def _check_graph(g, ref):
     eg = _edges(g)
     er = {(int(a), int(b)): float(s) for a, b, s in zip(ref["from"], ref["to"], ref["sv"])}
     assert eg == er
-    if g["score"] is not None:
-        d = np.abs(g["score"].astype(np.float64) - ref["score64"])
-        assert d.max() <= 1e-6
+    _check_scores(g, ref, near_tol)
+
+
+def _check_scores(g, ref, tol=1e-6):
+    """Edge weights: bit-equal to the reference's arithmetic (score_ref); the
+    near-cut list equal to the oracle's; off that list within tol of FP64,
+    and on it the FP64 weights equal the oracle's."""
+    if g["score"] is None:
+        return
+    assert np.array_equal(g["score"].view(np.uint32), ref["score_ref"].view(np.uint32))
+    assert np.array_equal(g["near_edge"], ref["near"])
+    off = np.ones(len(g["score"]), bool)
+    off[ref["near"].astype(np.int64)] = False
+    d = np.abs(g["score"].astype(np.float64) - ref["score64"])
+    assert len(d) == 0 or d[off].max(initial=0.0) <= tol
+    assert np.array_equal(g["near_score64"], ref["score64"][ref["near"].astype(np.int64)])
 
 
 def test_sketch_winners_exact(ctx):
@@ -49,9 +62,10 @@ def test_graph_small_regime_2k(ctx):
     _check_graph(g, ref)
 
 
-@pytest.mark.parametrize("S", [2, 3, 10, 30, 40, 64])
+@pytest.mark.parametrize("S", [0, 1, 2, 3, 10, 30, 40, 64])
 def test_graph_fused_regime_forced_3k(ctx, S):
-    # S selects the gate variant: 2 -> no gate (composition weight 1), 3..16
+    # S selects the gate variant: 0, 1 and 2 -> no gate (composition weight
+    # 1; every pair passing is_nz is counted into the dense matrix), 3..16
     # one 16-lane chunk, 30 two chunks in acc0, 40 a 32-wide chunk (acc1),
     # 64 two 32-wide chunks.
     seqs, off, depth, _ = synth.make_assembly(3000, 30, np.log(8000), 0.8, S, seed=99 + S)
@@ -60,6 +74,20 @@ def test_graph_fused_regime_forced_3k(ctx, S):
     ref = orc.full_pipeline(seqs, off, depth, rand_seed=3, force_regime=2)
     assert g["regime"] == 2
     assert g["gated_pairs"] == ref["gated_pairs"]
+    _check_graph(g, ref)
+
+
+@pytest.mark.parametrize("m,K,regime,S", [(100, 37, 2, 10), (256, 300, 2, 30), (512, 1024, 2, 16),
+                                         (64, 5, 1, 3), (33, 1, 2, 3), (16, 8, 2, 10), (500, 199, 1, 0)])
+def test_graph_sketch_sizes_and_max_edges(ctx, m, K, regime, S):
+    """Other --sketch-m / --max-edges: unaligned K (segment loads of the mutual
+    test), K > 256 (several keep-mask words, wider fence rows, a larger sort),
+    K = 1, every plane-word count W in {2, 4, 8, 16} of the compare kernels."""
+    seqs, off, depth, _ = synth.make_assembly(3000, 30, np.log(8000), 0.8, S, seed=1000 + m + K)
+    p = api.default_params(rand_seed=4, regime=regime, m=m, max_edges=K)
+    g = ctx.build_graph(seqs, off, depth, p, diag=True)
+    ref = orc.full_pipeline(seqs, off, depth, rand_seed=4, m=m, K=K, force_regime=regime)
+    assert g["regime"] == regime and g["gated_pairs"] == ref["gated_pairs"]
     _check_graph(g, ref)
 
 
@@ -108,15 +136,46 @@ def test_graph_natural_fused_regime_26k(ctx):
         frm, to, sc, score = asm.copy_edges(r["n_edges"])
         assert np.array_equal(frm, g["from"]) and np.array_equal(to, g["to"])
         assert np.array_equal(sc, g["scomp"]) and np.array_equal(score, g["score"])
+        assert np.array_equal(r["near_edge"], g["near_edge"])
     finally:
         asm.free()
 
 
+def test_non_default_semantics_refused(ctx, monkeypatch):
+    """A RABBIT_* variable that changes the path's results makes the build
+    refuse (the caller routes the run to the reference's CPU path)."""
+    seqs, off, depth, _ = synth.make_assembly(600, 12, np.log(8000), 0.8, 3, seed=5)
+    monkeypatch.setenv("RABBIT_MUTUAL_KNN", "0")
+    p = api.default_params(rand_seed=1)
+    assert p.semantics == api.RBG_SEM_NO_MUTUAL
+    with pytest.raises(api.RbgError, match="RABBIT_MUTUAL_KNN"):
+        ctx.build_graph(seqs, off, depth, p)
+    monkeypatch.delenv("RABBIT_MUTUAL_KNN")
+    g = ctx.build_graph(seqs, off, depth, api.default_params(rand_seed=1))
+    assert g["n_edges"] > 0
+
+
+@pytest.mark.parametrize("S", [16, 30])
+def test_near_cut_list(ctx, S):
+    """With a wide tolerance the near-cut list is long: it must equal the
+    oracle's, and every edge off it must agree with FP64 to that tolerance."""
+    seqs, off, depth, _ = synth.make_assembly(3000, 30, np.log(8000), 0.8, S, seed=99 + S)
+    # depths with multiplicative noise spread the weights over [0, 1] (the
+    # clean synthetic depths put nearly every weight above 0.9)
+    depth = (depth * np.random.default_rng(S).lognormal(0, 0.6, depth.shape)).astype(np.float32)
+    tol = 0.05
+    ref = orc.full_pipeline(seqs, off, depth, rand_seed=3, force_regime=2, near_tol=tol)
+    g = ctx.build_graph(seqs, off, depth, api.default_params(rand_seed=3, regime=2, near_tol=tol), diag=True)
+    assert len(ref["near"]) > 1000
+    _check_graph(g, ref, near_tol=tol)
+
+
 def test_edge_weights_entry_point(ctx):
-    """rbg_edge_weights on an arbitrary edge list == the FP64 oracle, bit for
-    bit in FP64, for S on both sides of the 16-wide chunking."""
+    """rbg_edge_weights on an arbitrary edge list: score bit-equal to the
+    reference's arithmetic and score64 to the FP64 oracle, for S on both sides
+    of the 16- and 32-wide chunking."""
     rng = np.random.default_rng(5)
-    for S in (2, 3, 16, 30, 70):
+    for S in (2, 3, 16, 30, 32, 47, 64, 70, 100):
         n = 500
         d = rng.poisson(rng.lognormal(1, 1, (n, 1)) * rng.lognormal(0, 1, (1, S)) * 4).astype(np.float32)
         _, unit, raw, colnorm, _ = orc.depth_features(d)
@@ -125,8 +184,9 @@ def test_edge_weights_entry_point(ctx):
         to = rng.integers(0, n, ne).astype(np.uint32)
         sc = rng.random(ne).astype(np.float32)
         w, w64 = ctx.edge_weights(fr, to, sc, unit, raw, colnorm, S, orc.w_comp_for(S))
-        rw, rw64 = orc.edge_weights(fr, to, sc, unit, raw, colnorm, S, mode=1)
-        assert np.array_equal(w64, rw64) and np.array_equal(w, rw)
+        _, rw64 = orc.edge_weights(fr, to, sc, unit, raw, colnorm, S, mode=1)
+        rw, _ = orc.edge_weights(fr, to, sc, unit, raw, colnorm, S, mode=0)
+        assert np.array_equal(w64, rw64) and np.array_equal(w.view(np.uint32), rw.view(np.uint32))
 
 
 def test_full_size_assembly_properties(ctx):
@@ -178,4 +238,46 @@ def test_full_size_assembly_properties(ctx):
         assert np.float32(svd[c]) == sc[e]
         assert not (orc.unit_dot_f(unit[i], unit[j]) < np.float32(0.7 - 1e-4))
     w, w64 = orc.edge_weights(frm[es], to[es], sc[es], unit, raw, colnorm, 30, mode=1)
-    assert np.abs(score[es].astype(np.float64) - w64).max() <= 1e-6
+    wr, _ = orc.edge_weights(frm[es], to[es], sc[es], unit, raw, colnorm, 30, mode=0)
+    assert np.array_equal(score[es].view(np.uint32), wr.view(np.uint32))
+    near = set(g["near_edge"].tolist())
+    keep = np.array([e not in near for e in es])
+    assert np.abs(score[es].astype(np.float64) - w64)[keep].max() <= 1e-6
+
+
+@pytest.mark.parametrize("S,regime", [(3, 1), (10, 2), (30, 2), (0, 1)])
+def test_graph_from_sketches(ctx, S, regime):
+    """rbg_build_graph_from_sketches fed the oracle's g_win16, depth rows and
+    anynz gives the oracle's graph (b0 estimated, and b0 given)."""
+    seqs, off, depth, _ = synth.make_assembly(3000, 30, np.log(8000), 0.8, S, seed=500 + S)
+    ref = orc.full_pipeline(seqs, off, depth, rand_seed=2, force_regime=regime)
+    kw = {}
+    if S:
+        _, unit, raw, colnorm, anynz = orc.depth_features(depth)
+        kw = dict(unit=unit, raw=raw, colnorm=colnorm if S > 1 else None, anynz=anynz)
+    p = api.default_params(rand_seed=2, regime=regime)
+    g = ctx.build_graph_from_sketches(ref["win16"], params=p, diag=True, **kw)
+    g["win16"] = ref["win16"]
+    assert g["regime"] == regime
+    _check_graph(g, ref)
+    g2 = ctx.build_graph_from_sketches(ref["win16"], b0=ref["b0"], params=p, **kw)
+    assert g2["b0"] == ref["b0"] and np.array_equal(g2["from"], g["from"]) and np.array_equal(g2["to"], g["to"])
+
+
+def test_graph_from_arbitrary_winners(ctx):
+    """Winner values that are not this library's own fold (random 16-bit
+    relabelling per register) give the same graph: the class numbering only
+    keeps equality."""
+    seqs, off, depth, _ = synth.make_assembly(2600, 25, np.log(8000), 0.8, 10, seed=77)
+    w32 = orc.sketch_batch(seqs, off)
+    w16 = orc.fold16(w32)
+    rng = np.random.default_rng(1)
+    relabel = rng.permutation(65535).astype(np.uint16) + 1  # non-zero -> non-zero, injective
+    wr = np.where(w16 == 0, 0, relabel[np.maximum(w16.astype(np.int64) - 1, 0)]).astype(np.uint16)
+    _, unit, raw, colnorm, anynz = orc.depth_features(depth)
+    b0 = orc.pmh_baseline(wr)
+    ref = orc.build_graph(wr, unit, anynz, 10, b0, rand_seed=1, force_regime=2)
+    g = ctx.build_graph_from_sketches(wr, unit=unit, raw=raw, colnorm=colnorm, anynz=anynz,
+                                      params=api.default_params(rand_seed=1, regime=2))
+    assert g["b0"] == b0
+    assert set(zip(g["from"].tolist(), g["to"].tolist())) == set(zip(ref["from"].tolist(), ref["to"].tolist()))

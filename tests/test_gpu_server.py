@@ -31,7 +31,9 @@ def _concat(parts):
 
 # sessions: (initial (n, genomes, S), extension sizes).  Covers the small
 # regime, small -> fused (24,500 + 2,000 > 25,000: a full fused pass), fused ->
-# fused (incremental), S = 0 and S = 1, and S on both sides of 16 and 32.
+# fused (incremental), S = 0 and S = 1, S on both sides of 16 and 32, and a
+# fused session without the abundance gate (S = 2: dense counts, extended
+# incrementally through its resident count matrix).
 SESSIONS = [
     ((3000, 30, 3), [2000]),
     ((26000, 200, 10), [2000, 2000]),
@@ -39,6 +41,7 @@ SESSIONS = [
     ((1000, 10, 0), [500]),
     ((1200, 12, 1), [800]),
     ((27000, 180, 40), [2000]),
+    ((25500, 150, 2), [2000]),
 ]
 
 
@@ -176,3 +179,4 @@ def test_server_stop_while_busy():
     t0 = time.time()
     srv.stop()
     assert time.time() - t0 < 60
+

@@ -433,6 +433,28 @@ def test_unit_dot_emulation_equals_avx512():
             assert abs(float(e) - float(np.dot(a.astype(np.float64), b))) < 1e-5
 
 
+def test_depth_corr_emulation_equals_avx512():
+    """The oracle's edge-weight correlation in the reference order
+    (depth_corr_fast: FP32 chunks of 32/16 reduced once, FP64 tail) == the
+    same loop with AVX-512 intrinsics, bit for bit, for S = 1..70; below 16 it
+    is the FP64 sum."""
+    rng = np.random.default_rng(16)
+    if orc.depth_corr_avx512(np.ones(3, np.float32), np.ones(3, np.float32)) is None:
+        pytest.skip("host lacks AVX-512F")
+    for S in range(1, 71):
+        for _ in range(60):
+            a = rng.normal(size=S).astype(np.float32)
+            b = (a * rng.uniform(-1.5, 1.5) + rng.normal(scale=0.5, size=S)).astype(np.float32)
+            a /= np.float32(np.linalg.norm(a))
+            b /= np.float32(np.linalg.norm(b))
+            e = orc.depth_corr(a, b, 0)
+            v = orc.depth_corr_avx512(a, b)
+            assert np.float64(e).view(np.uint64) == np.float64(v).view(np.uint64), (S, e, v)
+            if S < 16:
+                assert e == orc.depth_corr(a, b, 1)
+            assert abs(e - float(np.dot(a.astype(np.float64), b))) < 1e-5
+
+
 # ---------------------------------------------------------------------------
 # Calibration (A11)
 # ---------------------------------------------------------------------------
