@@ -371,6 +371,21 @@ rbg_status rbg_edge_weights(rbg_ctx* ctx, const uint32_t* from, const uint32_t* 
                             uint32_t S, double w_comp, double min_edge_weight,
                             double neg_depth_thr, float* score, double* score64);
 
+/* rbg_edge_weights for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, results and kernel as
+ * rbg_edge_weights; the edge list, depth rows and results are staged in a
+ * fixed 64 MiB device array that each GPU holds until the process ends, so
+ * the call allocates nothing.  Inputs needing more (24 bytes per edge plus
+ * 8 bytes per depth-row entry, each part 256-byte aligned) return
+ * RBG_ERR_UNSUPPORTED.  All calls on one GPU share the array: calls that
+ * overlap on one GPU race and must be serialised by the caller. */
+rbg_status rbg_edge_weights_noalloc(rbg_ctx* ctx, const uint32_t* from, const uint32_t* to,
+                                    const float* scomp, uint64_t ne, const float* unit,
+                                    const float* raw, const double* colnorm, uint32_t n,
+                                    uint32_t S, double w_comp, double min_edge_weight,
+                                    double neg_depth_thr, float* score, double* score64);
+
 /* ---------------------------------------------------------------------------
  * Host-side pieces, callable without a GPU.
  * rbg_sketch_tables: the per-(element, register) tables the sketch kernel

@@ -22,6 +22,7 @@ rbg_status rbg_open(int device, rbg_ctx** out) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return RBG_ERR_HIP;
   if (hipSetDevice(device) != hipSuccess) return RBG_ERR_HIP;
+  if (noalloc_setup(device) != hipSuccess) return RBG_ERR_HIP;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return RBG_ERR_HIP;
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return RBG_ERR_HIP;

@@ -17,4 +17,8 @@ struct DevBuf {
   void release();
 };
 
+// Takes the device address of the no-allocation entry point's scratch array
+// on `device` (current), once per device (rbg_noalloc.hip).
+hipError_t noalloc_setup(int device);
+
 }  // namespace rbg
