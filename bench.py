@@ -5,16 +5,21 @@ A step is one pass of the path over the resident synthetic assembly:
 sketches (canonical 4-mer weighted ProbMinHash), chance baseline b0, the
 all-pairs pass (abundance gate + register-match compare), per-row top-200,
 calibration, mutual filter and edge weights.  Inputs (contig bytes, depth
-features) are resident in HBM before the timed region; the edge list is
-copied back to the host inside it.
+features) are resident in HBM before the timed region and the edge list stays
+in HBM (rbg_asm_build_graph with RBG_EDGES_ON_DEVICE); the same step with the
+edges copied to host memory, and the whole call from host inputs to host
+edges (rbg_build_graph), are reported beside it.
 
 Metric: PMH pairs/sec = N(N-1)/2 / step time, on the 300,000-contig workload
 (1,000 genomes, lengths LogNormal(ln 6000, 1.0), 30 samples, m=500,
 max-edges=200, min-edge-score 70).
 
-Prints one JSON line.  The roofline object describes the dominant kernel,
-timed with HIP events on the library's stream; cpu_baseline times the CPU
-restatement on a bounded row sample of the same workload on the host cores.
+Prints one JSON line.  Every kernel of the step is timed with HIP events on
+the library's stream; `roofline` describes the kernel with the largest time
+and `kernels` every kernel above 10% of the step's kernel time, each against
+its bound (HBM bytes, int8 MFMA ops or VALU lane-ops per launch).
+cpu_baseline times the CPU restatement on a bounded row sample of the same
+workload on the host cores.
 """
 import argparse
 import json
@@ -27,10 +32,52 @@ sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
 PEAK_INT8_TOPS = 5000.0  # dense int8 MFMA, MI355X
+# 32-bit VALU lane-ops: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (no packed math)
+PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_work(name, w):
+    """Algorithmic work of one launch of a kernel of the step, for the
+    workload statistics w: (bound, amount, unit, peak).  Bytes are the HBM
+    bytes the algorithm must move (each input row read once, outputs written
+    once); VALU and MFMA work follow the per-pair operation counts of the
+    pair stage (a register compare of 125 packed words x 6 ops plus the S-term
+    FP32 gate per compared pair; 2*S int8 ops per prefiltered pair)."""
+    n, S, W, K, NC = w["n"], w["S"], w["W"], w["K"], w["NC"]
+    pairs, cand, gated, E = w["pairs"], w["candidates"], w["gated"], w["edges"]
+    if name == "sketch_kernel":
+        return "hbm", w["bytes"] + n * 32 * W, "GB/s", PEAK_HBM_GBS
+    if name == "gate_mfma_kernel":
+        return "mfma", 2.0 * S * pairs, "TOP/s", PEAK_INT8_TOPS
+    if name == "gate_compare_kernel":
+        return "valu", cand * (750.0 + S + 15), "TOP/s", PEAK_VALU_TOPS
+    if name == "dense_counts_kernel":
+        return "valu", pairs * 750.0, "TOP/s", PEAK_VALU_TOPS
+    if name == "scatter_kernel":  # survivor read + its two list entries
+        return "hbm", gated * (16.0 + 16.0), "GB/s", PEAK_HBM_GBS
+    if name == "select_topk_kernel":  # lists read, top-K written
+        return "hbm", 2.0 * gated * 8 + n * K * 6.0 + n * 16.0, "GB/s", PEAK_HBM_GBS
+    if name == "emit_mark_kernel":  # top-K lists read, keep bits written
+        return "hbm", n * K * 6.0 + n * 8.0 * ((K + 63) // 64), "GB/s", PEAK_HBM_GBS
+    if name == "emit_write_kernel":
+        return "hbm", n * K * 6.0 + E * 14.0, "GB/s", PEAK_HBM_GBS
+    if name == "edge_weight_pad_kernel":  # edge arrays + both rows of unit and raw depths
+        return "hbm", E * 16.0 + (E + n) * 2 * 64.0 * NC, "GB/s", PEAK_HBM_GBS
+    return None
+
+
+def kernel_entry(name, ms, share, w, traffic):
+    kw = kernel_work(name, w)
+    if kw is None:
+        return {"kernel": name, "ms": ms, "share": share}
+    bound, amount, unit, peak = kw
+    achieved = amount / (ms / 1e3) / (1e9 if unit == "GB/s" else 1e12)
+    return {"kernel": name, "ms": ms, "share": share, "bound": bound, "achieved": achieved, "peak": peak,
+            "unit": unit, "frac": achieved / peak, "algorithmic": amount, "traffic": traffic}
 
 
 def profiled_traffic(workload, full_size, kernel):
@@ -304,16 +351,27 @@ def plain_form(args, ctx):
     ms_host = 1e3 * (time.perf_counter() - t0)
     log(f"[bench] host-inclusive step: {ms_host:.1f} ms, edges={rh['n_edges']}")
 
-    # Dominant kernel: the pair kernel (int8 MFMA prefilter of the abundance
-    # gate, exact FP32 re-check and register compare of the survivors).
-    k_ms = sum(r["ms"]["k_pair"] for r in runs) / len(runs)
+    # per-kernel device times (HIP events around every launch), averaged over the steps
+    names = [k for k, _ in runs[0]["kernels"]]
+    kms = {k: sum(dict(r["kernels"])[k] for r in runs) / len(runs) for k in names}
+    total_k = sum(kms.values())
     S = w["S"]
-    # algorithmic work of the gate: an S-long dot product (2*S int8 ops) per pair
-    achieved = pairs * 2 * S / (k_ms / 1e3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
-                "frac": achieved / PEAK_INT8_TOPS,
-                "traffic": profiled_traffic(args.workload, n == w["n"], "gate_mfma_kernel"),
-                "kernel": "gate_mfma_kernel", "kernel_ms": k_ms}
+    stats = {"n": n, "S": S, "W": 16, "K": 200, "NC": (S + 15) // 16 if S > 1 else 1, "pairs": pairs,
+             "candidates": int(last["candidates"]), "gated": int(last["gated_pairs"]),
+             "edges": int(last["n_edges"]), "bytes": int(offsets[-1])}
+    full = n == w["n"]
+    kernels = [kernel_entry(k, kms[k], kms[k] / total_k, stats, profiled_traffic(args.workload, full, k))
+               for k in sorted(names, key=lambda k: -kms[k]) if kms[k] >= 0.10 * total_k]
+    dom = kernels[0]
+    roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
+                "frac": dom["frac"], "traffic": dom["traffic"], "kernel": dom["kernel"], "kernel_ms": dom["ms"]}
+    # host inputs to host edges: upload, depth features, int8 rows and
+    # pair-pass order, the step, and the edge list back (rbg_build_graph)
+    t0 = time.perf_counter()
+    rf = ctx.build_graph(seqs, offsets, depth, params)
+    ms_h2h = 1e3 * (time.perf_counter() - t0)
+    assert rf["n_edges"] == last["n_edges"]
+    log(f"[bench] host to host: {ms_h2h:.1f} ms")
 
     out = {
         "metric": "PMH pairs/sec at 300k contigs (bit-exact graph)",
@@ -331,9 +389,16 @@ def plain_form(args, ctx):
         "config": {"workload": f"{args.workload}: {n} contigs, {w['genomes']} genomes, S={S}, m=500, "
                                f"max-edges=200, min-edge-score=70",
                    "contigs": n, "samples": S, "edges": int(last["n_edges"]),
-                   "gated_pairs": int(last["gated_pairs"]), "stage_ms": last["ms"],
-                   "ms_per_step_with_edges_to_host": ms_host},
+                   "gated_pairs": int(last["gated_pairs"]), "candidates": int(last["candidates"]),
+                   "stage_ms": last["ms"], "kernel_ms": kms,
+                   "ms_per_step_with_edges_to_host": ms_host,
+                   "ms_host_to_host": ms_h2h,
+                   "note": "value: inputs resident in HBM, edges left in HBM; ms_per_step_with_edges_to_host "
+                           "adds the edge list's copy to host memory; ms_host_to_host is rbg_build_graph from "
+                           "host contigs and depths to host edges (upload, depth features, int8 rows, "
+                           "pair-pass order, the step, edges back)"},
         "roofline": roofline,
+        "kernels": kernels,
     }
     if not args.no_cpu_baseline:
         try:

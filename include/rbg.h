@@ -151,6 +151,13 @@ uint32_t rbg_env_semantics(char* names, size_t names_cap);
  * src/rabbitbin.cpp:446-449 reads it) and semantics = rbg_env_semantics(). */
 void rbg_default_params(rbg_graph_params* p);
 
+/* Device time of one kernel launch of a build (name: static storage). */
+typedef struct rbg_kernel_time {
+  const char* name;
+  double ms;
+} rbg_kernel_time;
+#define RBG_MAX_KERNEL_TIMES 16
+
 typedef struct rbg_graph_result {
   uint64_t n_edges;
   uint32_t* from;          /* edges sorted by (from, to), from < to */
@@ -176,6 +183,12 @@ typedef struct rbg_graph_result {
   uint64_t n_near;
   uint64_t* near_edge;
   double* near_score64;
+  /* Per-kernel device times of an eager rbg_asm_build_graph / rbg_build_graph
+   * (HIP events around each launch on the context's stream), in launch
+   * order; n_kernels = 0 for a replayed graph and for the other entry points. */
+  uint32_t n_kernels;
+  rbg_kernel_time kernels[RBG_MAX_KERNEL_TIMES];
+  uint64_t candidates;     /* int8 prefilter candidates of the build (fused regime) */
 } rbg_graph_result;
 
 /* Optional per-row diagnostics for parity tests (any pointer may be NULL):

@@ -101,8 +101,14 @@ extern "C" {
 
 // pmh_match_count16 (src/rabbitbin.cpp:1261-1293).
 uint32_t orc_match_count16(const uint16_t* a, const uint16_t* b, uint32_t m) {
-  uint32_t c = 0;
-  for (uint32_t k = 0; k < m; ++k) c += (a[k] != 0 && a[k] == b[k]) ? 1u : 0u;
+  // the same count, written without branches so that the compiler vectorises it
+  uint16_t c16[16] = {0};
+  uint32_t c = 0, k = 0;
+  for (; k + 16 <= m; k += 16)
+    for (int l = 0; l < 16; ++l)
+      c16[l] = (uint16_t)(c16[l] + ((a[k + l] == b[k + l]) & (a[k + l] != 0)));
+  for (; k < m; ++k) c += (uint32_t)((a[k] == b[k]) & (a[k] != 0));
+  for (int l = 0; l < 16; ++l) c += c16[l];
   return c;
 }
 

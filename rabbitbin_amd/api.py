@@ -75,6 +75,10 @@ class BatchStats(C.Structure):
                 ("replayed", C.c_int)]
 
 
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("ms", C.c_double)]
+
+
 class GraphResult(C.Structure):
     _fields_ = [
         ("n_edges", C.c_uint64),
@@ -97,6 +101,9 @@ class GraphResult(C.Structure):
         ("n_near", C.c_uint64),
         ("near_edge", u64p),
         ("near_score64", f64p),
+        ("n_kernels", C.c_uint32),
+        ("kernels", KernelTime * 16),
+        ("candidates", C.c_uint64),
     ]
 
 
@@ -284,6 +291,7 @@ class Context:
             "score": (np.ctypeslib.as_array(res.score, (ne,)).copy() if (ne and res.score) else None),
             "sim_cutoff": res.sim_cutoff, "b0": res.b0, "regime": res.regime,
             "gated_pairs": res.gated_pairs,
+            "candidates": res.candidates,
             "near_edge": (np.ctypeslib.as_array(res.near_edge, (res.n_near,)).copy() if res.n_near
                           else np.zeros(0, np.uint64)),
             "near_score64": (np.ctypeslib.as_array(res.near_score64, (res.n_near,)).copy() if res.n_near
@@ -291,6 +299,7 @@ class Context:
             "ms": {"sketch": res.ms_sketch, "pairs": res.ms_pairs, "select": res.ms_select,
                    "edges": res.ms_edges, "total": res.ms_total, "k_pair": res.ms_k_pair,
                    "k_compare": res.ms_k_compare, "k_select": res.ms_k_select},
+            "kernels": [(res.kernels[k].name.decode(), res.kernels[k].ms) for k in range(res.n_kernels)],
         }
         if diag_arrays is not None:
             out.update(diag_arrays)

@@ -26,6 +26,8 @@ rbg_status rbg_open(int device, rbg_ctx** out) {
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return RBG_ERR_HIP;
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return RBG_ERR_HIP;
+  for (auto& e : ctx->kev)
+    if (hipEventCreate(&e) != hipSuccess) return RBG_ERR_HIP;
   *out = ctx.release();
   return RBG_OK;
 }
@@ -36,6 +38,8 @@ void rbg_close(rbg_ctx* ctx) {
   ctx->tables.clear();
   release_staging(ctx);
   for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->kev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -242,6 +246,9 @@ rbg_status rbg_asm_build_graph(rbg_ctx* ctx, rbg_asm* a, const rbg_graph_params*
     out->ms_k_pair = elapsed_ms(ev[2], ev[6]);
     out->ms_k_compare = elapsed_ms(ev[7], ev[8]);
     out->ms_k_select = elapsed_ms(ev[9], ev[10]);
+    out->n_kernels = ctx->nk;
+    for (uint32_t k = 0; k < ctx->nk; ++k)
+      out->kernels[k] = rbg_kernel_time{ctx->kname[k], elapsed_ms(ctx->kev[2 * k], ctx->kev[2 * k + 1])};
   }
   return RBG_OK;
 }

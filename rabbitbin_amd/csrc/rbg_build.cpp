@@ -874,6 +874,22 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
     if (timed) RBG_TRY_HIP(ctx, hipEventRecord(ev[k], st));
     return RBG_OK;
   };
+  // events around single launches (eager builds): kbeg(name) ... kend()
+  ctx->nk = 0;
+  auto kbeg = [&](const char* name) -> rbg_status {
+    if (timed && ctx->nk < RBG_MAX_KERNEL_TIMES) {
+      ctx->kname[ctx->nk] = name;
+      RBG_TRY_HIP(ctx, hipEventRecord(ctx->kev[2 * ctx->nk], st));
+    }
+    return RBG_OK;
+  };
+  auto kend = [&]() -> rbg_status {
+    if (timed && ctx->nk < RBG_MAX_KERNEL_TIMES) {
+      RBG_TRY_HIP(ctx, hipEventRecord(ctx->kev[2 * ctx->nk + 1], st));
+      ++ctx->nk;
+    }
+    return RBG_OK;
+  };
   RBG_TRY(mark(0));
   RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->counters.p, 8, 0u, st));
   const uint32_t mpad = 32 * W;
@@ -884,20 +900,25 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
       if (h.src && h.n_old)
         RBG_TRY_HIP(ctx, hipMemcpyAsync((uint32_t*)b->planes.p + (size_t)h.row0 * 8 * W, h.src->planes.p,
                                         (size_t)h.n_old * 8 * W * 4, hipMemcpyDeviceToDevice, st));
+  RBG_TRY(kbeg("sketch_kernel"));
   RBG_TRY_HIP(ctx, launch_sketch((const uint8_t*)b->seqs.p, (const uint64_t*)b->offs.p, b->n_sketch, m, W,
                                  (const double*)t->sT.p, (const uint8_t*)t->sX.p,
                                  (const uint8_t*)t->cls.p, (const uint8_t*)t->codes.p,
                                  want_wx ? (uint8_t*)b->wx.p : nullptr, mpad, (uint32_t*)b->planes.p,
                                  b->has_src ? (const uint32_t*)b->sk_rows.p : nullptr, st));
+  RBG_TRY(kend());
   if (b->cls_rows)
     RBG_TRY_HIP(ctx, launch_planes_from_classes((const uint8_t*)b->cls_in.p, b->cls_rows, m, b->cls_mpad, W, 0,
                                                 (uint32_t*)b->planes.p, st));
   RBG_TRY(mark(1));
   // b0 and the sv tables (the pair draws depend on n only)
-  if (b->np_total)
+  if (b->np_total) {
+    RBG_TRY(kbeg("list_counts_kernel"));
     RBG_TRY_HIP(ctx, launch_list_counts((const uint32_t*)b->planes.p, W, (const uint32_t*)b->pi.p,
                                         (const uint32_t*)b->pj.p, b->np_total,
                                         (uint32_t*)b->pcnt.p, st));
+    RBG_TRY(kend());
+  }
   RBG_TRY_HIP(ctx, launch_baseline_tables((const uint32_t*)b->pcnt.p, desc, A, m, bs,
                                           (double*)b->svd.p, (float*)b->svf.p,
                                           (uint16_t*)b->level.p, st));
@@ -910,15 +931,20 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                           h.src->mat.p, (size_t)h.n_old * 2, (size_t)h.n_old * 2,
                                           h.n_old, hipMemcpyDeviceToDevice, st));
     }
-  if (b->n_dense_items)
+  if (b->n_dense_items) {
+    RBG_TRY(kbeg("dense_counts_kernel"));
     RBG_TRY_HIP(ctx, launch_dense_counts((const uint32_t*)b->planes.p, W, desc,
                                          (const uint32_t*)b->dense_items.p, b->n_dense_items,
                                          (uint16_t*)b->mat.p, st));
+    RBG_TRY(kend());
+  }
   unsigned long long* ncand = (unsigned long long*)b->counters.p;
   if (b->has_gate_items) {
+    RBG_TRY(kbeg("gate_mfma_kernel"));
     RBG_TRY_HIP(ctx, launch_gate_mfma((const int8_t*)b->q8.p, b->kpad, desc,
                                       (const uint32_t*)b->perm.p, (const uint32_t*)b->gate_items.p,
                                       b->n_gate_items, (uint64_t*)b->cand.p, ncand, b->cand_cap, st));
+    RBG_TRY(kend());
   }
   RBG_TRY(mark(6));
   RBG_TRY(mark(7));
@@ -929,11 +955,13 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
     const MergeDesc* md = (const MergeDesc*)b->merge.p;
     RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->deg.p, N, 0u, st));
     RBG_TRY_HIP(ctx, launch_merge_deg(md, b->n_incr, b->merge_max_old, (uint32_t*)b->deg.p, st));
+    RBG_TRY(kbeg("gate_compare_kernel"));
     RBG_TRY_HIP(ctx, launch_gate_compare((uint64_t*)b->cand.p, b->cand_cap, ncand,
                                          (const float*)b->unit_pad.p, b->NC,
                                          (const uint8_t*)b->anynz.p, desc, asm_of,
                                          (const uint32_t*)b->planes.p, W, (uint32_t*)b->deg.p,
                                          (uint32_t*)b->slot.p, st));
+    RBG_TRY(kend());
   }
   RBG_TRY(mark(8));
   if (b->has_gate_items) {
@@ -943,40 +971,51 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                                b->scan_tmp.p, tmp, st));
     RBG_TRY_HIP(ctx, launch_merge_lists(md, b->n_incr, b->merge_max_old, (const uint64_t*)b->off.p,
                                         (uint64_t*)b->list.p, st));
+    RBG_TRY(kbeg("scatter_kernel"));
     RBG_TRY_HIP(ctx, launch_scatter((const uint64_t*)b->cand.p, b->cand_cap, ncand,
                                     (const uint32_t*)b->slot.p, (const uint64_t*)b->off.p,
                                     (uint64_t*)b->list.p, st));
+    RBG_TRY(kend());
   }
   RBG_TRY(mark(3));
   RBG_TRY(mark(9));
+  RBG_TRY(kbeg("select_topk_kernel"));
   RBG_TRY_HIP(ctx, launch_select((const uint64_t*)b->list.p, (const uint64_t*)b->off.p,
                                  (const uint16_t*)b->mat.p, (const uint8_t*)b->anynz.p, desc, asm_of,
                                  N, (const uint16_t*)b->level.p, m, K, (uint32_t*)b->topk_id.p,
                                  (uint16_t*)b->topk_cnt.p, (uint32_t*)b->topk_len.p,
                                  (uint32_t*)b->rowmax.p, (uint32_t*)b->fence.p, FW, st));
+  RBG_TRY(kend());
   RBG_TRY(mark(10));
   // calibration (the glibc rand() draws are prepared per (n, seed))
+  RBG_TRY(kbeg("calibrate_kernel"));
   RBG_TRY_HIP(ctx, launch_calibrate((const uint32_t*)b->rowmax.p, desc, A, m,
                                     (const double*)b->svd.p, (const float*)b->svf.p,
                                     (const int32_t*)b->cal_rnd.p, (const uint32_t*)b->cal_members.p,
                                     (const uint32_t*)b->cal_roff.p, P->coverage, bs, st));
+  RBG_TRY(kend());
+  RBG_TRY(kbeg("emit_mark_kernel"));
   RBG_TRY_HIP(ctx, launch_emit_mark((const uint32_t*)b->topk_id.p, (const uint16_t*)b->topk_cnt.p,
                                     (const uint32_t*)b->topk_len.p, (const uint32_t*)b->fence.p, FW,
                                     N, K, bs, desc, asm_of, (const uint32_t*)b->perm.p,
                                     (uint64_t*)b->keep.p, (uint32_t*)b->row_edges.p, st));
+  RBG_TRY(kend());
   {
     size_t tmp = b->scan_tmp.bytes;
     RBG_TRY_HIP(ctx, exclusive_scan_u32_to_u64((const uint32_t*)b->row_edges.p,
                                                (uint64_t*)b->row_off.p, N, b->scan_tmp.p, tmp, st));
   }
   const uint64_t* ne = (const uint64_t*)b->row_off.p + N;
+  RBG_TRY(kbeg("emit_write_kernel"));
   RBG_TRY_HIP(ctx, launch_emit_write((const uint32_t*)b->topk_id.p, (const uint16_t*)b->topk_cnt.p,
                                      (const uint32_t*)b->topk_len.p, N, K, (const uint64_t*)b->keep.p,
                                      (const uint64_t*)b->row_off.p, asm_of, m,
                                      (const float*)b->svf.p, (uint32_t*)b->efrom.p,
                                      (uint32_t*)b->eto.p, (uint16_t*)b->ecnt.p,
                                      (float*)b->escomp.p, st));
+  RBG_TRY(kend());
   RBG_TRY(mark(4));
+  RBG_TRY(kbeg("edge_weight_pad_kernel"));
   RBG_TRY_HIP(ctx, launch_edge_weights_batch((const uint32_t*)b->efrom.p, (const uint32_t*)b->eto.p,
                                              (const float*)b->escomp.p, b->edge_cap, ne,
                                              (const float*)b->unit_pad.p, (const float*)b->raw_pad.p,
@@ -984,6 +1023,7 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                              P->min_edge_weight, P->neg_depth_thr, P->near_tol,
                                              (float*)b->escore.p, ncand + 1, b->near_cap,
                                              (uint64_t*)b->near_e.p, (double*)b->near_w.p, st));
+  RBG_TRY(kend());
   RBG_TRY_HIP(ctx, launch_gather_bounds((const uint64_t*)b->row_off.p,
                                         b->has_gate_items ? (const uint64_t*)b->off.p : nullptr, desc,
                                         A, (uint64_t*)b->bounds.p, st));
@@ -1066,6 +1106,7 @@ rbg_status assembly_result(rbg_ctx* ctx, rbg_asm* b, uint32_t a, const rbg_graph
   out->sim_cutoff = s.sim_cutoff;
   out->regime = (int)b->desc_h[a].regime;
   out->gated_pairs = (b->bounds_h[4 * a + 3] - b->bounds_h[4 * a + 2]) / 2;
+  if (b->asms.size() == 1) out->candidates = b->ncand_last;
   if (b->desc_h[a].dense && out->regime == 2)  // no gate: every pair passing is_nz was evaluated
     out->gated_pairs = (uint64_t)h.n * (h.n - 1) / 2 - (uint64_t)h.n_zero * (h.n_zero - (h.n_zero > 0)) / 2;
   const uint32_t K = P->max_edges, m = P->m, n = h.n, r0 = h.row0;

@@ -30,6 +30,10 @@ struct rbg_ctx {
   };
   std::vector<std::unique_ptr<Tables>> tables;
   hipEvent_t ev[12] = {};
+  // per-kernel timing of an eager build: events 2k, 2k+1 around launch k
+  hipEvent_t kev[2 * RBG_MAX_KERNEL_TIMES] = {};
+  const char* kname[RBG_MAX_KERNEL_TIMES] = {};
+  uint32_t nk = 0;
   // Pinned host staging for large transfers: two halves, so the pageable side
   // of one chunk is copied while the other chunk's DMA runs.
   struct Staging {
