@@ -789,7 +789,8 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   RBG_TRY_HIP(ctx, b->svd.ensure((size_t)A * (m + 1) * 8));
   RBG_TRY_HIP(ctx, b->svf.ensure((size_t)A * (m + 1) * 4));
   RBG_TRY_HIP(ctx, b->level.ensure((size_t)A * (m + 1) * 2));
-  RBG_TRY_HIP(ctx, b->mat.ensure(std::max<uint64_t>(mat_total, 1) * 2));
+  // +16 bytes: the selection reads dense rows in whole 16-byte chunks
+  RBG_TRY_HIP(ctx, b->mat.ensure(std::max<uint64_t>(mat_total, 1) * 2 + 16));
   RBG_TRY_HIP(ctx, b->cand.ensure(std::max<uint64_t>(cap, 1) * 8));
   RBG_TRY_HIP(ctx, b->counters.ensure(32));  // candidates, near-cut edges (u64 each)
   RBG_TRY_HIP(ctx, b->deg.ensure((size_t)N * 4));

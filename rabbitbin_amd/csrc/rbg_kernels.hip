@@ -66,10 +66,18 @@ __device__ __forceinline__ void column_fix(uint32_t* b) {
   b[0] &= ~all;
 }
 
+// (a ^ b) | c in one v_bitop3_b32 (truth table 0xDE over (S0, S1, S2) =
+// (a, c, b): 0xDE = (0xF0 ^ 0xAA) | 0xCC).
+__device__ __forceinline__ uint32_t xor_or(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, c, b, 0xDE);
+}
+
+// Registers of a 32-register group whose classes differ: OR over the eight
+// planes of a ^ b, as one chain (8 instructions).
 __device__ __forceinline__ uint32_t word_mismatch(const uint32_t* a, const uint32_t* b) {
   uint32_t mm = a[0] ^ b[0];
 #pragma unroll
-  for (int p = 1; p < 8; ++p) mm |= a[p] ^ b[p];
+  for (int p = 1; p < 8; ++p) mm = xor_or(a[p], b[p], mm);
   return mm;
 }
 
@@ -477,31 +485,37 @@ __global__ __launch_bounds__(256) void win16_counts_kernel(const uint16_t* __res
 }
 
 // ---------------------------------------------------------------------------
-// Small regime (n <= 25000): every pair's count into an n x n u16 matrix,
-// computed once per unordered pair.  Thread = column j with its bit planes in
-// VGPRs; rows r < j are uniform across the wave and come in through scalar
-// loads.  The count is symmetric (a register matches only where both sketches
-// set it), so (r, j) is written at [r][j] (a coalesced row segment) and at
-// [j][r] (the thread's own row segment, filled over the item's rows).  The
-// diagonal is neither written nor read.
+// Dense counts (the small regime, n <= 25000, and the fused regime without the
+// abundance gate): every pair's count into an n x n u16 matrix, computed once
+// per unordered pair.  Thread = column j with its bit planes in VGPRs; rows
+// r < j are uniform across the wave and come in through scalar loads.  The
+// count is symmetric (a register matches only where both sketches set it), so
+// (r, j) is written at [r][j] (lanes write a coalesced row segment) and at
+// [j][r]: the counts of 64 rows are staged in LDS, and each wave then writes
+// 64 columns' row segments [j][r0 .. r0+64), one 128-byte segment per store
+// instruction.  The diagonal is neither written nor read.
 // ---------------------------------------------------------------------------
 // Work item (assembly a, column block cb, row block rb, rows): thread = local
 // column 256*cb + t of assembly a against its rows [rb*rows, +rows), pairs
 // r < j with j >= jmin only (jmin > 0: a served session's earlier pairs are in
 // the matrix already); items without such a pair are not generated.
+constexpr uint32_t kDenseSub = 64;  // rows staged per LDS tile
 template <int W>
 __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __restrict__ planes,
                                                            const AsmDesc* __restrict__ desc,
                                                            const uint4* __restrict__ items,
                                                            uint16_t* __restrict__ mat) {
+  __shared__ uint16_t tile[kDenseSub][256 + 2];  // odd word stride: column reads spread over banks
   const uint4 it = items[blockIdx.x];
   const uint32_t n = desc[it.x].n, jmin = desc[it.x].jmin;
   const uint32_t* __restrict__ rows = planes + (size_t)desc[it.x].row0 * 8 * W;
   uint16_t* __restrict__ out = mat + desc[it.x].mat0;
-  const uint32_t j = it.y * 256 + threadIdx.x;
-  const uint32_t jlast = min(n, it.y * 256 + 256) - 1;  // the block's last column
+  const uint32_t tid = threadIdx.x, ln = tid & 63u, wv = tid >> 6;
+  const uint32_t j0 = it.y * 256;
+  const uint32_t j = j0 + tid;
+  const uint32_t jlast = min(n, j0 + 256) - 1;  // the block's last column
   const uint32_t r0 = it.z * it.w;
-  const uint32_t r1 = min(min(n, r0 + it.w), jlast);    // rows with some column j > r
+  const uint32_t r1 = min(min(n, r0 + it.w), jlast);  // rows with some column j > r
   uint32_t b[8 * W];
   const uint32_t jj = j < n ? j : n - 1;
 #pragma unroll
@@ -509,20 +523,30 @@ __global__ __launch_bounds__(256) void dense_counts_kernel(const uint32_t* __res
     load_word(rows + (size_t)jj * 8 * W, w, W, b + 8 * w);
     column_fix(b + 8 * w);
   }
-  for (uint32_t r = r0; r < r1; ++r) {
-    const uint32_t* __restrict__ ra = rows + (size_t)r * 8 * W;
-    uint32_t mism = 0;
+  const bool jok = j < n && j >= jmin;
+  for (uint32_t rs = r0; rs < r1; rs += kDenseSub) {
+    const uint32_t re = min(r1, rs + kDenseSub);
+    for (uint32_t r = rs; r < re; ++r) {
+      const uint32_t* __restrict__ ra = rows + (size_t)r * 8 * W;
+      uint32_t mism = 0;
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      uint32_t a[8];
-      load_word(ra, w, W, a);
-      mism += __popc(word_mismatch(a, b + 8 * w));
-    }
-    if (j < n && j > r && j >= jmin) {
+      for (int w = 0; w < W; ++w) {
+        uint32_t a[8];
+        load_word(ra, w, W, a);
+        mism += __popc(word_mismatch(a, b + 8 * w));
+      }
       const uint16_t c = (uint16_t)(32u * W - mism);
-      out[(size_t)r * n + j] = c;
-      out[(size_t)j * n + r] = c;
+      tile[r - rs][tid] = c;
+      if (jok && j > r) out[(size_t)r * n + j] = c;
     }
+    __syncthreads();
+    // [j][r] for this wave's 64 columns: lane = row r = rs + ln
+    const uint32_t r = rs + ln;
+    for (uint32_t q = 0; q < 64; ++q) {
+      const uint32_t col = 64 * wv + q, jc = j0 + col;
+      if (jc < n && jc >= jmin && r < re && jc > r) out[(size_t)jc * n + r] = tile[ln][col];
+    }
+    __syncthreads();
   }
 }
 
@@ -1223,6 +1247,164 @@ __device__ void select_staged_row(const RowSource& src, uint32_t i, uint32_t len
   }
 }
 
+// Block-wide exclusive prefix sum of one value per thread (256 threads);
+// `total` receives the block's sum.  wtot: 4 words of LDS.
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* wtot, uint32_t& total) {
+  const uint32_t ln = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(incl, d);
+    if (ln >= d) incl += t;
+  }
+  if (ln == 63) wtot[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t x = wtot[k];
+    before += k < w ? x : 0u;
+    tot += x;
+  }
+  __syncthreads();  // wtot is reused by the next call
+  total = tot;
+  return before + incl - v;
+}
+
+// Dense rows (the small regime, and the fused regime without the abundance
+// gate): the row's counts against every row of its assembly, in id order;
+// the diagonal and partners failing is_nz are skipped.  Pass 1 reads the row
+// eight counts per 128-bit load and builds the count histogram (counts at
+// level 0, the similarity clamped to 0, are counted in registers) and the
+// row maximum; the threshold level L* and the number `need` of entries to
+// take at L* follow from the histogram folded into levels.  Pass 2 keeps, in
+// id order, every entry above L* and the first `need` entries at L*: an
+// ordered compaction with block-wide prefix counts per 2048-entry step, so
+// the list comes out sorted by id with no sort.  The row's counts are read
+// through 16-byte aligned chunks, so the matrix buffer keeps 16 readable
+// bytes past its last row.  `lds` holds 4096 words.
+__device__ void select_dense_row(const RowSource& src, uint32_t i, const uint16_t* __restrict__ level_g,
+                                 uint32_t m, uint32_t K, uint32_t* lds, uint32_t* wtot, uint32_t* tmp_bin,
+                                 uint32_t* tmp_below, uint32_t* rmax, uint32_t* ztot,
+                                 uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
+                                 uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out,
+                                 uint32_t* __restrict__ fence, uint32_t FW) {
+  const uint32_t tid = threadIdx.x, wv = tid >> 6;
+  uint32_t* lvl = lds;                       // level of each count, [0, m]
+  uint32_t* whist = lds + 1024 + 520 * wv;   // this wave's count histogram
+  uint32_t* lhist = lds + 3104;              // level histogram, [0, m]
+  const uint32_t nb = m + 1;
+  for (uint32_t c = tid; c < nb; c += 256) {
+    lvl[c] = level_g[c];
+    lhist[c] = 0;
+  }
+  for (uint32_t c = tid; c < 4 * 520; c += 256) lds[1024 + c] = 0;
+  if (tid == 0) {
+    *rmax = 0;
+    *ztot = 0;
+  }
+  __syncthreads();
+  const uint32_t n = src.n, local = src.local, row0 = src.row0;
+  const uint16_t* __restrict__ row = src.mrow;
+  const uint32_t head = (uint32_t)(((uintptr_t)row & 15u) >> 1);  // counts before the row in its first chunk
+  const uint4* __restrict__ chunks = reinterpret_cast<const uint4*>(row - head);
+  const uint32_t nch = (head + n + 7u) >> 3;
+  const bool nz_i = !src.anynz || src.anynz[i];  // then every partner passes is_nz
+  // one chunk's entries: valid (a partner of the row) and is_nz
+  auto entry = [&](uint32_t ch, uint32_t e, uint32_t word, uint32_t& c, bool& ok, bool& fm) {
+    const int64_t t = (int64_t)ch * 8 + e - head;
+    c = (word >> (16 * (e & 1u))) & 0xFFFFu;
+    const bool in = t >= 0 && t < (int64_t)n && (uint32_t)t != local;
+    const bool nz = nz_i || (in && src.anynz[row0 + (uint32_t)t]);
+    ok = in && nz;
+    fm = in && (src.small || nz);
+  };
+  uint32_t my_max = 0, zeros = 0;
+  for (uint32_t ch = tid; ch < nch; ch += 256) {
+    const uint4 v = chunks[ch];
+    const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) {
+      uint32_t c;
+      bool ok, fm;
+      entry(ch, e, wd[e >> 1], c, ok, fm);
+      if (fm && c > my_max) my_max = c;
+      if (ok) {
+        if (lvl[c] == 0)
+          ++zeros;
+        else
+          atomicAdd(&whist[c], 1u);
+      }
+    }
+  }
+  if (my_max) atomicMax(rmax, my_max);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) zeros += __shfl_xor(zeros, o);
+  if ((tid & 63u) == 0 && zeros) atomicAdd(ztot, zeros);
+  __syncthreads();
+  for (uint32_t c = tid; c < nb; c += 256) {
+    const uint32_t k = lds[1024 + c] + lds[1544 + c] + lds[2064 + c] + lds[2584 + c];
+    if (k) atomicAdd(&lhist[lvl[c]], k);
+  }
+  __syncthreads();
+  if (tid == 0) lhist[0] += *ztot;
+  __syncthreads();
+  uint32_t part = 0;
+  for (uint32_t b = tid; b < nb; b += 256) part += lhist[b];
+  uint32_t total;
+  (void)block_scan_excl(part, wtot, total);
+  const bool take_all = total <= K;
+  uint32_t Ls = 0, need = 0xFFFFFFFFu;
+  if (!take_all) {
+    find_reaching(lhist, nb, true, K, wtot, tmp_bin, tmp_below);
+    Ls = *tmp_bin;
+    need = K - *tmp_below;
+  }
+  const uint32_t ns = take_all ? total : K;
+  // pass 2: ordered compaction
+  uint32_t out = 0, at_ls = 0;  // kept so far, entries at L* seen so far (block-uniform)
+  for (uint32_t step = 0; step * 256 < nch && out < ns; ++step) {
+    const uint32_t ch = step * 256 + tid;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (ch < nch) v = chunks[ch];
+    const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+    uint32_t gt = 0, eq = 0;  // bit e: entry e above L* / at L*
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) {
+      uint32_t c;
+      bool ok, fm;
+      entry(ch, e, wd[e >> 1], c, ok, fm);
+      ok = ok && ch < nch;
+      const uint32_t lv = ok ? lvl[c] : 0u;  // counts outside the row are not read as table indices
+      if (ok && (take_all || lv > Ls)) gt |= 1u << e;
+      if (ok && !take_all && lv == Ls) eq |= 1u << e;
+    }
+    uint32_t eq_total;
+    const uint32_t eq_before = at_ls + block_scan_excl(__popc(eq), wtot, eq_total);
+    // entries at L* whose rank among them is below need are kept
+    uint32_t keep = gt;
+    for (uint32_t b = eq, r = eq_before; b; b &= b - 1u, ++r)
+      if (r < need) keep |= 1u << (__ffs(b) - 1u);
+    uint32_t keep_total;
+    uint32_t pos = out + block_scan_excl(__popc(keep), wtot, keep_total);
+    for (uint32_t b = keep; b; b &= b - 1u, ++pos) {
+      const uint32_t e = __ffs(b) - 1u;
+      const uint32_t t = ch * 8 + e - head;
+      const uint32_t c = (wd[e >> 1] >> (16 * (e & 1u))) & 0xFFFFu;
+      topk_id[(size_t)i * K + pos] = row0 + t;
+      topk_cnt[(size_t)i * K + pos] = (uint16_t)c;
+      if ((pos & 15u) == 0) fence[(size_t)i * FW + pos / 16] = row0 + t;
+    }
+    out += keep_total;
+    at_ls += eq_total;
+  }
+  for (uint32_t q = (ns + 15) / 16 + tid; q < FW; q += 256) fence[(size_t)i * FW + q] = 0xFFFFFFFFu;
+  if (tid == 0) {
+    topk_len[i] = ns;
+    rowmax_out[i] = *rmax;
+  }
+}
+
 __global__ __launch_bounds__(256) void select_topk_kernel(
     const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
     const uint16_t* __restrict__ mat, const uint8_t* __restrict__ anynz,
@@ -1245,6 +1427,11 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
   const uint32_t nlevels = m + 1;
   const uint32_t len = row_len(src, i);
 
+  if (src.dense && nlevels <= 520) {
+    select_dense_row(src, i, level, m, K, hist, wave_tot, &tmp_bin, &tmp_below, &rmax, &tot, topk_id,
+                     topk_cnt, topk_len, rowmax_out, fence, FW);
+    return;
+  }
   if (!src.dense && len <= kSelectStage && nlevels <= 1024) {
     select_staged_row(src, i, len, level, nlevels, K, hist, sel_id, sel_cnt, wave_tot, &tmp_bin,
                       &tmp_below, &rmax, &nsel, &tot, topk_id, topk_cnt, topk_len, rowmax_out,
