@@ -417,6 +417,42 @@ rbg_status rbg_sketch_tables(uint32_t m, uint64_t seed, double* T, uint64_t* W, 
 rbg_status rbg_calibrate(const uint32_t* rowmax_cnt, uint32_t n, uint32_t m, double b0,
                          double coverage, uint32_t rand_seed, int regime, double* sim_cutoff);
 
+/* ---------------------------------------------------------------------------
+ * RabbitBin's graph cache file, RBCACHE1 version 3 (src/impl/rb_cache.cpp:
+ * 94-175 writes it, :178-322 reads it; `rabbitbin bin --save-cache` /
+ * `--load-cache`).  rbg_cache_write writes a graph built here (with the
+ * caller's contig names, lengths and depth state) in the reference's layout;
+ * rbg_cache_read reads versions 1-3 (size_t endpoints of versions 1-2 are
+ * range-checked into 32-bit ids, as the reference does) into arrays it
+ * allocates (release with rbg_cache_free).  Host only, no GPU.  Names are
+ * stored back to back: name i is names[name_off[i] .. name_off[i+1]).  An
+ * array may be NULL (written as empty).  Returns RBG_ERR_UNSUPPORTED for a
+ * file that is not an RBCACHE1 cache of a supported version and ABI, and
+ * RBG_ERR_ARG for a truncated or inconsistent one.
+ * ------------------------------------------------------------------------- */
+typedef struct rbg_cache {
+  uint32_t version;                 /* read: the file's version; written: 3 */
+  uint64_t nobs, nobs1, samples;    /* large and small contigs, depth samples */
+  uint64_t seed, total_size, total_size1;
+  double sim_cutoff, b0;            /* simCutoff (per-mille), g_pmh_baseline */
+  uint64_t min_contig, min_small_contig;
+  int has_depth;
+  char* names;        uint64_t* name_off;        /* contig_names (nobs) */
+  char* small_names;  uint64_t* small_name_off;  /* small_contig_names (nobs1) */
+  uint64_t* seq_lens;                            /* nobs */
+  uint64_t* small_seq_lens;                      /* nobs1 */
+  uint64_t depth_rows, depth_cols;   float* depth;              /* depth_matrix (ranked) */
+  uint64_t small_depth_rows, small_depth_cols; float* small_depth;
+  uint64_t n_large_means;  float* large_means;   /* g_large_means */
+  uint64_t n_depth_raw;    float* depth_raw;     /* g_depth_raw */
+  uint64_t n_colnorm;      double* colnorm;      /* g_depth_colnorm */
+  uint64_t n_edges;  uint32_t* from;  uint32_t* to;  float* scomp;  /* the graph */
+  int has_snv;  uint64_t n_snv;  float* snv_pi;  uint32_t* snv_nsnv;  uint32_t* snv_cov;
+} rbg_cache;
+rbg_status rbg_cache_write(const char* path, const rbg_cache* c);
+rbg_status rbg_cache_read(const char* path, rbg_cache* out);
+void rbg_cache_free(rbg_cache* c);
+
 /* Chance baseline b0 over 16-bit winners (src/rabbitbin.cpp:1340-1369). */
 rbg_status rbg_pmh_baseline(rbg_ctx* ctx, const uint16_t* win16, uint32_t n, uint32_t m,
                             double* b0);

@@ -137,6 +137,23 @@ class GraphDiag(C.Structure):
     ]
 
 
+class CacheFile(C.Structure):
+    _fields_ = [("version", C.c_uint32), ("nobs", C.c_uint64), ("nobs1", C.c_uint64), ("samples", C.c_uint64),
+                ("seed", C.c_uint64), ("total_size", C.c_uint64), ("total_size1", C.c_uint64),
+                ("sim_cutoff", C.c_double), ("b0", C.c_double), ("min_contig", C.c_uint64),
+                ("min_small_contig", C.c_uint64), ("has_depth", C.c_int),
+                ("names", C.c_void_p), ("name_off", u64p), ("small_names", C.c_void_p), ("small_name_off", u64p),
+                ("seq_lens", u64p), ("small_seq_lens", u64p),
+                ("depth_rows", C.c_uint64), ("depth_cols", C.c_uint64), ("depth", f32p),
+                ("small_depth_rows", C.c_uint64), ("small_depth_cols", C.c_uint64), ("small_depth", f32p),
+                ("n_large_means", C.c_uint64), ("large_means", f32p),
+                ("n_depth_raw", C.c_uint64), ("depth_raw", f32p),
+                ("n_colnorm", C.c_uint64), ("colnorm", f64p),
+                ("n_edges", C.c_uint64), ("from_", u32p), ("to", u32p), ("scomp", f32p),
+                ("has_snv", C.c_int), ("n_snv", C.c_uint64), ("snv_pi", f32p), ("snv_nsnv", u32p),
+                ("snv_cov", u32p)]
+
+
 _lib = None
 
 
@@ -172,6 +189,9 @@ def lib():
     L.rbg_edge_weights.argtypes = [vp, u32p, u32p, f32p, C.c_uint64, f32p, f32p, f64p, C.c_uint32,
                                    C.c_uint32, C.c_double, C.c_double, C.c_double, f32p, f64p]
     L.rbg_pmh_baseline.argtypes = [vp, u16p, C.c_uint32, C.c_uint32, f64p]
+    L.rbg_cache_write.argtypes = [C.c_char_p, C.POINTER(CacheFile)]
+    L.rbg_cache_read.argtypes = [C.c_char_p, C.POINTER(CacheFile)]
+    L.rbg_cache_free.argtypes = [C.POINTER(CacheFile)]
     L.rbg_env_semantics.argtypes = [C.c_char_p, C.c_size_t]
     L.rbg_env_semantics.restype = C.c_uint32
     L.rbg_batch_create.argtypes = [vp, C.POINTER(AssemblyIn), C.c_uint32, C.c_double, C.POINTER(vp)]
@@ -196,6 +216,101 @@ def lib():
 
 def _p(a, t):
     return a.ctypes.data_as(t) if a is not None else None
+
+
+_CACHE_SCALARS = ("nobs", "nobs1", "samples", "seed", "total_size", "total_size1", "sim_cutoff", "b0",
+                  "min_contig", "min_small_contig", "has_depth")
+
+
+def _names_buffer(names):
+    enc = [x.encode() if isinstance(x, str) else bytes(x) for x in names]
+    off = np.zeros(len(enc) + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in enc])
+    return np.frombuffer(b"".join(enc) or b"\0", np.uint8).copy(), off
+
+
+def write_cache(path, **kw):
+    """rbg_cache_write: an RBCACHE1 v3 file from keyword fields (the scalars
+    of rbg_cache, names/small_names as lists of str, seq_lens/small_seq_lens,
+    depth/small_depth as 2-D float32, large_means, depth_raw, colnorm, and the
+    graph as from_/to/scomp)."""
+    c = CacheFile()
+    keep = []
+    for k in _CACHE_SCALARS:
+        if k in kw:
+            setattr(c, k, kw[k])
+
+    def arr(name, dt, ptr_t):
+        a = kw.get(name)
+        if a is None:
+            return None, 0
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data_as(ptr_t), a.size
+
+    for key, off_key, n_key in (("names", "name_off", "nobs"), ("small_names", "small_name_off", "nobs1")):
+        if kw.get(key) is not None:
+            buf, off = _names_buffer(kw[key])
+            keep += [buf, off]
+            setattr(c, key, buf.ctypes.data)
+            setattr(c, off_key, off.ctypes.data_as(u64p))
+    c.seq_lens, _ = arr("seq_lens", np.uint64, u64p)
+    c.small_seq_lens, _ = arr("small_seq_lens", np.uint64, u64p)
+    for key, rk, ck in (("depth", "depth_rows", "depth_cols"), ("small_depth", "small_depth_rows", "small_depth_cols")):
+        if kw.get(key) is not None:
+            a = np.ascontiguousarray(kw[key], np.float32)
+            keep.append(a)
+            setattr(c, key, a.ctypes.data_as(f32p))
+            setattr(c, rk, a.shape[0])
+            setattr(c, ck, a.shape[1])
+    c.large_means, c.n_large_means = arr("large_means", np.float32, f32p)
+    c.depth_raw, c.n_depth_raw = arr("depth_raw", np.float32, f32p)
+    c.colnorm, c.n_colnorm = arr("colnorm", np.float64, f64p)
+    c.from_, c.n_edges = arr("from_", np.uint32, u32p)
+    c.to, _ = arr("to", np.uint32, u32p)
+    c.scomp, _ = arr("scomp", np.float32, f32p)
+    st = lib().rbg_cache_write(path.encode(), C.byref(c))
+    if st != 0:
+        raise RbgError(f"rbg_cache_write({path}): status {st}")
+
+
+def read_cache(path):
+    """rbg_cache_read: the fields of an RBCACHE1 file (versions 1-3) as a dict."""
+    c = CacheFile()
+    st = lib().rbg_cache_read(path.encode(), C.byref(c))
+    if st != 0:
+        raise RbgError(f"rbg_cache_read({path}): status {st}")
+    try:
+        out = {k: getattr(c, k) for k in ("version",) + _CACHE_SCALARS}
+
+        def arr(ptr, n):
+            return np.ctypeslib.as_array(ptr, (n,)).copy() if (n and ptr) else None
+
+        def names(buf, off, n):
+            if not buf or not n:
+                return None
+            o = np.ctypeslib.as_array(off, (n + 1,)).copy()
+            raw = C.string_at(buf, int(o[-1]))
+            return [raw[o[i]:o[i + 1]].decode() for i in range(n)]
+
+        out["names"] = names(c.names, c.name_off, c.nobs)
+        out["small_names"] = names(c.small_names, c.small_name_off, c.nobs1)
+        out["seq_lens"] = arr(c.seq_lens, c.nobs)
+        out["small_seq_lens"] = arr(c.small_seq_lens, c.nobs1)
+        d = arr(c.depth, c.depth_rows * c.depth_cols)
+        out["depth"] = None if d is None else d.reshape(c.depth_rows, c.depth_cols)
+        d = arr(c.small_depth, c.small_depth_rows * c.small_depth_cols)
+        out["small_depth"] = None if d is None else d.reshape(c.small_depth_rows, c.small_depth_cols)
+        out["large_means"] = arr(c.large_means, c.n_large_means)
+        out["depth_raw"] = arr(c.depth_raw, c.n_depth_raw)
+        out["colnorm"] = arr(c.colnorm, c.n_colnorm)
+        out["from_"] = arr(c.from_, c.n_edges)
+        out["to"] = arr(c.to, c.n_edges)
+        out["scomp"] = arr(c.scomp, c.n_edges)
+        out["n_edges"] = c.n_edges
+        return out
+    finally:
+        lib().rbg_cache_free(C.byref(c))
 
 
 def env_semantics():

@@ -281,3 +281,24 @@ def test_graph_from_arbitrary_winners(ctx):
                                       params=api.default_params(rand_seed=1, regime=2))
     assert g["b0"] == b0
     assert set(zip(g["from"].tolist(), g["to"].tolist())) == set(zip(ref["from"].tolist(), ref["to"].tolist()))
+
+
+def test_graph_round_trips_through_cache(ctx, tmp_path):
+    """A graph built on the GPU written as an RBCACHE1 v3 file (with the
+    ranked depth matrix, raw depths and column norms the reference caches)
+    reads back unchanged."""
+    seqs, off, depth, _ = synth.make_assembly(2600, 25, np.log(8000), 0.8, 10, seed=31)
+    g = ctx.build_graph(seqs, off, depth, api.default_params(rand_seed=1, regime=2))
+    ranked, _, raw, colnorm, _ = api.depth_features(depth)
+    n = len(off) - 1
+    p = str(tmp_path / "asm.cache")
+    api.write_cache(p, nobs=n, nobs1=0, samples=10, seed=1, total_size=int(off[-1]), total_size1=0,
+                    sim_cutoff=g["sim_cutoff"], b0=g["b0"], min_contig=2500, min_small_contig=1000, has_depth=1,
+                    names=[f"c{i}" for i in range(n)], small_names=[], seq_lens=np.diff(off.astype(np.int64)),
+                    small_seq_lens=np.zeros(0, np.uint64), depth=ranked, small_depth=np.zeros((0, 10), np.float32),
+                    large_means=depth.ravel(), depth_raw=raw.ravel(), colnorm=colnorm, from_=g["from"], to=g["to"],
+                    scomp=g["scomp"])
+    c = api.read_cache(p)
+    assert c["sim_cutoff"] == g["sim_cutoff"] and c["b0"] == g["b0"] and c["n_edges"] == g["n_edges"]
+    assert np.array_equal(c["from_"], g["from"]) and np.array_equal(c["to"], g["to"])
+    assert np.array_equal(c["scomp"], g["scomp"]) and np.array_equal(c["depth"], ranked)
