@@ -453,6 +453,23 @@ rbg_status rbg_cache_write(const char* path, const rbg_cache* c);
 rbg_status rbg_cache_read(const char* path, rbg_cache* out);
 void rbg_cache_free(rbg_cache* c);
 
+/* ---------------------------------------------------------------------------
+ * Fisher label propagation (host): cluster_by_propagation with its default
+ * options (src/impl/rb_cluster.cpp:376-717) on the graph the reference
+ * prepares for it (src/rabbitbin.cpp:5597-5651): the edges with score > 0,
+ * scores clamped to 1 - 1e-6, each node's incident edges in ascending
+ * neighbour id, the node order iota(n) shuffled by std::shuffle with
+ * std::default_random_engine(seed) (seed = --seed).  membership (n) receives
+ * each node's label (a node id); rounds_out (may be NULL) the number of
+ * sweeps.  With no positive edge every node keeps its own label.  Scores must
+ * be non-negative; endpoints distinct and below n.  Sequential by definition
+ * (each update sees the labels changed before it in the order), so it runs
+ * on the host.  rbg_lpa_node_order returns that order alone.
+ * ------------------------------------------------------------------------- */
+rbg_status rbg_label_propagation(const uint32_t* from, const uint32_t* to, const float* score, uint64_t ne,
+                                 uint32_t n, uint64_t seed, uint32_t* membership, uint32_t* rounds_out);
+rbg_status rbg_lpa_node_order(uint32_t n, uint64_t seed, uint32_t* order);
+
 /* Chance baseline b0 over 16-bit winners (src/rabbitbin.cpp:1340-1369). */
 rbg_status rbg_pmh_baseline(rbg_ctx* ctx, const uint16_t* win16, uint32_t n, uint32_t m,
                             double* b0);

@@ -210,6 +210,39 @@ def edge_near(frm, to, scomp, unit, raw, colnorm, S, tol, min_edge_weight=0.70, 
     return np.flatnonzero(near).astype(np.uint64)
 
 
+def chi2_cdf(m, x):
+    """CDF of the chi-squared distribution with 2m degrees of freedom at x, in
+    the reference's Poisson-sum form."""
+    L = orc()
+    L.orc_chi2_cdf.restype = C.c_double
+    L.orc_chi2_cdf.argtypes = [C.c_int, C.c_double]
+    return L.orc_chi2_cdf(m, x)
+
+
+def lpa_node_order(n, seed):
+    L = orc()
+    L.orc_lpa_node_order.argtypes = [C.c_uint64, C.c_uint64, u64p]
+    out = np.zeros(n, np.uint64)
+    L.orc_lpa_node_order(n, seed, _p(out, u64p))
+    return out
+
+
+def label_propagation(frm, to, score, n, seed, with_frozen=False):
+    """Fisher label propagation (the reference's default sequential sweep):
+    (membership, rounds[, frozen flags])."""
+    L = orc()
+    L.orc_label_propagation.restype = C.c_int
+    L.orc_label_propagation.argtypes = [u32p, u32p, f32p, C.c_uint64, C.c_uint64, C.c_uint64, u64p, u8p]
+    frm = np.ascontiguousarray(frm, np.uint32)
+    to = np.ascontiguousarray(to, np.uint32)
+    sc = np.ascontiguousarray(score, np.float32)
+    out = np.zeros(n, np.uint64)
+    frozen = np.zeros(n, np.uint8)
+    r = L.orc_label_propagation(_p(frm, u32p), _p(to, u32p), _p(sc, f32p), len(frm), n, seed, _p(out, u64p),
+                                _p(frozen, u8p))
+    return (out, r, frozen) if with_frozen else (out, r)
+
+
 def pair_pass_rows(win16, unit, anynz, S, b0, row0, row1, K=200, threads=0):
     """Timed fused pair pass of rows [row0, row1) against all partners."""
     L = orc()

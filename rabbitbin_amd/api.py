@@ -192,6 +192,9 @@ def lib():
     L.rbg_cache_write.argtypes = [C.c_char_p, C.POINTER(CacheFile)]
     L.rbg_cache_read.argtypes = [C.c_char_p, C.POINTER(CacheFile)]
     L.rbg_cache_free.argtypes = [C.POINTER(CacheFile)]
+    L.rbg_label_propagation.argtypes = [u32p, u32p, f32p, C.c_uint64, C.c_uint32, C.c_uint64, u32p,
+                                        C.POINTER(C.c_uint32)]
+    L.rbg_lpa_node_order.argtypes = [C.c_uint32, C.c_uint64, u32p]
     L.rbg_env_semantics.argtypes = [C.c_char_p, C.c_size_t]
     L.rbg_env_semantics.restype = C.c_uint32
     L.rbg_batch_create.argtypes = [vp, C.POINTER(AssemblyIn), C.c_uint32, C.c_double, C.POINTER(vp)]
@@ -311,6 +314,28 @@ def read_cache(path):
         return out
     finally:
         lib().rbg_cache_free(C.byref(c))
+
+
+def label_propagation(frm, to, score, n, seed=1):
+    """rbg_label_propagation: (membership uint32[n], rounds)."""
+    frm = np.ascontiguousarray(frm, np.uint32)
+    to = np.ascontiguousarray(to, np.uint32)
+    sc = np.ascontiguousarray(score, np.float32)
+    mem = np.zeros(n, np.uint32)
+    rounds = C.c_uint32()
+    st = lib().rbg_label_propagation(_p(frm, u32p), _p(to, u32p), _p(sc, f32p), len(frm), n, seed, _p(mem, u32p),
+                                     C.byref(rounds))
+    if st != 0:
+        raise RbgError(f"rbg_label_propagation: status {st}")
+    return mem, rounds.value
+
+
+def lpa_node_order(n, seed=1):
+    out = np.zeros(n, np.uint32)
+    st = lib().rbg_lpa_node_order(n, seed, _p(out, u32p))
+    if st != 0:
+        raise RbgError(f"rbg_lpa_node_order: status {st}")
+    return out
 
 
 def env_semantics():
