@@ -11,6 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libref_probmh.so")
+PORT512_LIB = os.path.join(HERE, "liboracle_port512.so")
 
 u8p = C.POINTER(C.c_uint8)
 u16p = C.POINTER(C.c_uint16)
@@ -33,8 +34,16 @@ class GraphOut(C.Structure):
                 ("rowmax_cnt", u32p), ("regime", C.c_int), ("gated_pairs", C.c_uint64)]
 
 
+class PortStats(C.Structure):
+    _fields_ = [("prep_s", C.c_double), ("pass_s", C.c_double), ("calib_s", C.c_double),
+                ("mutual_s", C.c_double), ("tile", C.c_uint64), ("tilepairs", C.c_uint64),
+                ("q8_pruned", C.c_uint64), ("float_checked", C.c_uint64), ("ub_pruned", C.c_uint64),
+                ("compares", C.c_uint64), ("avx512", C.c_int), ("q8", C.c_int)]
+
+
 _orc = None
 _ref = None
+_port512 = None
 
 
 def _p(a, t):
@@ -241,6 +250,73 @@ def label_propagation(frm, to, score, n, seed, with_frozen=False):
     r = L.orc_label_propagation(_p(frm, u32p), _p(to, u32p), _p(sc, f32p), len(frm), n, seed, _p(out, u64p),
                                 _p(frozen, u8p))
     return (out, r, frozen) if with_frozen else (out, r)
+
+
+def host_avx512_vnni():
+    """True when the host CPU has AVX-512 F/BW/VNNI (from /proc/cpuinfo)."""
+    try:
+        flags = set()
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+        return {"avx512f", "avx512bw", "avx512_vnni"} <= flags
+    except OSError:
+        return False
+
+
+def _port_fn(avx512):
+    global _port512
+    L = orc()
+    if avx512:
+        if _port512 is None:
+            _port512 = C.CDLL(PORT512_LIB)
+            _port512.orc_port_fused_graph_avx512.argtypes = [C.POINTER(GraphIn), C.POINTER(GraphOut),
+                                                             C.POINTER(PortStats)]
+        return _port512.orc_port_fused_graph_avx512
+    L.orc_port_fused_graph.argtypes = [C.POINTER(GraphIn), C.POINTER(GraphOut), C.POINTER(PortStats)]
+    return L.orc_port_fused_graph
+
+
+def port_fused_graph(win16, unit, anynz, S, b0, rand_seed=1, K=200, min_edge_weight=0.70, coverage=0.95,
+                     threads=0, avx512=None):
+    """The fused regime by the performance port of the reference's fused pass
+    (triangle tile pass, int8 prefilter, composition early exit, spin-locked
+    row heaps, mutual drain).  avx512=None uses the AVX-512 build where the
+    host has it.  Edges come back in the port's emission order."""
+    if avx512 is None:
+        avx512 = host_avx512_vnni() and os.path.exists(PORT512_LIB)
+    w = np.ascontiguousarray(win16, np.uint16)
+    n, m = w.shape
+    u = np.ascontiguousarray(unit, np.float32) if unit is not None else None
+    nz = np.ascontiguousarray(anynz, np.uint8) if anynz is not None else None
+    gi = GraphIn(_p(w, u16p), _p(u, f32p), _p(nz, u8p), n, m, S, K, min_edge_weight, coverage,
+                 w_comp_for(S), b0, rand_seed, threads, 2)
+    go = GraphOut()
+    ps = PortStats()
+    st = _port_fn(avx512)(C.byref(gi), C.byref(go), C.byref(ps))
+    if st != 0:
+        raise RuntimeError(f"orc_port_fused_graph status {st}")
+    ne = go.n_edges
+    res = {
+        "sim_cutoff": go.sim_cutoff, "regime": go.regime, "gated_pairs": go.gated_pairs,
+        "from": np.ctypeslib.as_array(go.from_, (ne,)).copy() if ne else np.zeros(0, np.uint32),
+        "to": np.ctypeslib.as_array(go.to, (ne,)).copy() if ne else np.zeros(0, np.uint32),
+        "sv": np.ctypeslib.as_array(go.sv, (ne,)).copy() if ne else np.zeros(0, np.float32),
+        "stats": {k: getattr(ps, k) for k, _ in PortStats._fields_},
+    }
+    orc().orc_graph_free(C.byref(go))
+    return res
+
+
+def omp_info():
+    """OpenMP placement of liboracle.so: (proc_bind, places, max_threads)."""
+    L = orc()
+    L.orc_omp_info.argtypes = [C.POINTER(C.c_int)] * 3
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    L.orc_omp_info(C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
 
 
 def pair_pass_rows(win16, unit, anynz, S, b0, row0, row1, K=200, threads=0):

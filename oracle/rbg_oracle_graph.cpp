@@ -20,64 +20,13 @@
 
 #include <omp.h>
 
+#include "rbg_oracle_calib.h"
+
 namespace {
 
-struct RandStream {
-  struct random_data rd;
-  char state[128];
-  explicit RandStream(unsigned seed) {
-    std::memset(&rd, 0, sizeof(rd));
-    initstate_r(seed, state, sizeof(state), &rd);
-  }
-  int next() {
-    int32_t r;
-    random_r(&rd, &r);
-    return r;
-  }
-};
-
-// random_unique (src/rabbitbin.h:330-341).
-template <class It>
-void random_unique(It begin, It end, size_t num, RandStream& rs) {
-  size_t left = (size_t)std::distance(begin, end);
-  while (num--) {
-    It r = begin;
-    std::advance(r, (size_t)rs.next() % left);
-    std::swap(*begin, *r);
-    ++begin;
-    --left;
-  }
-}
-
-// calib_converge (src/impl/rb_graph.cpp:390-415), RB_SIM_FLOOR = 50.
-size_t calib_converge(const std::vector<double>& maxsim, double coverage, RandStream& rs) {
-  const size_t n = maxsim.size();
-  size_t p = 999, pp = 1000;
-  double cov = 0, pcov = 0;
-  for (; p > 50;) {
-    const double cutoff = (double)p / 1000.;
-    size_t on = 0;
-    for (size_t i = 0; i < n; ++i)
-      if (maxsim[i] >= cutoff) on++;
-    cov = (double)on / n;
-    if (cov >= coverage) {
-      if (cov - coverage > coverage - pcov) {
-        p = pp;
-        cov = pcov;
-      }
-      break;
-    }
-    pp = p;
-    pcov = cov;
-    if (p > 990)
-      p -= rs.next() % 3 + 1;
-    else if (p > 900)
-      p -= rs.next() % 3 + 3;
-    else
-      p -= rs.next() % 3 + 9;
-  }
-  return p;
-}
+using orc_calib::RandStream;
+using orc_calib::calib_converge;
+using orc_calib::random_unique;
 
 struct Cand {
   uint32_t id;
