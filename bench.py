@@ -30,10 +30,9 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-# OpenMP placement for the host-side work, the CPU baseline's above all:
-# read once, when the first OpenMP library loads, so set before any import.
-os.environ.setdefault("OMP_PROC_BIND", "spread")
-os.environ.setdefault("OMP_PLACES", "cores")
+# CPUs this process may run on, read before any OpenMP library loads (with
+# OMP_PROC_BIND set, OpenMP binds the initial thread to one place)
+AFFINITY = len(os.sched_getaffinity(0))
 
 PEAK_HBM_GBS = 8000.0
 PEAK_INT8_TOPS = 5000.0  # dense int8 MFMA, MI355X
@@ -195,7 +194,7 @@ def cpu_baseline(seqs, offsets, depth, n, budget_s, threads, contigs=None):
                         "early_exit": st["ub_pruned"], "compares": st["compares"], "tile": st["tile"]},
         "edges": edges,
         "nproc": os.cpu_count(),
-        "affinity": len(os.sched_getaffinity(0)),
+        "affinity": AFFINITY,
         "cpu_model": model,
         "avx512_flags": avx,
         "omp_proc_bind": {0: "false", 1: "true", 2: "primary", 3: "close", 4: "spread"}.get(bind, str(bind)),
@@ -475,17 +474,36 @@ def plain_form(args, ctx):
         "roofline": roofline,
         "kernels": kernels,
     }
-    if not args.no_cpu_baseline:
-        try:
-            # the GPU's share of the host: 16 cores per GPU, fewer where the
-            # process may run on fewer
-            threads = min(16, len(os.sched_getaffinity(0)))
-            out["cpu_baseline"] = cpu_baseline(seqs, offsets, depth, n, args.cpu_seconds, threads,
-                                               args.cpu_contigs)
-        except Exception as e:  # the baseline is reported, never required
-            out["cpu_baseline"] = {"error": repr(e)}
     asm.free()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_child(args)
     return out
+
+
+def cpu_baseline_child(args):
+    """Run the CPU baseline in a child process of its own, with OpenMP threads
+    spread over the cores (OMP_PROC_BIND=spread, OMP_PLACES=cores): OpenMP
+    reads its placement once per process and binds the initial thread, which
+    must not happen to the process that drives the GPU.  The child makes the
+    same synthetic workload and touches no GPU."""
+    import subprocess
+    # the GPU's share of the host: 16 cores per GPU, fewer where the process
+    # may run on fewer
+    threads = min(16, AFFINITY)
+    env = dict(os.environ, OMP_PROC_BIND="spread", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload", args.workload,
+           "--cpu-seconds", str(args.cpu_seconds), "--cpu-threads", str(threads)]
+    if args.n:
+        cmd += ["--n", str(args.n)]
+    if args.cpu_contigs:
+        cmd += ["--cpu-contigs", str(args.cpu_contigs)]
+    try:
+        r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, timeout=20 * args.cpu_seconds + 600)
+        if r.returncode != 0:
+            return {"error": f"CPU baseline exited with status {r.returncode}"}
+        return json.loads(r.stdout.decode().strip().splitlines()[-1])
+    except Exception as e:  # the baseline is reported, never required
+        return {"error": repr(e)}
 
 
 def main():
@@ -499,11 +517,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-contigs", type=int, default=None,
                     help="time the CPU baseline on this many contigs (default: sized from --cpu-seconds)")
+    ap.add_argument("--cpu-baseline-only", action="store_true",
+                    help="print the CPU baseline's JSON object and exit (no GPU use)")
+    ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--forms", default="plain,batched,served",
                     help="comma list of: plain (the headline line), batched, served")
     ap.add_argument("--served-sessions", type=int, default=24)
     ap.add_argument("--served-rate", type=float, default=4.0)
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        from rabbitbin_amd import synth
+        seqs, offsets, depth, _ = synth.workload(args.workload, n=args.n)
+        threads = args.cpu_threads or min(16, AFFINITY)
+        print(json.dumps(cpu_baseline(seqs, offsets, depth, len(offsets) - 1, args.cpu_seconds, threads,
+                                      args.cpu_contigs)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               f"(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
