@@ -166,7 +166,10 @@ def cpu_baseline(seqs, offsets, depth, n, budget_s, threads, contigs=None):
         target = budget_s / 3.0
         ns = int((-lin + (lin * lin + 2 * quad * target) ** 0.5) / quad) if quad > 0 else n
         ns = max(pilot_n, min(n, ns))
-    runs = [cpu_pipeline(orc, seqs, offsets, depth, ns, threads) for _ in range(3)]
+    runs = []
+    for r in range(3):
+        runs.append(cpu_pipeline(orc, seqs, offsets, depth, ns, threads))
+        log(f"[bench/cpu] run {r + 1} of 3 over {ns} contigs: {runs[-1][0]['end_to_end']:.2f} s")
     med = {k: float(np.median([r[0][k] for r in runs])) for k in runs[0][0]}
     st, edges = runs[0][1], runs[0][2]
     pairs_s = ns * (ns - 1) / 2
