@@ -474,6 +474,66 @@ rbg_status rbg_lpa_node_order(uint32_t n, uint64_t seed, uint32_t* order);
 rbg_status rbg_pmh_baseline(rbg_ctx* ctx, const uint16_t* win16, uint32_t n, uint32_t m,
                             double* b0);
 
+/* ---------------------------------------------------------------------------
+ * Ingest on the device side of the boundary: the assembly FASTA and the depth
+ * file are parsed in HBM, and the large contigs stay there for the graph.
+ *
+ * rbg_ingest_fasta replaces parse_fasta_mmap_parallel (src/rabbitbin.cpp:
+ * 1890-2248) with its default options: the file's bytes (e.g. an mmap) are
+ * copied to the device, the records found and classified there (large:
+ * >= min_contig bases, small: >= min_small_contig, the rest tiny and
+ * dropped), and the large contigs' bases packed on the device in file order
+ * (multi-line records compacted, '\r' before a line end dropped).  Names
+ * (the header up to its first blank) and lengths come back to the host.
+ *
+ * rbg_ingest_depth replaces parse_depth_buf (src/rabbitbin.cpp:2832-2942)
+ * and the merge by name (:4495-4545): the depth file (header line, then per
+ * contig name, contigLen, totalAvgDepth and S mean / variance pairs;
+ * cv_ext: name, contigLen and S means) is parsed on the device with the
+ * C library's strtod / strtof rounding (exact), rows whose contigLen is below
+ * min_depth_len skipped (the reference's default prefilter, min_depth_len =
+ * min_small_contig; 0 keeps every row), the first row of a name kept, and
+ * each large and small contig given its row.  S = 0 derives the sample count
+ * from the first data line as the reference does (ncols - 3, halved; ncols - 1
+ * with cv_ext).  Returns RBG_ERR_ARG when a large contig has no row (the
+ * reference leaves that row undefined), and RBG_ERR_UNSUPPORTED for
+ * duplicate large-contig names (the reference drops the later ones), for
+ * hexadecimal floats, NaN payloads, more than 19 significant digits or
+ * magnitudes beyond the exact conversion's range (about 1e-30 .. 1e38).
+ *
+ * rbg_ingest_assembly makes a resident assembly (rbg_asm) of the large
+ * contigs and their depth rows without taking the bases back to the host;
+ * it then builds graphs like any rbg_asm_create assembly.
+ * ------------------------------------------------------------------------- */
+typedef struct rbg_ingest rbg_ingest;
+typedef struct rbg_ingest_info {
+  uint64_t records;            /* FASTA records with a header line (the reference's num_seqs) */
+  uint32_t n_large, n_small;   /* large and small contigs, file order */
+  uint64_t large_bases;        /* total length of the large contigs */
+  const char* names;           /* large contigs' names back to back: name i = */
+  const uint64_t* name_off;    /*   names[name_off[i] .. name_off[i+1]) */
+  const uint64_t* lens;        /* large contigs' lengths */
+  const char* small_names;
+  const uint64_t* small_name_off;
+  const uint64_t* small_lens;
+  uint32_t S;                  /* depth samples (0 before rbg_ingest_depth) */
+  uint64_t depth_rows;         /* rows of the depth file's name map */
+  double ms_fasta, ms_depth;   /* device time of the parses (upload included) */
+} rbg_ingest_info;
+rbg_status rbg_ingest_fasta(rbg_ctx* ctx, const char* data, uint64_t len, uint64_t min_contig,
+                            uint64_t min_small_contig, rbg_ingest** out);
+rbg_status rbg_ingest_depth(rbg_ctx* ctx, rbg_ingest* ig, const char* data, uint64_t len, uint32_t S,
+                            int cv_ext, uint64_t min_depth_len);
+/* The pointers stay valid until rbg_ingest_free. */
+rbg_status rbg_ingest_get(const rbg_ingest* ig, rbg_ingest_info* info);
+/* Copies to the host (any may be NULL): the large contigs' bases (large_bases
+ * bytes), their offsets (n_large + 1), and the depth rows of the large and
+ * small contigs (n x S raw means). */
+rbg_status rbg_ingest_copy(rbg_ctx* ctx, const rbg_ingest* ig, char* seqs, uint64_t* offsets, float* depth,
+                           float* small_depth);
+rbg_status rbg_ingest_assembly(rbg_ctx* ctx, const rbg_ingest* ig, double min_cv, rbg_asm** out);
+void rbg_ingest_free(rbg_ingest* ig);
+
 #ifdef __cplusplus
 }
 #endif

@@ -252,6 +252,62 @@ def label_propagation(frm, to, score, n, seed, with_frozen=False):
     return (out, r, frozen) if with_frozen else (out, r)
 
 
+class FaRec(C.Structure):
+    _fields_ = [("name_beg", C.c_uint64), ("seq_off", C.c_uint64), ("seq_len", C.c_uint64),
+                ("name_len", C.c_uint32), ("cls", C.c_uint32)]
+
+
+def fasta_parse(data, min_contig=2500, min_small=1000):
+    """The reference's FASTA parse of `data` (bytes): a list of records
+    (name, class 'tiny' | 'small' | 'large', sequence bytes or None for tiny)
+    in file order, and num_seqs."""
+    L = orc()
+    L.orc_fasta_parse.restype = C.c_uint64
+    L.orc_fasta_parse.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(FaRec), C.c_uint64,
+                                  C.c_char_p, u64p]
+    buf = bytes(data)
+    cap = buf.count(b">") + 1
+    recs = (FaRec * cap)()
+    bases = C.create_string_buffer(len(buf) + 1)
+    ns = np.zeros(1, np.uint64)
+    nr = L.orc_fasta_parse(buf, len(buf), min_contig, min_small, recs, cap, bases, _p(ns, u64p))
+    out = []
+    raw = bases.raw
+    for k in range(nr):
+        r = recs[k]
+        name = buf[r.name_beg:r.name_beg + r.name_len]
+        cls = {1: "tiny", 2: "small", 3: "large"}[r.cls]
+        seq = raw[r.seq_off:r.seq_off + r.seq_len] if r.cls > 1 else None
+        out.append((name, cls, seq))
+    return out, int(ns[0])
+
+
+def depth_parse_match(data, names, S, cv_ext=False, min_depth_len=1000):
+    """The reference's depth-file parse (strtod / strtof) and the row of each
+    name: (n x S float32 means, found flags, rows in the map)."""
+    L = orc()
+    L.orc_depth_parse_match.restype = C.c_uint64
+    L.orc_depth_parse_match.argtypes = [C.c_char_p, C.c_uint64, C.c_uint32, C.c_int, C.c_uint64, C.c_char_p, u64p,
+                                        C.c_uint64, f32p, u8p]
+    buf = bytes(data)
+    blob = b"".join(names)
+    off = np.zeros(len(names) + 1, np.uint64)
+    np.cumsum([len(x) for x in names], out=off[1:])
+    out = np.zeros((len(names), S), np.float32)
+    found = np.zeros(len(names), np.uint8)
+    rows = L.orc_depth_parse_match(buf, len(buf), S, int(cv_ext), min_depth_len, blob, _p(off, u64p), len(names),
+                                   _p(out, f32p), _p(found, u8p))
+    return out, found.astype(bool), int(rows)
+
+
+def depth_ncols(data):
+    L = orc()
+    L.orc_depth_ncols.restype = C.c_uint64
+    L.orc_depth_ncols.argtypes = [C.c_char_p, C.c_uint64]
+    buf = bytes(data)
+    return int(L.orc_depth_ncols(buf, len(buf)))
+
+
 def host_avx512_vnni():
     """True when the host CPU has AVX-512 F/BW/VNNI (from /proc/cpuinfo)."""
     try:

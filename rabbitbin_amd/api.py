@@ -61,6 +61,13 @@ SEMANTICS = {
 RBG_SEM_NO_MUTUAL = SEMANTICS["RABBIT_MUTUAL_KNN"]
 
 
+class IngestInfo(C.Structure):
+    _fields_ = [("records", C.c_uint64), ("n_large", C.c_uint32), ("n_small", C.c_uint32),
+                ("large_bases", C.c_uint64), ("names", C.c_void_p), ("name_off", u64p), ("lens", u64p),
+                ("small_names", C.c_void_p), ("small_name_off", u64p), ("small_lens", u64p),
+                ("S", C.c_uint32), ("depth_rows", C.c_uint64), ("ms_fasta", C.c_double), ("ms_depth", C.c_double)]
+
+
 class AssemblyIn(C.Structure):
     _fields_ = [("seqs", C.c_char_p), ("offsets", C.POINTER(C.c_uint64)), ("n", C.c_uint32),
                 ("depth", C.POINTER(C.c_float)), ("S", C.c_uint32)]
@@ -203,6 +210,12 @@ def lib():
     L.rbg_batch_result.argtypes = [vp, vp, C.c_uint32, C.POINTER(GraphResult)]
     L.rbg_batch_autotune.argtypes = [vp, vp, C.POINTER(GraphParams), C.c_uint32, C.POINTER(Schedule),
                                      C.POINTER(C.c_double)]
+    L.rbg_ingest_fasta.argtypes = [vp, C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(vp)]
+    L.rbg_ingest_depth.argtypes = [vp, vp, C.c_char_p, C.c_uint64, C.c_uint32, C.c_int, C.c_uint64]
+    L.rbg_ingest_get.argtypes = [vp, C.POINTER(IngestInfo)]
+    L.rbg_ingest_copy.argtypes = [vp, vp, C.c_char_p, u64p, f32p, f32p]
+    L.rbg_ingest_assembly.argtypes = [vp, vp, C.c_double, C.POINTER(vp)]
+    L.rbg_ingest_free.argtypes = [vp]
     L.rbg_server_default_config.argtypes = [C.POINTER(ServerConfig)]
     L.rbg_server_start.argtypes = [C.c_int, C.POINTER(ServerConfig), C.POINTER(vp)]
     L.rbg_server_stop.argtypes = [vp]
@@ -553,6 +566,78 @@ class Assembly:
     def free(self):
         if self._h:
             self.ctx._L.rbg_asm_free(self._h)
+            self._h = C.c_void_p()
+
+
+class Ingest:
+    """FASTA and depth file parsed on the device (rbg_ingest_fasta /
+    rbg_ingest_depth); the large contigs stay in HBM."""
+
+    def __init__(self, ctx, fasta, min_contig=2500, min_small_contig=1000):
+        self.ctx = ctx
+        data = bytes(fasta)
+        self.min_small_contig = min_small_contig
+        self._h = C.c_void_p()
+        ctx._check(ctx._L.rbg_ingest_fasta(ctx._h, data, len(data), min_contig, min_small_contig,
+                                           C.byref(self._h)), "rbg_ingest_fasta")
+
+    def depth(self, data, S=0, cv_ext=False, min_depth_len=None):
+        """Parse the depth file; min_depth_len defaults to min_small_contig
+        (the reference's prefilter)."""
+        data = bytes(data)
+        mdl = self.min_small_contig if min_depth_len is None else min_depth_len
+        self.ctx._check(self.ctx._L.rbg_ingest_depth(self.ctx._h, self._h, data, len(data), S, int(cv_ext), mdl),
+                        "rbg_ingest_depth")
+
+    def info(self):
+        i = IngestInfo()
+        st = self.ctx._L.rbg_ingest_get(self._h, C.byref(i))
+        if st != 0:
+            raise RbgError(f"rbg_ingest_get: status {st}")
+
+        def names(ptr, off, n):
+            if n == 0:
+                return []
+            o = np.ctypeslib.as_array(off, (n + 1,)).copy()
+            blob = C.string_at(ptr, int(o[-1])) if o[-1] else b""
+            return [blob[o[k]:o[k + 1]] for k in range(n)]
+
+        def arr(p, n):
+            return np.ctypeslib.as_array(p, (n,)).copy() if n else np.zeros(0, np.uint64)
+
+        return {"records": i.records, "n_large": i.n_large, "n_small": i.n_small, "large_bases": i.large_bases,
+                "names": names(i.names, i.name_off, i.n_large), "lens": arr(i.lens, i.n_large),
+                "small_names": names(i.small_names, i.small_name_off, i.n_small),
+                "small_lens": arr(i.small_lens, i.n_small), "S": i.S, "depth_rows": i.depth_rows,
+                "ms_fasta": i.ms_fasta, "ms_depth": i.ms_depth}
+
+    def copy(self):
+        """(seqs uint8, offsets uint64[n+1], depth float32[n, S] or None, small_depth or None)."""
+        inf = self.info()
+        n, ns, S = inf["n_large"], inf["n_small"], inf["S"]
+        seqs = np.zeros(max(int(inf["large_bases"]), 1), np.uint8)
+        off = np.zeros(n + 1, np.uint64)
+        dep = np.zeros((n, S), np.float32) if S else None
+        sdep = np.zeros((ns, S), np.float32) if S else None
+        self.ctx._check(self.ctx._L.rbg_ingest_copy(self.ctx._h, self._h, C.c_char_p(seqs.ctypes.data),
+                                                    _p(off, u64p), _p(dep, f32p), _p(sdep, f32p)),
+                        "rbg_ingest_copy")
+        return seqs[:int(inf["large_bases"])], off, dep, sdep
+
+    def assembly(self, min_cv=1.0):
+        """A resident Assembly of the large contigs and their depth rows."""
+        a = Assembly.__new__(Assembly)
+        a.ctx = self.ctx
+        inf = self.info()
+        a.n, a.S = inf["n_large"], inf["S"]
+        a._h = C.c_void_p()
+        self.ctx._check(self.ctx._L.rbg_ingest_assembly(self.ctx._h, self._h, min_cv, C.byref(a._h)),
+                        "rbg_ingest_assembly")
+        return a
+
+    def free(self):
+        if self._h:
+            self.ctx._L.rbg_ingest_free(self._h)
             self._h = C.c_void_p()
 
 

@@ -247,14 +247,11 @@ rbg_status d2h_parts(rbg_ctx* ctx, const D2HPart* parts, size_t np) {
   return RBG_OK;
 }
 
+}  // namespace
+
 // Several host -> device copies through the staging halves at once (parts
 // laid end to end as in d2h_parts): the host fills half h while half h - 1 is
 // in flight.  Returns once the staging is free again.
-struct H2DPart {
-  void* dst;
-  const void* src;
-  size_t bytes;
-};
 rbg_status h2d_parts(rbg_ctx* ctx, const H2DPart* parts, size_t np) {
   hipStream_t st = ctx->stream;
   RBG_TRY(ensure_staging(ctx));
@@ -287,7 +284,6 @@ rbg_status h2d_parts(rbg_ctx* ctx, const H2DPart* parts, size_t np) {
   return RBG_OK;
 }
 
-}  // namespace
 
 void* alloc_host_result(size_t bytes) {
   // Large arrays: 2 MiB aligned with transparent huge pages requested, so the
@@ -566,7 +562,11 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     for (uint32_t a = 0; a < count; ++a) {
       const rbg_assembly_in& x = in[a].in;
       const uint64_t nb = (x.n && !in[a].sk) ? x.offsets[x.n] - x.offsets[0] : 0;
-      if (nb) seq_parts.push_back(H2DPart{(char*)b->seqs.p + at, x.seqs + x.offsets[0], nb});
+      if (nb && in[a].seqs_on_device)
+        RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->seqs.p + at, x.seqs + x.offsets[0], nb, hipMemcpyDeviceToDevice,
+                                        st));
+      else if (nb)
+        seq_parts.push_back(H2DPart{(char*)b->seqs.p + at, x.seqs + x.offsets[0], nb});
       at += nb;
     }
     if (!seq_parts.empty()) RBG_TRY(h2d_parts(ctx, seq_parts.data(), seq_parts.size()));

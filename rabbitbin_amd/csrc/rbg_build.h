@@ -124,6 +124,7 @@ struct AsmInput {
   rbg_assembly_in in{};          // host contigs (and depth): the new rows
   const Resident* src = nullptr; // rows [0, src->n) come from here when set
   const SketchInput* sk = nullptr;  // set: in.seqs/in.depth are unused
+  bool seqs_on_device = false;   // in.seqs is device memory (ingest); offsets and depth stay host
 };
 
 // Host side of one assembly of a batch.
@@ -240,6 +241,13 @@ namespace rbg {
 // int8 rows, pair-pass order, b0 pair draws); rows from resident sessions are
 // copied on the device.
 rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t count, double min_cv);
+// Host -> device copies through the context's pinned staging.
+struct H2DPart {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+rbg_status h2d_parts(rbg_ctx* ctx, const H2DPart* parts, size_t np);
 // Descriptors, work items, capacities and calibration plans of one build
 // configuration, and every buffer at its size.
 rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P);
