@@ -98,26 +98,29 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // Stage (a): sketch.  One workgroup of four waves sketches four contigs.
 //
-// Phase 1: canonical 4-mer histograms in LDS (src/rabbitbin.cpp:1459-1484).
-// The 16-byte aligned chunks of the four contigs are numbered one after
-// another and the workgroup's 256 threads stride over all of them, so the
+// Phase 1: 4-mer histograms in LDS (src/rabbitbin.cpp:1459-1484).  The
+// 16-byte aligned chunks of the four contigs are numbered one after another
+// and the workgroup's 256 threads stride over all of them, so the
 // workgroup's time follows the sum of its contigs' lengths, not the longest
-// (contig lengths are log-normal).  Each wave counts into its own histogram
-// per contig; the four are summed when the weights are formed.  A thread
-// takes a chunk plus the dword before it (the 3-base overlap of 4-mers that
-// end in the chunk) and decodes them four bytes at a time:
+// (contig lengths are log-normal).  The four waves share one histogram per
+// contig.  A thread takes a chunk plus the dword before it (the 3-base
+// overlap of 4-mers that end in the chunk) and decodes them four bytes at a
+// time:
 //   the 2-bit code is ((byte >> 1) ^ (byte >> 2)) & 3 (A 0, C 1, G 2, T 3,
 //   either case); a byte is a base when byte & 0xDF is the upper-case
 //   letter of its code (any other byte breaks every 4-mer covering it, as
 //   the reference's window reset does); bytes outside the contig count as
 //   non-bases.
-// A 4-mer ending at byte r is valid when bytes r-3..r all are; its window
-// holds codes r-3..r in bits 0..7 (earliest base lowest), so the forward code
-// is the window with its 2-bit groups reversed and the reverse complement is
-// ~window & 0xFF.  The canonical code is the smaller of the two (the
-// histograms hold it at slot 255 - code).  Then the GC-normalised weights in
-// FP64, in the reference's operation order (src/rabbitbin.cpp:1491-1516),
-// kept as 1/w per element.
+// A 4-mer ending at byte r is valid when bytes r-3..r all are.  Its window E
+// holds codes r-3..r in bits 0..7 (earliest base lowest) and is counted as
+// it is: the reference's forward code F is E with its 2-bit groups reversed
+// and the reverse complement's code is 255 - E, so the windows whose
+// canonical code min(F, RC) is v are exactly those with E = rev(v) or
+// E = 255 - v (one slot when v is its own reverse complement), and the
+// canonical count is read from those two bins when the weights are formed.
+// Then the GC-normalised weights in FP64, in the reference's operation order
+// (src/rabbitbin.cpp:1491-1516), kept as 1/w per element; the base
+// probabilities and 1/total are computed once per contig.
 //
 // Phase 2: thread t owns registers k = t, t+256, ...; for each register the
 // winner is argmin_x fl((1/w_x) * T[x][k]) over present elements, ties to the
@@ -128,13 +131,20 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
 // smaller, and rounding is monotone), so once fl(wmin * T_r) > best no later
 // element can win or tie and the walk stops: typically after a few ranks.
 // Ranks are loaded and applied in groups of eight with the stop tested per
-// group (applying elements past the stop changes nothing).
+// group (applying elements past the stop changes nothing); a register's
+// first group is loaded before it is needed (the first register's while the
+// weights are formed, the next register's while the current one's planes
+// are written).  A wave's 64 registers give each contig two plane-word
+// groups: the eight ballots of a contig are gathered into lanes 0..15 with
+// writelane and written by one store (two 32-byte segments).
 //
 // The sequence buffer must be readable 16 bytes past its last contig (the
-// last chunk of a contig is read whole).
+// last chunk of a contig is read whole).  The dword before chunk 0 of the
+// buffer does not exist; the first dword is read in its place, and its bytes
+// stand before every contig, so no window or base takes them.
 // ---------------------------------------------------------------------------
 constexpr int kSketchWaves = 4;
-constexpr int kRankGroup = 8;  // proposal ranks loaded together in phase 2
+constexpr int kRankGroup = 4;  // proposal ranks loaded together in phase 2
 
 // Bit 7 of each byte set iff that byte of x is non-zero.
 __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
@@ -171,50 +181,36 @@ struct KmerTally {
   uint32_t kmers = 0;
 };
 
-// Histogram slot of the 4-mer ending at chunk byte i.  c40 holds the codes
+// Histogram slot E of the 4-mer ending at chunk byte i.  c40 holds the codes
 // of the chunk's 20 stream bytes (the dword before the chunk, then its 16
-// bytes; code t at bits 2t..2t+1) and nr40 the complement of the same codes
-// in reverse order (code t at bits 2(19-t)).  With E the window read from
-// c40 (earliest base lowest) and F the window with the earliest base highest
-// (the reference's forward code), the reverse complement's code is 255 - E,
-// so the canonical code min(F, 255 - E) = 255 - max(255 - F, E): the slot is
-// max(E, 255 - F), bins stored in reverse order.
-__device__ __forceinline__ uint32_t window_slot(uint64_t c40, uint64_t nr40, uint32_t i) {
-  const uint32_t e = (uint32_t)(c40 >> (2 * (i + 1))) & 0xFFu;
-  const uint32_t nf = (uint32_t)(nr40 >> (2 * (15 - i))) & 0xFFu;
-  return e > nf ? e : nf;
+// bytes; code t at bits 2t..2t+1), so the window of stream bytes i+1..i+4 is
+// bits 2(i+1)..2(i+1)+7.
+__device__ __forceinline__ uint32_t window_slot(uint64_t c40, uint32_t i) {
+  return (uint32_t)(c40 >> (2 * (i + 1))) & 0xFFu;
 }
 
-// The 4-mers ending in the 16 bytes [16j, 16j+16): every lane counts all 16
-// windows (nearly every chunk is all bases and inside its contig), then a
-// lane whose chunk is not takes the invalid windows off again, from the same
-// codes, so they add nothing.  prev is the dword at 16j-4, `first` the buffer
+// The 4-mers ending in the 16 bytes [16j, 16j+16), counted in pairs: the
+// 5-mer ending at chunk byte 2t+1 (t = 0..7) holds the 4-mers ending at 2t
+// (its first four bases) and 2t+1 (its last four), so one add to a 5-mer bin
+// counts two windows.  Every lane counts all eight 5-mers (nearly every chunk
+// is all bases and inside its contig); a lane whose chunk is not takes each
+// 5-mer with an invalid window off again and adds its valid window, if any,
+// to the 4-mer bins h4.  prev is the dword at 16j-4, `first` the buffer
 // position 16j-4 and [beg, end) the contig.
-__device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_t prev, uint4 cur,
-                                            int64_t first, int64_t beg, int64_t end,
-                                            KmerTally& tl) {
+__device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ h5, uint32_t* __restrict__ h4,
+                                            uint32_t prev, uint4 cur, int64_t first, int64_t beg,
+                                            int64_t end, KmerTally& tl) {
   uint32_t x[5];
   const uint32_t c0 = word_codes(prev, x[0]), c1 = word_codes(cur.x, x[1]),
                  c2 = word_codes(cur.y, x[2]), c3 = word_codes(cur.z, x[3]),
                  c4 = word_codes(cur.w, x[4]);
   const uint64_t c40 = (uint64_t)(c0 | (c1 << 8) | (c2 << 16) | (c3 << 24)) | ((uint64_t)c4 << 32);
-  // the same 20 codes in reverse order, complemented
-  uint64_t r40 = (uint64_t)__builtin_bitreverse64(c40) >> 24;  // each code's two bits swapped
-  r40 = ((r40 >> 1) & 0x5555555555555555ull) | ((r40 & 0x5555555555555555ull) << 1);
-  const uint64_t nr40 = ~r40;
-  // Window i = 4a + p: E is byte a of c40 >> 2(p+1), 255 - F byte 3-a of
-  // nr40 >> 2(3-p), so each window is one byte-select max.
+  // 5-mer t: stream bytes 2t+1 .. 2t+5, bits 4t+2 .. 4t+11 of c40
   const uint32_t clo = (uint32_t)c40, chi = (uint32_t)(c40 >> 32);
-  const uint32_t nlo = (uint32_t)nr40, nhi = (uint32_t)(nr40 >> 32);
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    const uint32_t sv = __builtin_amdgcn_alignbit(chi, clo, 2 * (p + 1));
-    const uint32_t uv = __builtin_amdgcn_alignbit(nhi, nlo, 2 * (3 - p));
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const uint32_t e = (sv >> (8 * a)) & 0xFFu, nf = (uv >> (8 * (3 - a))) & 0xFFu;
-      atomicAdd(&hist[e > nf ? e : nf], 1u);
-    }
+  for (int t = 0; t < 8; ++t) {
+    const uint32_t v = t == 0 ? clo : __builtin_amdgcn_alignbit(chi, clo, 4 * t);
+    atomicAdd(&h5[(v >> 2) & 0x3FFu], 1u);
   }
   const bool inside = first >= beg && first + 20 <= end;
   const bool full = inside && ((x[0] | x[1] | x[2] | x[3] | x[4]) & 0xDFDFDFDFu) == 0;
@@ -229,8 +225,14 @@ __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_
     vm &= ((1u << hi_c) - 1u) & ~((1u << lo_c) - 1u);
     // bit i: the 4-mer ending at chunk byte i (buffer bytes 16j+i-3 .. 16j+i) is valid
     km = ((vm & (vm >> 1) & (vm >> 2) & (vm >> 3)) >> 1) & 0xFFFFu;
-    for (uint32_t inv = ~km & 0xFFFFu; inv != 0; inv &= inv - 1u)
-      atomicSub(&hist[window_slot(c40, nr40, (uint32_t)__ffs(inv) - 1u)], 1u);
+    const uint32_t inv = ~km & 0xFFFFu;
+    // bit 2t: 5-mer t covers an invalid window
+    for (uint32_t pm = (inv | (inv >> 1)) & 0x5555u; pm != 0; pm &= pm - 1u) {
+      const uint32_t i = (uint32_t)__ffs(pm) - 1u;  // = 2t
+      atomicSub(&h5[(uint32_t)(c40 >> (2 * i + 2)) & 0x3FFu], 1u);
+      if ((km >> i) & 1u) atomicAdd(&h4[window_slot(c40, i)], 1u);
+      if ((km >> (i + 1)) & 1u) atomicAdd(&h4[window_slot(c40, i + 1)], 1u);
+    }
     valid = spread16((vm >> 4) & 0xFFFFu);
   }
   tl.kmers += __popc(km);
@@ -242,44 +244,93 @@ __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ hist, uint32_
   }
 }
 
+// Windows with value E (earliest base lowest) in a contig: its 4-mer bin
+// plus the 5-mers whose first four bases (E | x << 8) or last four
+// (x | E << 2) are E.
+__device__ __forceinline__ uint32_t window_count(const uint32_t* __restrict__ h5,
+                                                 const uint32_t* __restrict__ h4, uint32_t e) {
+  uint32_t c = h4[e];
+#pragma unroll
+  for (uint32_t x = 0; x < 4; ++x) c += h5[e | (x << 8)] + h5[x | (e << 2)];
+  return c;
+}
+
+// Plane words of a wave's 64 class bytes (one per lane) into lanes 0..15 of
+// val: plane P's ballot gives group g0's word (lanes 0-31) at lane
+// 8(P/4) + P%4 and group g0+1's (lanes 32-63) at that lane + 4.  Each lane
+// picks its word from the wave-uniform ballots with selects.
+template <int P>
+__device__ __forceinline__ void plane_lanes(uint32_t& val, uint32_t code, uint32_t ln) {
+  const uint64_t bits = __ballot((code >> P) & 1u);
+  const bool mine = (ln & 3u) == (uint32_t)(P & 3) && ((ln >> 3) & 1u) == (uint32_t)(P >> 2);
+  const uint32_t word = (ln & 4u) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
+  val = mine ? word : val;
+  if constexpr (P < 7) plane_lanes<P + 1>(val, code, ln);
+}
+
+// A 4-mer code with its four 2-bit groups in reverse order.
+__device__ __forceinline__ uint32_t rev4(uint32_t v) {
+  return ((v & 3u) << 6) | ((v & 0xCu) << 2) | ((v >> 2) & 0xCu) | ((v >> 6) & 3u);
+}
+
+// Ranks r0 .. r0+7 of register k (the last rank repeated past the end).
+__device__ __forceinline__ void load_rank_group(const double* __restrict__ sT, const uint8_t* __restrict__ sX,
+                                                uint32_t m, uint32_t k, int r0, double* t, uint32_t* x) {
+#pragma unroll
+  for (int g = 0; g < kRankGroup; ++g) {
+    const int r = min(r0 + g, kElemCount - 1);
+    t[g] = sT[(size_t)r * m + k];
+    x[g] = sX[(size_t)r * m + k];
+  }
+}
+
 template <int CPB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void sketch_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void sketch_kernel(
     const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
     uint32_t W, const double* __restrict__ sT, const uint8_t* __restrict__ sX,
     const uint8_t* __restrict__ cls, const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx,
     uint32_t mpad, uint32_t* __restrict__ planes, const uint32_t* __restrict__ out_row) {
   static_assert(CPB == kSketchWaves, "four contigs per workgroup");
-  __shared__ uint32_t hist[kSketchWaves][CPB][256];  // per wave, per contig
+  // Phase-1 bins per contig: 1024 5-mer bins, then 256 4-mer bins.  After
+  // the counts are read the same bytes hold winv (1/w per element).
+  constexpr uint32_t kBins = 1024 + 256;
+  __shared__ __align__(16) uint32_t hbuf[CPB * kBins];
+  double (*winv)[kElemCount] = reinterpret_cast<double (*)[kElemCount]>(hbuf);
+  static_assert(sizeof(double) * CPB * kElemCount <= sizeof(hbuf), "winv fits the bins");
   __shared__ uint32_t bases[CPB][4];
   __shared__ uint32_t total[CPB];
-  __shared__ double winv[CPB][kElemCount];
+  __shared__ double pbase[CPB][4];     // per contig: base probabilities
+  __shared__ double tinv[CPB];         // per contig: 1 / total
   __shared__ unsigned long long wmin_bits[CPB];  // positive doubles order as integers
+  __shared__ uint64_t soff[CPB + 1];   // contig q is [soff[q], soff[q+1])
   const uint32_t tid = threadIdx.x;
-  const uint32_t wv = tid >> 6, ln = tid & 63u;
+  const uint32_t ln = tid & 63u;
   const uint32_t c0 = blockIdx.x * CPB;
 
-  for (uint32_t v = tid; v < kSketchWaves * CPB * 256; v += 256) (&hist[0][0][0])[v] = 0;
+  for (uint32_t v = tid; v < CPB * kBins / 4; v += 256) reinterpret_cast<uint4*>(hbuf)[v] = make_uint4(0, 0, 0, 0);
   if (tid < CPB) {
     wmin_bits[tid] = 0x7FF0000000000000ull;  // +inf: no element present
     total[tid] = 0;
   }
   if (tid < 4 * CPB) (&bases[0][0])[tid] = 0;
+  if (tid <= CPB) soff[tid] = offs[min(c0 + tid, n)];  // contigs past n are empty
+  __syncthreads();
   // chunk ranges of the four contigs: contig q owns chunks cj[q] .. cj[q] +
   // nch - 1 of the buffer, numbered cpre[q] .. cpre[q+1] - 1 in the block
   __shared__ int64_t cbeg[CPB], cend[CPB], cj[CPB];
   __shared__ uint32_t cpre[CPB + 1];
-  if (tid == 0) {
+  if (tid <= CPB) {
     uint32_t acc = 0;
-    for (int q = 0; q < CPB; ++q) {
-      const uint32_t c = c0 + q;
-      const int64_t b = c < n ? (int64_t)offs[c] : 0, e = c < n ? (int64_t)offs[c + 1] : 0;
-      cbeg[q] = b;
-      cend[q] = e;
-      cj[q] = b >> 4;
-      cpre[q] = acc;
+    for (uint32_t q = 0; q < tid; ++q) {
+      const int64_t b = (int64_t)soff[q], e = (int64_t)soff[q + 1];
       acc += e > b ? (uint32_t)(((e - 1) >> 4) - (b >> 4) + 1) : 0u;
     }
-    cpre[CPB] = acc;
+    cpre[tid] = acc;
+    if (tid < CPB) {
+      cbeg[tid] = (int64_t)soff[tid];
+      cend[tid] = (int64_t)soff[tid + 1];
+      cj[tid] = (int64_t)soff[tid] >> 4;
+    }
   }
   __syncthreads();
   {
@@ -294,7 +345,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
       q = (g >= cpre[1]) + (g >= cpre[2]) + (g >= cpre[3]);
       j = cj[q] + (int64_t)(g - cpre[q]);
       cur = *reinterpret_cast<const uint4*>(seqs + 16 * j);
-      prev = j > 0 ? *reinterpret_cast<const uint32_t*>(seqs + 16 * j - 4) : 0u;
+      prev = *reinterpret_cast<const uint32_t*>(seqs + (j > 0 ? 16 * j - 4 : 0));
     }
     while (g < nall) {
       const uint32_t gn = g + 256;
@@ -306,7 +357,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
         qn = (gn >= cpre[1]) + (gn >= cpre[2]) + (gn >= cpre[3]);
         jn = cj[qn] + (int64_t)(gn - cpre[qn]);
         nxt = *reinterpret_cast<const uint4*>(seqs + 16 * jn);
-        nprev = jn > 0 ? *reinterpret_cast<const uint32_t*>(seqs + 16 * jn - 4) : 0u;
+        nprev = *reinterpret_cast<const uint32_t*>(seqs + (jn > 0 ? 16 * jn - 4 : 0));
       }
       if (q != tq) {  // the tally so far belongs to contig tq
         if (tl.kmers | tl.base[0] | tl.base[1] | tl.base[2] | tl.base[3]) {
@@ -317,7 +368,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
         tl = KmerTally();
         tq = q;
       }
-      chunk_kmers(hist[wv][q], prev, cur, 16 * j - 4, cbeg[q], cend[q], tl);
+      chunk_kmers(hbuf + q * 1024u, hbuf + CPB * 1024u + q * 256u, prev, cur, 16 * j - 4, cbeg[q], cend[q], tl);
       cur = nxt;
       prev = nprev;
       g = gn;
@@ -331,31 +382,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
     }
   }
   __syncthreads();
-  for (uint32_t e = tid; e < CPB * kElemCount; e += 256) {
+  if (tid < CPB) {  // per-contig factors, in the reference's operations
+    const uint32_t q = tid;
+    const uint64_t vb = (uint64_t)bases[q][0] + bases[q][1] + bases[q][2] + bases[q][3];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pbase[q][b] = vb > 0 ? (double)bases[q][b] / (double)vb : 0.25;
+    tinv[q] = total[q] != 0 ? 1.0 / (double)total[q] : 0.0;
+  }
+  // canonical counts of this thread's elements e = tid + 256r: code v's
+  // windows have E = rev(v) or E = 255 - v (one value when v is its own
+  // reverse complement)
+  constexpr int kEpt = (CPB * kElemCount + 255) / 256;
+  uint32_t ecnt[kEpt];
+#pragma unroll
+  for (int r = 0; r < kEpt; ++r) {
+    const uint32_t e = tid + 256u * r;
+    ecnt[r] = 0;
+    if (e < CPB * kElemCount) {
+      const uint32_t q = e / kElemCount, x = e % kElemCount;
+      const uint32_t v = codes[x];
+      const uint32_t sa = rev4(v), sb = 255u - v;
+      const uint32_t* h5 = hbuf + q * 1024u;
+      const uint32_t* h4 = hbuf + CPB * 1024u + q * 256u;
+      ecnt[r] = window_count(h5, h4, sa) + (sa != sb ? window_count(h5, h4, sb) : 0u);
+    }
+  }
+  __syncthreads();  // bins read: their bytes now take winv
+#pragma unroll
+  for (int r = 0; r < kEpt; ++r) {
+    const uint32_t e = tid + 256u * r;
+    if (e >= CPB * kElemCount) break;
     const uint32_t q = e / kElemCount, x = e % kElemCount;
     const uint32_t c = c0 + q;
-    double wi = __builtin_huge_val();  // absent element: fl(inf * T) = inf never wins
     const uint32_t v = codes[x];
-    uint32_t cnt = 0;
-#pragma unroll
-    for (int w = 0; w < kSketchWaves; ++w) cnt += hist[w][q][255u - v];  // reversed bins
-    const uint32_t tot = total[q];
-    if (c < n && cnt != 0 && tot != 0) {
-      const uint64_t vb = (uint64_t)bases[q][0] + bases[q][1] + bases[q][2] + bases[q][3];
-      double p[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) p[b] = vb > 0 ? (double)bases[q][b] / (double)vb : 0.25;
-      const double inv = 1.0 / (double)tot;
+    const uint32_t cnt = ecnt[r];
+    double wi = __builtin_huge_val();  // absent element: fl(inf * T) = inf never wins
+    if (c < n && cnt != 0 && total[q] != 0) {
       double pf = 1.0, pr = 1.0;
       uint32_t code = v;
 #pragma unroll
       for (int pos = 0; pos < 4; ++pos) {
         const uint32_t b = code & 3u;
-        pf *= p[b];
-        pr *= p[3u - b];
+        pf *= pbase[q][b];
+        pr *= pbase[q][3u - b];
         code >>= 2;
       }
-      const double obs = (double)cnt * inv;
+      const double obs = (double)cnt * tinv[q];
       const double ex = pf + pr;
       double w = 0.0;
       if (ex > 1e-14) {
@@ -370,8 +442,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
   }
   __syncthreads();
   double wmin[CPB];
+  uint32_t orow[CPB];  // output row of each contig
 #pragma unroll
-  for (int q = 0; q < CPB; ++q) wmin[q] = __longlong_as_double((long long)wmin_bits[q]);
+  for (int q = 0; q < CPB; ++q) {
+    // uniform: kept in scalar registers
+    const unsigned long long wb = wmin_bits[q];
+    const uint32_t wlo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)wb);
+    const uint32_t whi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(wb >> 32));
+    wmin[q] = __longlong_as_double((long long)(((uint64_t)whi << 32) | wlo));
+    const uint32_t c = c0 + q;
+    orow[q] = c < n ? (out_row ? out_row[c] : c) : 0u;
+  }
 
   const uint32_t kspan = ((32u * W + 255u) / 256u) * 256u;
   for (uint32_t k = tid; k < kspan; k += 256) {
@@ -383,18 +464,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
       bx[q] = -1;
     }
     if (k < m) {
-      // Ranks in groups of kRankGroup, loaded together: applying an element
-      // past the break point changes nothing (it can neither win nor tie), so
-      // the break is tested once per group, on the group's last rank.
+      // Ranks in groups of kRankGroup: applying an element past the break
+      // point changes nothing (it can neither win nor tie), so the break is
+      // tested once per group, on the group's last rank.
       for (int r0 = 0; r0 < kElemCount; r0 += kRankGroup) {
         double t[kRankGroup];
         uint32_t x[kRankGroup];
-#pragma unroll
-        for (int g = 0; g < kRankGroup; ++g) {
-          const int r = min(r0 + g, kElemCount - 1);  // past the last rank: repeat it (harmless)
-          t[g] = sT[(size_t)r * m + k];
-          x[g] = sX[(size_t)r * m + k];
-        }
+        load_rank_group(sT, sX, m, k, r0, t, x);
         // Branch-free update, so that the LDS reads of several ranks are in
         // flight together: an absent element has 1/w = inf, hence hv = inf,
         // which neither beats nor ties best (best = inf only while bx = -1,
@@ -419,21 +495,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
         if (done) break;
       }
     }
-    const uint32_t w0 = k / 32u;  // lanes 0..31 -> w0, lanes 32..63 -> w0+1
+    // class bytes of the four winners, loaded together
+    uint32_t code[CPB];
 #pragma unroll
     for (int q = 0; q < CPB; ++q) {
-      const uint32_t c = c0 + q;
-      const size_t orow = c < n ? (out_row ? out_row[c] : c) : 0;  // output row of contig c
-      uint32_t code = 0xFFu;
-      if (k < m && bx[q] >= 0) code = cls[(size_t)k * kElemCount + bx[q]];
-      if (c < n && wx != nullptr && k < mpad)
-        wx[orow * mpad + k] = (k < m && bx[q] >= 0) ? (uint8_t)bx[q] : (uint8_t)0xFF;
+      const bool has = k < m && bx[q] >= 0;
+      const uint32_t cv = cls[has ? (size_t)k * kElemCount + (uint32_t)bx[q] : 0];
+      code[q] = has ? cv : 0xFFu;
+    }
+    if (wx != nullptr && k < mpad) {
 #pragma unroll
-      for (int p = 0; p < 8; ++p) {
-        const uint64_t bits = __ballot((code >> p) & 1u);
-        if (c < n && (ln == 0 || ln == 32) && w0 < W)
-          planes[orow * 8u * W + plane_word(w0, p, W)] = ln == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
-      }
+      for (int q = 0; q < CPB; ++q)
+        if (c0 + q < n) wx[(size_t)orow[q] * mpad + k] = (k < m && bx[q] >= 0) ? (uint8_t)bx[q] : (uint8_t)0xFF;
+    }
+    // plane words of the wave's 64 registers: groups g0 (lanes 0-31) and g0+1
+    const uint32_t g0 = (k - ln) / 32u;
+    const uint32_t s = ln & 7u, h = (ln >> 3) & 1u;  // lane < 16 writes word (group g0 + s/4, plane 4h + s%4)
+#pragma unroll
+    for (int q = 0; q < CPB; ++q) {
+      uint32_t val = 0;
+      plane_lanes<0>(val, code[q], ln);
+      if (ln < 16 && c0 + q < n && g0 + (s >> 2) < W)
+        planes[(size_t)orow[q] * 8u * W + h * 4u * W + 4u * g0 + s] = val;
     }
   }
 }
