@@ -959,7 +959,7 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
     RBG_TRY(kbeg("gate_compare_kernel"));
     RBG_TRY_HIP(ctx, launch_gate_compare((uint64_t*)b->cand.p, b->cand_cap, ncand,
                                          (const float*)b->unit_pad.p, b->NC,
-                                         (const uint8_t*)b->anynz.p, desc, asm_of,
+                                         (const uint8_t*)b->anynz.p, desc, asm_of, A,
                                          (const uint32_t*)b->planes.p, W, (uint32_t*)b->deg.p,
                                          (uint32_t*)b->slot.p, st));
     RBG_TRY(kend());

@@ -108,10 +108,11 @@ hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, const AsmDesc* desc,
 // survivor t = (i, j) takes the next entry of row i's and of row j's list:
 // deg counts the entries per row and slot[2t], slot[2t+1] receive the
 // survivor's positions in row i's and row j's list (cap < 2^31).  unit rows
-// are zero-padded to 16*NC floats.
+// are zero-padded to 16*NC floats.  n_asm: assemblies in the build (with one,
+// its gate parameters are read once instead of per candidate).
 hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
                                const float* unit, uint32_t NC, const uint8_t* anynz,
-                               const AsmDesc* desc, const uint16_t* asm_of,
+                               const AsmDesc* desc, const uint16_t* asm_of, uint32_t n_asm,
                                const uint32_t* planes, uint32_t W, uint32_t* deg, uint32_t* slot,
                                hipStream_t st);
 // Incremental assemblies: a served session's candidate lists from its earlier

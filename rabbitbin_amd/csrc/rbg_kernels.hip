@@ -863,22 +863,37 @@ __device__ __forceinline__ void unpack_pair(uint64_t e, uint32_t& i, uint32_t& j
 // of both rows; lanes q and q + 4 compute the same values); the first
 // reduction level pairs lanes q and q^2, the second q and q^1, the last runs
 // in every lane (IEEE addition is commutative, so which lane holds which
-// operand does not change a sum).  For the register compare, lane q holds
-// word groups q, q + 8, ... of both rows' planes (32 bytes each: one group's
-// 8 planes, in two 16-byte halves), so each load instruction of the eight
-// lanes of a candidate reads one contiguous 128-byte line of a row (the
-// same half of eight groups).  All loads are issued before the gate is
-// known: nearly every candidate survives.
+// operand does not change a sum, and every lane of the eight ends with the
+// same dot).  The reductions move values between lanes with DPP quad
+// permutes (and a half-row mirror for the register counts), not LDS.  For
+// the register compare, lane q holds word groups q, q + 8, ... of both
+// rows' planes (32 bytes each: one group's 8 planes, in two 16-byte halves),
+// so each load instruction of the eight lanes of a candidate reads one
+// contiguous 128-byte line of a row (the same half of eight groups).  The
+// candidate e comes in from the previous step, and every load that depends
+// on it (plane rows, unit rows, anynz, and the assembly's gate parameters
+// when the build holds several assemblies) is issued before any is used, so
+// a step waits for one memory round trip: nearly every candidate survives.
 // Returns true on lane 0 of a survivor's eight, with its rows in ri, rj.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __uint_as_float(dpp_u32<CTRL>(__float_as_uint(v)));
+}
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;  // lane i of each 8 from lane 7 - i
+
 template <int NC, int W>
 __device__ __forceinline__ bool gate_compare_one(
-    uint64_t* __restrict__ cand, uint64_t t, uint64_t nc, uint32_t q,
+    uint64_t* __restrict__ cand, uint64_t t, uint64_t e, bool live, uint32_t q,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
-    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
-    const uint32_t* __restrict__ planes, uint32_t& ri, uint32_t& rj) {
+    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of, bool single, float thr0,
+    uint32_t acc1_end0, const uint32_t* __restrict__ planes, uint32_t& ri, uint32_t& rj) {
   constexpr int GPL = (W + 7) / 8;  // word groups per lane
-  const bool live = t < nc;
-  const uint64_t e = live ? cand[t] : 0;
   // Row indices are taken as 64-bit fields and masked in 64 bits: with the
   // 32-bit form hipcc (ROCm 7.2) dropped the 24-bit mask of j in the row
   // address of some instantiations while keeping it elsewhere.
@@ -887,9 +902,8 @@ __device__ __forceinline__ bool gate_compare_one(
   // proven by the prefilter (its dot reached qcut_hi): no FP32 gate needed
   const bool sure = live && ((e >> 15) & 1u);
   const uint32_t q4 = q & 3u;
-  const AsmDesc& ad = desc[live ? asm_of[i] : 0];
-  const float thr = ad.thr;
-  const uint32_t acc1_end = ad.acc1_end;
+  // The plane loads have no branch (a dead lane reads row 0's planes); the
+  // unit rows of a proven candidate are not read.
   float4 ua[NC], ub[NC];
   uint4 pa[2 * GPL], pb[2 * GPL];
 #pragma unroll
@@ -903,19 +917,25 @@ __device__ __forceinline__ bool gate_compare_one(
       ub[c] = rb[4 * c + q4];
     }
   }
-  if (live) {
 #pragma unroll
-    for (int x = 0; x < GPL; ++x) {
-      const uint32_t w = 8u * x + q;
-      if (w < (uint32_t)W) {
-        const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W + 4u * w);
-        const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W + 4u * w);
-        pa[2 * x] = wa[0];             // planes 0-3 of group w
-        pa[2 * x + 1] = wa[W];         // planes 4-7
-        pb[2 * x] = wb[0];
-        pb[2 * x + 1] = wb[W];
-      }
-    }
+  for (int x = 0; x < GPL; ++x) {
+    const uint32_t w = 8u * x + q;
+    const uint32_t wl = w < (uint32_t)W ? w : 0u;  // lanes past the last group re-read group 0 (unused)
+    const uint4* wa = reinterpret_cast<const uint4*>(planes + i64 * 8u * W + 4u * wl);
+    const uint4* wb = reinterpret_cast<const uint4*>(planes + j64 * 8u * W + 4u * wl);
+    pa[2 * x] = wa[0];             // planes 0-3 of group w
+    pa[2 * x + 1] = wa[W];         // planes 4-7
+    pb[2 * x] = wb[0];
+    pb[2 * x + 1] = wb[W];
+  }
+  // is_nz inputs (rows 0 stand in for a dead lane; its result is unused)
+  const uint32_t nzi = anynz ? anynz[i] : 1u, nzj = anynz ? anynz[j] : 1u;
+  float thr = thr0;
+  uint32_t acc1_end = acc1_end0;
+  if (!single) {
+    const AsmDesc& ad = desc[asm_of[i]];
+    thr = ad.thr;
+    acc1_end = ad.acc1_end;
   }
   // gate value
   float acc0[4] = {0.f, 0.f, 0.f, 0.f}, acc1[4] = {0.f, 0.f, 0.f, 0.f};
@@ -941,13 +961,12 @@ __device__ __forceinline__ bool gate_compare_one(
   }
   float t8[4], t4[4];
 #pragma unroll
-  for (int l = 0; l < 4; ++l) t8[l] = __shfl_xor(acc0[l], 2) + acc0[l];  // l+8 with l
+  for (int l = 0; l < 4; ++l) t8[l] = dpp_f32<kDppXor2>(acc0[l]) + acc0[l];  // l+8 with l
 #pragma unroll
-  for (int l = 0; l < 4; ++l) t4[l] = __shfl_xor(t8[l], 1) + t8[l];  // l+4 with l
+  for (int l = 0; l < 4; ++l) t4[l] = dpp_f32<kDppXor1>(t8[l]) + t8[l];  // l+4 with l
   const float dot = (t4[0] + t4[2]) + (t4[1] + t4[3]);
-  const bool nz = sure || !anynz || (live && (anynz[i] || anynz[j]));
-  bool pass = live && (sure || (nz && !(dot < thr)));
-  pass = __shfl(pass, (int)(lane_id() & ~7u));  // the group's lane 0 decides
+  const bool nz = sure || (nzi | nzj) != 0;
+  const bool pass = live && (sure || (nz && !(dot < thr)));
   // register compare
   uint32_t mism = 0;
   if (pass) {
@@ -963,9 +982,9 @@ __device__ __forceinline__ bool gate_compare_one(
       }
     }
   }
-  mism += __shfl_xor(mism, 1);
-  mism += __shfl_xor(mism, 2);
-  mism += __shfl_xor(mism, 4);
+  mism += dpp_u32<kDppXor1>(mism);
+  mism += dpp_u32<kDppXor2>(mism);        // every lane: its quad's sum
+  mism += dpp_u32<kDppHalfMirror>(mism);  // plus the other quad's
   if (live && q == 0) cand[t] = pass ? (e & ~0xFFFFull) | (uint64_t)(32u * W - mism) : kDeadPair;
   ri = i;
   rj = j;
@@ -1004,13 +1023,17 @@ template <int NC, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6 : 1))) void gate_compare_kernel(
     uint64_t* __restrict__ cand, uint64_t cap, const unsigned long long* __restrict__ ncand,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
-    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
+    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of, uint32_t n_asm,
     const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg, uint32_t* __restrict__ slot) {
   __shared__ uint2 dq[4][kDegQueue];
   const uint64_t nc = min((uint64_t)*ncand, cap);
   uint64_t s0, s1;
   xcd_segment(nc, s0, s1);
   const uint32_t q = threadIdx.x & 7u, ln = lane_id(), wv = threadIdx.x >> 6;
+  // one assembly: its gate parameters are uniform
+  const bool single = n_asm == 1;
+  const float thr0 = desc[0].thr;
+  const uint32_t acc1_end0 = desc[0].acc1_end;
   uint32_t dn = 0;  // queued entries (wave-uniform)
   auto flush = [&]() {
     __builtin_amdgcn_wave_barrier();  // entries written by other lanes
@@ -1026,17 +1049,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6
     __builtin_amdgcn_wave_barrier();  // read before the queue is refilled
     dn = 0;
   };
-  // wave-uniform loop over the segment: 8 candidates per wave, 32 per workgroup per step
-  for (uint64_t tw = s0 + (wv << 3); tw < s1; tw += 32) {
+  // wave-uniform loop over the segment: 8 candidates per wave, 32 per
+  // workgroup per step; each step's candidate is loaded one step ahead
+  // (loads without branches: past the segment the last candidate is read
+  // and not used)
+  uint64_t tw = s0 + (wv << 3);
+  if (tw >= s1) return;  // wave-uniform; nothing queued
+  uint64_t e_next = cand[min(tw + (ln >> 3), s1 - 1)];
+  for (; tw < s1; tw += 32) {
+    const uint64_t t = tw + (ln >> 3);
+    const uint64_t e = t < s1 ? e_next : 0;
+    e_next = cand[min(t + 32, s1 - 1)];
     uint32_t ri, rj;
-    const bool add = gate_compare_one<NC, W>(cand, tw + (ln >> 3), s1, q, unit, anynz, desc,
-                                             asm_of, planes, ri, rj);
+    const bool add = gate_compare_one<NC, W>(cand, t, e, t < s1, q, unit, anynz, desc, asm_of, single,
+                                             thr0, acc1_end0, planes, ri, rj);
     const uint64_t mask = __ballot(add);
     if (mask) {
       if (dn > kDegQueue - 16) flush();  // a step queues at most 16 entries
       if (add) {
         const uint32_t at = dn + 2u * lanes_below(mask);
-        const uint32_t t2 = 2u * (uint32_t)(tw + (ln >> 3));  // slot index of row i's entry
+        const uint32_t t2 = 2u * (uint32_t)t;  // slot index of row i's entry
         dq[wv][at] = make_uint2(ri, t2);
         dq[wv][at + 1] = make_uint2(rj, t2 + 1u);
       }
@@ -2476,7 +2508,7 @@ hipError_t launch_gate_mfma(const int8_t* q, uint32_t kpad, const AsmDesc* desc,
 
 hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long long* ncand,
                                const float* unit, uint32_t NC, const uint8_t* anynz,
-                               const AsmDesc* desc, const uint16_t* asm_of,
+                               const AsmDesc* desc, const uint16_t* asm_of, uint32_t n_asm,
                                const uint32_t* planes, uint32_t W, uint32_t* deg, uint32_t* slot,
                                hipStream_t st) {
   if (cap == 0) return hipSuccess;
@@ -2485,7 +2517,7 @@ hipError_t launch_gate_compare(uint64_t* cand, uint64_t cap, const unsigned long
   const dim3 grid((unsigned)std::max<uint64_t>(kXcds, std::min<uint64_t>((cap * 8 + 255) / 256, 65536) & ~7ull));
 #define RBG_GC(NCV, WV)                                                                            \
   hipLaunchKernelGGL((gate_compare_kernel<NCV, WV>), grid, dim3(256), 0, st, cand, cap, ncand, unit, \
-                     anynz, desc, asm_of, planes, deg, slot)
+                     anynz, desc, asm_of, n_asm, planes, deg, slot)
 #define RBG_GC_W(NCV)                   \
   switch (W) {                          \
     case 16: RBG_GC(NCV, 16); break;    \
