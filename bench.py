@@ -17,7 +17,7 @@ max-edges=200, min-edge-score 70).
 Prints one JSON line.  Every kernel of the step is timed with HIP events on
 the library's stream; `roofline` describes the kernel with the largest time
 and `kernels` every kernel above 10% of the step's kernel time, each against
-its bound (HBM bytes, int8 MFMA ops or VALU lane-ops per launch).
+its bound (HBM bytes or int8 MFMA ops per launch).
 cpu_baseline times the reference's CPU path (a performance port of its fused
 pass, with the oracle's other stages) on a prefix of the same workload on the
 host cores, three runs, median per phase.
@@ -36,8 +36,6 @@ AFFINITY = len(os.sched_getaffinity(0))
 
 PEAK_HBM_GBS = 8000.0
 PEAK_INT8_TOPS = 5000.0  # dense int8 MFMA, MI355X
-# 32-bit VALU lane-ops: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (no packed math)
-PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12
 
 
 def log(msg):
@@ -48,19 +46,18 @@ def kernel_work(name, w):
     """Algorithmic work of one launch of a kernel of the step, for the
     workload statistics w: (bound, amount, unit, peak).  Bytes are the HBM
     bytes the algorithm must move (each input row read once, outputs written
-    once); VALU and MFMA work follow the per-pair operation counts of the
-    pair stage (a register compare of 125 packed words x 6 ops plus the S-term
-    FP32 gate per compared pair; 2*S int8 ops per prefiltered pair)."""
+    once); the MFMA work is 2*S int8 ops per prefiltered pair.  The exact
+    gate's plane gathers (mostly L2 hits) are reported beside its entry."""
     n, S, W, K, NC = w["n"], w["S"], w["W"], w["K"], w["NC"]
     pairs, cand, gated, E = w["pairs"], w["candidates"], w["gated"], w["edges"]
     if name == "sketch_kernel":
         return "hbm", w["bytes"] + n * 32 * W, "GB/s", PEAK_HBM_GBS
     if name == "gate_mfma_kernel":
         return "mfma", 2.0 * S * pairs, "TOP/s", PEAK_INT8_TOPS
-    if name == "gate_compare_kernel":
-        return "valu", cand * (750.0 + S + 15), "TOP/s", PEAK_VALU_TOPS
-    if name == "dense_counts_kernel":
-        return "valu", pairs * 750.0, "TOP/s", PEAK_VALU_TOPS
+    if name == "gate_compare_kernel":  # candidates read and rewritten, survivors' slots, rows once
+        return "hbm", cand * 16.0 + gated * 8.0 + n * (32.0 * W + 64.0 * NC + 4.0), "GB/s", PEAK_HBM_GBS
+    if name == "dense_counts_kernel":  # rows once, the count matrix written
+        return "hbm", n * 32.0 * W + pairs * 4.0, "GB/s", PEAK_HBM_GBS
     if name == "scatter_kernel":  # survivor read + its two list entries
         return "hbm", gated * (16.0 + 16.0), "GB/s", PEAK_HBM_GBS
     if name == "select_topk_kernel":  # lists read, top-K written
@@ -80,8 +77,15 @@ def kernel_entry(name, ms, share, w, traffic):
         return {"kernel": name, "ms": ms, "share": share}
     bound, amount, unit, peak = kw
     achieved = amount / (ms / 1e3) / (1e9 if unit == "GB/s" else 1e12)
-    return {"kernel": name, "ms": ms, "share": share, "bound": bound, "achieved": achieved, "peak": peak,
-            "unit": unit, "frac": achieved / peak, "algorithmic": amount, "traffic": traffic}
+    e = {"kernel": name, "ms": ms, "share": share, "bound": bound, "achieved": achieved, "peak": peak,
+         "unit": unit, "frac": achieved / peak, "algorithmic": amount, "traffic": traffic}
+    if name == "gate_compare_kernel":
+        # what the kernel moves through the vector memory path (mostly L2 hits):
+        # both plane rows of every candidate
+        g = w["candidates"] * 2.0 * 32 * w["W"]
+        e["gathered_bytes"] = g
+        e["gathered_tb_s"] = g / (ms / 1e3) / 1e12
+    return e
 
 
 def profiled_traffic(workload, full_size, kernel):
