@@ -470,6 +470,43 @@ rbg_status rbg_label_propagation(const uint32_t* from, const uint32_t* to, const
                                  uint32_t n, uint64_t seed, uint32_t* membership, uint32_t* rounds_out);
 rbg_status rbg_lpa_node_order(uint32_t n, uint64_t seed, uint32_t* order);
 
+/* ---------------------------------------------------------------------------
+ * Frozen-core recruitment: recruit_to_frozen_cores (src/impl/rb_split.cpp:
+ * 729-919), the bins' recruitment of unbinned contigs by mean Spearman score.
+ * Bins come in any order as (bin_id[b], members bin_off[b] .. bin_off[b+1]),
+ * members in the bin's own order: member c < n_large is large contig c,
+ * n_large + s is small contig s.  The cores are the bins, by ascending id,
+ * whose length (the sum of all members' lengths) reaches core_floor and
+ * which hold at least max(2, min_cs) large members with a nonzero unit row;
+ * a core's centroid is the mean of those unit rows and its threshold their
+ * mean pairwise Spearman correlation in the reference's closed form
+ * (||sum u||^2 - sum ||u||^2) / (n(n-1)), clamped to [-1, 1].  Every large
+ * contig in no bin, then every small contig (its unit row from its raw
+ * depths, ranked as the reference ranks them), is scored against every core
+ * (u . centroid in FP64, summed in sample order) and goes to the one core
+ * whose threshold it reaches; with none or several it stays unassigned.
+ * assign_large (n_large) and assign_small (n_small) receive the core's bin
+ * id, or -1 (binned large contigs: -1).  Appending the assigned contigs to
+ * their bins in ascending order, large before small, gives the reference's
+ * bins.  unit_large: the large contigs' unit rows (n_large x S, as
+ * rbg_depth_features returns them); depth_small: the small contigs' raw
+ * depths (n_small x S).  The contig x core scores run on the device.
+ * Bin ids must be distinct and non-negative.  S < 2 or no core: nothing is
+ * assigned (as in the reference).  S > 64: RBG_ERR_UNSUPPORTED.  stats may be
+ * NULL.
+ * ------------------------------------------------------------------------- */
+typedef struct rbg_recruit_stats {
+  uint64_t cores;
+  uint64_t long_assigned, long_ambiguous, long_unmatched;
+  uint64_t short_assigned, short_ambiguous, short_unmatched;
+} rbg_recruit_stats;
+rbg_status rbg_recruit_frozen_cores(rbg_ctx* ctx, uint32_t S, uint32_t n_large, const float* unit_large,
+                                    const uint64_t* len_large, uint32_t n_small, const float* depth_small,
+                                    const uint64_t* len_small, uint32_t n_bins, const int32_t* bin_id,
+                                    const uint64_t* bin_off, const uint64_t* members, uint64_t core_floor,
+                                    uint64_t min_cs, int32_t* assign_large, int32_t* assign_small,
+                                    rbg_recruit_stats* stats);
+
 /* Chance baseline b0 over 16-bit winners (src/rabbitbin.cpp:1340-1369). */
 rbg_status rbg_pmh_baseline(rbg_ctx* ctx, const uint16_t* win16, uint32_t n, uint32_t m,
                             double* b0);

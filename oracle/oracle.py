@@ -257,6 +257,70 @@ class FaRec(C.Structure):
                 ("name_len", C.c_uint32), ("cls", C.c_uint32)]
 
 
+def _bins_csr(bins):
+    ids = np.array(list(bins.keys()), dtype=np.int32)
+    sizes = [len(v) for v in bins.values()]
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes, dtype=np.uint64)
+    mem = np.concatenate([np.asarray(v, np.uint64) for v in bins.values()]) if sizes else np.zeros(0, np.uint64)
+    return ids, off, np.ascontiguousarray(mem, np.uint64)
+
+
+def recruit_cores(S, unit_large, len_large, len_small, bins, core_floor, min_cs=10):
+    """The frozen cores: (ids int32, thresholds f64, centroids f64 [cores x S])."""
+    L = orc()
+    i32p = C.POINTER(C.c_int32)
+    L.orc_recruit_cores.restype = C.c_uint64
+    L.orc_recruit_cores.argtypes = [C.c_uint32, C.c_uint32, f32p, u64p, u64p, C.c_uint32, i32p, u64p, u64p,
+                                    C.c_uint64, C.c_uint64, C.c_uint64, i32p, f64p, f64p]
+    ul = np.ascontiguousarray(unit_large, np.float32).reshape(-1)
+    ll = np.ascontiguousarray(len_large, np.uint64)
+    ls = np.ascontiguousarray(len_small, np.uint64)
+    ids, off, mem = _bins_csr(bins)
+    cap = max(1, len(ids))
+    oid = np.zeros(cap, np.int32)
+    thr = np.zeros(cap, np.float64)
+    cent = np.zeros(cap * S, np.float64)
+    n = L.orc_recruit_cores(S, len(ll), _p(ul, f32p), _p(ll, u64p), _p(ls, u64p), len(ids), _p(ids, i32p),
+                            _p(off, u64p), _p(mem, u64p), core_floor, min_cs, cap, _p(oid, i32p), _p(thr, f64p),
+                            _p(cent, f64p))
+    return oid[:n], thr[:n], cent[:n * S].reshape(n, S)
+
+
+def recruit_scores(S, unit, cent):
+    """u . centroid of every (row, core) in the reference's order (n x cores)."""
+    L = orc()
+    L.orc_recruit_scores.argtypes = [C.c_uint32, C.c_uint64, f32p, C.c_uint64, f64p, f64p]
+    u = np.ascontiguousarray(unit, np.float32).reshape(-1, S)
+    c = np.ascontiguousarray(cent, np.float64).reshape(-1, S)
+    out = np.zeros((u.shape[0], c.shape[0]), np.float64)
+    L.orc_recruit_scores(S, u.shape[0], _p(u.reshape(-1), f32p), c.shape[0], _p(c.reshape(-1), f64p),
+                         _p(out.reshape(-1), f64p))
+    return out
+
+
+def recruit_frozen_cores(S, unit_large, len_large, depth_small, len_small, bins, core_floor, min_cs=10):
+    """(assign_large, assign_small, stats dict) of the reference's frozen-core recruitment."""
+    L = orc()
+    i32p = C.POINTER(C.c_int32)
+    L.orc_recruit_frozen_cores.argtypes = [C.c_uint32, C.c_uint32, f32p, u64p, C.c_uint32, f32p, u64p,
+                                           C.c_uint32, i32p, u64p, u64p, C.c_uint64, C.c_uint64, i32p, i32p, u64p]
+    ul = np.ascontiguousarray(unit_large, np.float32).reshape(-1)
+    ll = np.ascontiguousarray(len_large, np.uint64)
+    ds = np.ascontiguousarray(depth_small, np.float32).reshape(-1)
+    ls = np.ascontiguousarray(len_small, np.uint64)
+    ids, off, mem = _bins_csr(bins)
+    al = np.zeros(len(ll), np.int32)
+    as_ = np.zeros(len(ls), np.int32)
+    st = np.zeros(7, np.uint64)
+    L.orc_recruit_frozen_cores(S, len(ll), _p(ul, f32p), _p(ll, u64p), len(ls), _p(ds, f32p), _p(ls, u64p),
+                               len(ids), _p(ids, i32p), _p(off, u64p), _p(mem, u64p), core_floor, min_cs,
+                               _p(al, i32p), _p(as_, i32p), _p(st, u64p))
+    names = ("cores", "long_assigned", "long_ambiguous", "long_unmatched", "short_assigned", "short_ambiguous",
+             "short_unmatched")
+    return al, as_, {k: int(v) for k, v in zip(names, st)}
+
+
 def fasta_parse(data, min_contig=2500, min_small=1000):
     """The reference's FASTA parse of `data` (bytes): a list of records
     (name, class 'tiny' | 'small' | 'large', sequence bytes or None for tiny)

@@ -202,6 +202,10 @@ def lib():
     L.rbg_label_propagation.argtypes = [u32p, u32p, f32p, C.c_uint64, C.c_uint32, C.c_uint64, u32p,
                                         C.POINTER(C.c_uint32)]
     L.rbg_lpa_node_order.argtypes = [C.c_uint32, C.c_uint64, u32p]
+    i32p = C.POINTER(C.c_int32)
+    L.rbg_recruit_frozen_cores.argtypes = [vp, C.c_uint32, C.c_uint32, f32p, u64p, C.c_uint32, f32p, u64p,
+                                           C.c_uint32, i32p, u64p, u64p, C.c_uint64, C.c_uint64, i32p, i32p,
+                                           C.POINTER(RecruitStats)]
     L.rbg_env_semantics.argtypes = [C.c_char_p, C.c_size_t]
     L.rbg_env_semantics.restype = C.c_uint32
     L.rbg_batch_create.argtypes = [vp, C.POINTER(AssemblyIn), C.c_uint32, C.c_double, C.POINTER(vp)]
@@ -388,6 +392,33 @@ def _as_seqs(seqs, offsets):
     return _Seqs(seqs), off
 
 
+class RecruitStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("cores", "long_assigned", "long_ambiguous", "long_unmatched",
+                                         "short_assigned", "short_ambiguous", "short_unmatched")]
+
+
+def bins_csr(bins):
+    """(ids int32, offsets uint64, members uint64) of a {bin id: [members]}
+    mapping, bins in the mapping's order."""
+    ids = np.array(list(bins.keys()), dtype=np.int32)
+    sizes = [len(v) for v in bins.values()]
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes, dtype=np.uint64)
+    mem = np.concatenate([np.asarray(v, np.uint64) for v in bins.values()]) if sizes else np.zeros(0, np.uint64)
+    return ids, off, np.ascontiguousarray(mem, np.uint64)
+
+
+def apply_recruitment(bins, assign_large, assign_small, n_large):
+    """The bins after recruitment, as the reference appends: the assigned
+    large contigs in ascending order, then the small ones (n_large + s)."""
+    out = {k: list(v) for k, v in bins.items()}
+    for c in np.nonzero(np.asarray(assign_large) >= 0)[0]:
+        out[int(assign_large[c])].append(int(c))
+    for s in np.nonzero(np.asarray(assign_small) >= 0)[0]:
+        out[int(assign_small[s])].append(int(n_large + s))
+    return out
+
+
 class Context:
     """One HIP device + stream (rbg_open)."""
 
@@ -421,6 +452,26 @@ class Context:
         self._check(self._L.rbg_sketch_winners(self._h, buf, _p(off, u64p), n, m, seed,
                                                _p(w32, u32p), _p(w16, u16p)), "rbg_sketch_winners")
         return w32, w16
+
+    def recruit_frozen_cores(self, S, unit_large, len_large, depth_small, len_small, bins, core_floor,
+                             min_cs=10):
+        """rbg_recruit_frozen_cores: (assign_large int32, assign_small int32,
+        stats dict).  bins: {bin id: [members]} (large c, small n_large + s)."""
+        ul = np.ascontiguousarray(unit_large, np.float32).reshape(-1)
+        ll = np.ascontiguousarray(len_large, np.uint64)
+        ds = np.ascontiguousarray(depth_small, np.float32).reshape(-1)
+        ls = np.ascontiguousarray(len_small, np.uint64)
+        nl, ns = len(ll), len(ls)
+        ids, off, mem = bins_csr(bins)
+        al = np.zeros(nl, np.int32)
+        as_ = np.zeros(ns, np.int32)
+        stt = RecruitStats()
+        i32p = C.POINTER(C.c_int32)
+        self._check(self._L.rbg_recruit_frozen_cores(self._h, S, nl, _p(ul, f32p), _p(ll, u64p), ns, _p(ds, f32p),
+                                                     _p(ls, u64p), len(ids), _p(ids, i32p), _p(off, u64p),
+                                                     _p(mem, u64p), core_floor, min_cs, _p(al, i32p),
+                                                     _p(as_, i32p), C.byref(stt)), "rbg_recruit_frozen_cores")
+        return al, as_, {f: getattr(stt, f) for f, _ in RecruitStats._fields_}
 
     def pmh_baseline(self, win16):
         w = np.ascontiguousarray(win16, dtype=np.uint16)

@@ -52,6 +52,14 @@ struct AsmDesc {
 // per assembly.
 constexpr uint32_t kCalibRandDev = 2048;
 
+// ---- frozen-core recruitment: out_id[r] = the id of the one core whose
+// threshold unit row r's FP64 score reaches (sample order, fused multiply-
+// adds), -1 with none, several or an all-zero row; out_amb[r] = 1 with
+// several.  S in [1, 64].
+hipError_t launch_recruit_match(const float* unit, uint32_t n, uint32_t S, const double* cent, const double* thr,
+                                const int32_t* core_id, uint32_t ncores, int32_t* out_id, uint8_t* out_amb,
+                                hipStream_t st);
+
 // ---- stage (a)
 // Sketches of contigs [0, n) of (seqs, offs); contig c goes to row out_row[c]
 // of planes / wx (row c when out_row is null).

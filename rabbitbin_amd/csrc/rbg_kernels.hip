@@ -285,7 +285,7 @@ __device__ __forceinline__ void load_rank_group(const double* __restrict__ sT, c
 }
 
 template <int CPB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void sketch_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void sketch_kernel(
     const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
     uint32_t W, const double* __restrict__ sT, const uint8_t* __restrict__ sX,
     const uint8_t* __restrict__ cls, const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx,
