@@ -891,8 +891,37 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
     }
     return RBG_OK;
   };
+  // the same around a launch on the side stream
+  auto kside = [&](const char* name, auto&& launch) -> rbg_status {
+    const bool t = timed && ctx->nk < RBG_MAX_KERNEL_TIMES;
+    if (t) {
+      ctx->kname[ctx->nk] = name;
+      RBG_TRY_HIP(ctx, hipEventRecord(ctx->kev[2 * ctx->nk], ctx->side));
+    }
+    RBG_TRY_HIP(ctx, launch());
+    if (t) {
+      RBG_TRY_HIP(ctx, hipEventRecord(ctx->kev[2 * ctx->nk + 1], ctx->side));
+      ++ctx->nk;
+    }
+    return RBG_OK;
+  };
   RBG_TRY(mark(0));
   RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->counters.p, 8, 0u, st));
+  unsigned long long* ncand = (unsigned long long*)b->counters.p;
+  // The int8 prefilter of the pair pass reads only the int8 rows, the
+  // pair-pass order and the gate items, which the plan has uploaded: it runs
+  // on the side stream beside the sketch and b0 (they are bound by LDS and
+  // VALU, it by the MFMA tile cadence) and joins before the exact gate.
+  if (b->has_gate_items) {
+    RBG_TRY_HIP(ctx, hipEventRecord(ctx->fork_ev, st));
+    RBG_TRY_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+    RBG_TRY(kside("gate_mfma_kernel", [&] {
+      return launch_gate_mfma((const int8_t*)b->q8.p, b->kpad, desc, (const uint32_t*)b->perm.p,
+                              (const uint32_t*)b->gate_items.p, b->n_gate_items, (uint64_t*)b->cand.p, ncand,
+                              b->cand_cap, ctx->side);
+    }));
+    RBG_TRY_HIP(ctx, hipEventRecord(ctx->join_ev, ctx->side));
+  }
   const uint32_t mpad = 32 * W;
   if (want_wx) RBG_TRY_HIP(ctx, b->wx.ensure((size_t)N * mpad));
   // sketches: rows of resident sessions are copied, the others computed
@@ -939,14 +968,8 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                          (uint16_t*)b->mat.p, st));
     RBG_TRY(kend());
   }
-  unsigned long long* ncand = (unsigned long long*)b->counters.p;
-  if (b->has_gate_items) {
-    RBG_TRY(kbeg("gate_mfma_kernel"));
-    RBG_TRY_HIP(ctx, launch_gate_mfma((const int8_t*)b->q8.p, b->kpad, desc,
-                                      (const uint32_t*)b->perm.p, (const uint32_t*)b->gate_items.p,
-                                      b->n_gate_items, (uint64_t*)b->cand.p, ncand, b->cand_cap, st));
-    RBG_TRY(kend());
-  }
+  if (b->has_gate_items)
+    RBG_TRY_HIP(ctx, hipStreamWaitEvent(st, ctx->join_ev, 0));  // the candidates are in
   RBG_TRY(mark(6));
   RBG_TRY(mark(7));
   if (b->has_gate_items) {

@@ -22,6 +22,11 @@
 struct rbg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // A second stream (default priority) for work of a build that does not
+  // depend on the stages before it: the int8 prefilter runs there beside the
+  // sketch, between fork_ev (recorded on `stream`) and join_ev.
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
   struct Tables {
     uint32_t m;
