@@ -1009,73 +1009,132 @@ __device__ __forceinline__ void xcd_segment(uint64_t total, uint64_t& s0, uint64
 // wave of gathers in flight measured 5% faster.  Wider rows would spill.
 //
 // A survivor (i, j) at candidate t takes the next entry of row i's list and
-// of row j's: deg[i] and deg[j] count the entries, and the values the atomics
-// return are the survivor's slots, slot[2t] in row i and slot[2t+1] in row j
-// (the scatter then needs no atomics).  Those atomics are executed at the
-// memory side, and on this architecture a wave's later loads cannot be
-// waited for without waiting for its earlier atomics (one vector-memory
-// counter), so atomics issued every step would put a memory-side round trip
-// into every step's gathers.  Each wave queues its survivors' (row, slot
-// index) in LDS instead and claims them when the queue is nearly full and
-// once at its end, one round trip per 64 entries.
-constexpr uint32_t kDegQueue = 512;  // entries per wave
+// of row j's: deg[i] and deg[j] count the entries, and slot[2t], slot[2t+1]
+// receive the survivor's slots in row i and row j (the scatter then needs no
+// atomics).  The counters are claimed in two levels.  A workgroup counts its
+// survivors' entries per row in an LDS hash table (kClaimTable slots, open
+// addressing); an entry takes its rank among its row's entries from an LDS
+// atomic.  Every kClaimSteps steps, and at its end, the workgroup adds each
+// row's count to deg with one global atomic, whose return value is the base
+// of the row's ranks, and writes base + rank for its queued entries.  Device
+// atomics on this part are executed at the memory side, at a limited rate
+// (about 30 G/s): one atomic per entry cost the kernel 2.1 ms at 300k
+// contigs.  The rows of one prefilter tile share many partners, so the table
+// turns the per-entry atomics into far fewer per-row ones (63.6M into 9.3M
+// there).  The atomics of a round are all issued before the first is waited
+// for, since a wave waits for its loads and atomics through one counter.
+constexpr uint32_t kClaimTable = 2048;               // hash slots per workgroup
+constexpr uint32_t kClaimQueue = 1024;               // entries between two rounds
+constexpr uint32_t kClaimSteps = kClaimQueue / 64;  // a step queues at most 64 entries
+constexpr uint32_t kClaimEmpty = 0xFFFFFFFFu;
+
+// Slot of key in the table (inserted when absent) and the entry's rank among
+// the key's entries.  The table never holds more than kClaimQueue < kClaimTable
+// keys, so the probe ends.
+__device__ __forceinline__ uint32_t claim_insert(uint32_t* tkey, uint32_t* tval, uint32_t key,
+                                                 uint32_t& rank) {
+  uint32_t h = (key * 0x9E3779B1u) >> (32 - 11);  // 11 bits: kClaimTable slots
+  while (true) {
+    const uint32_t old = atomicCAS(&tkey[h], kClaimEmpty, key);
+    if (old == kClaimEmpty || old == key) break;
+    h = (h + 1u) & (kClaimTable - 1u);
+  }
+  rank = atomicAdd(&tval[h], 1u);
+  return h;
+}
+
 template <int NC, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 2 ? 6 : 1))) void gate_compare_kernel(
     uint64_t* __restrict__ cand, uint64_t cap, const unsigned long long* __restrict__ ncand,
     const float* __restrict__ unit, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of, uint32_t n_asm,
     const uint32_t* __restrict__ planes, uint32_t* __restrict__ deg, uint32_t* __restrict__ slot) {
-  __shared__ uint2 dq[4][kDegQueue];
+  static_assert(kClaimTable == 2048 && kClaimTable % 256 == 0, "hash width");
+  __shared__ uint32_t tkey[kClaimTable], tval[kClaimTable];
+  __shared__ uint2 queue[kClaimQueue];  // (table slot | rank << 16, slot index)
+  __shared__ uint32_t qn;
+  const uint32_t tid = threadIdx.x;
+#pragma unroll
+  for (uint32_t k = 0; k < kClaimTable / 256; ++k) {
+    tkey[tid + 256 * k] = kClaimEmpty;
+    tval[tid + 256 * k] = 0;
+  }
+  if (tid == 0) qn = 0;
+  __syncthreads();
   const uint64_t nc = min((uint64_t)*ncand, cap);
   uint64_t s0, s1;
   xcd_segment(nc, s0, s1);
-  const uint32_t q = threadIdx.x & 7u, ln = lane_id(), wv = threadIdx.x >> 6;
+  if (s0 >= s1) return;  // workgroup-uniform
+  const uint32_t q = threadIdx.x & 7u, ln = lane_id();
   // one assembly: its gate parameters are uniform
   const bool single = n_asm == 1;
   const float thr0 = desc[0].thr;
   const uint32_t acc1_end0 = desc[0].acc1_end;
-  uint32_t dn = 0;  // queued entries (wave-uniform)
-  auto flush = [&]() {
-    __builtin_amdgcn_wave_barrier();  // entries written by other lanes
-    for (uint32_t t = 0; t < dn; t += 64) {
-      const bool act = t + ln < dn;
-      const uint2 en = act ? dq[wv][t + ln] : make_uint2(0u, 0u);
-      const WaveSlot g = wave_group_budget<8>(en.x, act);
-      uint32_t base = 0;
-      if (g.count) base = atomicAdd(&deg[en.x], g.count);
-      const uint32_t sl = wave_slot_finish(base, g);
-      if (act) slot[en.y] = sl;
+  // one round of claims: a global atomic per counter in the table, all issued
+  // before the first is waited for, then base + rank for the queued entries
+  auto round = [&]() {
+    __syncthreads();  // entries inserted and queued
+    const uint32_t n = qn;
+    uint32_t base[kClaimTable / 256];
+#pragma unroll
+    for (uint32_t k = 0; k < kClaimTable / 256; ++k) {
+      const uint32_t key = tkey[tid + 256 * k];
+      base[k] = 0;
+      if (key != kClaimEmpty) base[k] = atomicAdd(&deg[key], tval[tid + 256 * k]);
     }
-    __builtin_amdgcn_wave_barrier();  // read before the queue is refilled
-    dn = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kClaimTable / 256; ++k) tval[tid + 256 * k] = base[k];
+    __syncthreads();
+    for (uint32_t x = tid; x < n; x += 256) {
+      const uint2 en = queue[x];
+      slot[en.y] = tval[en.x & 0xFFFFu] + (en.x >> 16);
+    }
+    __syncthreads();  // bases read before the table is cleared
+#pragma unroll
+    for (uint32_t k = 0; k < kClaimTable / 256; ++k) {
+      tkey[tid + 256 * k] = kClaimEmpty;
+      tval[tid + 256 * k] = 0;
+    }
+    if (tid == 0) qn = 0;
+    __syncthreads();
   };
-  // wave-uniform loop over the segment: 8 candidates per wave, 32 per
+  // workgroup-uniform loop over the segment: 8 candidates per wave, 32 per
   // workgroup per step; each step's candidate is loaded one step ahead
   // (loads without branches: past the segment the last candidate is read
   // and not used)
-  uint64_t tw = s0 + (wv << 3);
-  if (tw >= s1) return;  // wave-uniform; nothing queued
-  uint64_t e_next = cand[min(tw + (ln >> 3), s1 - 1)];
-  for (; tw < s1; tw += 32) {
-    const uint64_t t = tw + (ln >> 3);
+  uint64_t tb = s0;
+  uint64_t e_next = cand[min(tb + (tid >> 3), s1 - 1)];
+  uint32_t steps = 0;
+  for (; tb < s1; tb += 32) {
+    const uint64_t t = tb + (tid >> 3);
     const uint64_t e = t < s1 ? e_next : 0;
     e_next = cand[min(t + 32, s1 - 1)];
     uint32_t ri, rj;
     const bool add = gate_compare_one<NC, W>(cand, t, e, t < s1, q, unit, anynz, desc, asm_of, single,
                                              thr0, acc1_end0, planes, ri, rj);
+    uint32_t hi = 0, hj = 0, ki = 0, kj = 0;
+    if (add) {
+      hi = claim_insert(tkey, tval, ri, ki);
+      hj = claim_insert(tkey, tval, rj, kj);
+    }
     const uint64_t mask = __ballot(add);
-    if (mask) {
-      if (dn > kDegQueue - 16) flush();  // a step queues at most 16 entries
+    if (mask) {  // wave-uniform: one LDS atomic reserves the wave's queue entries
+      uint32_t at = 0;
+      if (ln == (uint32_t)(__ffsll((long long)mask) - 1))
+        at = atomicAdd(&qn, 2u * (uint32_t)__popcll(mask));
+      at = __shfl(at, __ffsll((long long)mask) - 1) + 2u * lanes_below(mask);
       if (add) {
-        const uint32_t at = dn + 2u * lanes_below(mask);
         const uint32_t t2 = 2u * (uint32_t)t;  // slot index of row i's entry
-        dq[wv][at] = make_uint2(ri, t2);
-        dq[wv][at + 1] = make_uint2(rj, t2 + 1u);
+        queue[at] = make_uint2(hi | ki << 16, t2);
+        queue[at + 1] = make_uint2(hj | kj << 16, t2 + 1u);
       }
-      dn += 2u * (uint32_t)__popcll(mask);
+    }
+    if (++steps == kClaimSteps) {
+      round();
+      steps = 0;
     }
   }
-  if (dn) flush();
+  round();
 }
 
 // Scatter each gated pair into both rows' candidate lists, entry =
