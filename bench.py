@@ -60,7 +60,7 @@ def kernel_work(name, w):
         return "hbm", n * 32.0 * W + pairs * 4.0, "GB/s", PEAK_HBM_GBS
     if name == "scatter_kernel":  # survivor read + its two list entries
         return "hbm", gated * (16.0 + 16.0), "GB/s", PEAK_HBM_GBS
-    if name == "select_topk_kernel":  # lists read, top-K written
+    if name == "select_wave_kernel":  # lists read, top-K written (with the queued long rows)
         return "hbm", 2.0 * gated * 8 + n * K * 6.0 + n * 16.0, "GB/s", PEAK_HBM_GBS
     if name == "emit_mark_kernel":  # top-K lists read, keep bits written
         return "hbm", n * K * 6.0 + n * 8.0 * ((K + 63) // 64), "GB/s", PEAK_HBM_GBS

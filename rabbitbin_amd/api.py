@@ -387,6 +387,26 @@ class _Seqs:
         return self.ptr
 
 
+class _ResultOwner:
+    """A graph result's host arrays, freed (rbg_graph_result_free) once no
+    array built on them remains: the edge arrays are handed out without a
+    copy."""
+
+    def __init__(self, L, res):
+        self._L, self._res = L, res
+
+    def __del__(self):
+        self._L.rbg_graph_result_free(C.byref(self._res))
+
+
+def _owned_array(owner, ptr, count):
+    """numpy view of count elements at the ctypes pointer ptr that keeps owner
+    alive."""
+    buf = (ptr._type_ * count).from_address(C.cast(ptr, C.c_void_p).value)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=np.dtype(ptr._type_))
+
+
 def _as_seqs(seqs, offsets):
     off = np.ascontiguousarray(offsets, dtype=np.uint64)
     return _Seqs(seqs), off
@@ -483,16 +503,17 @@ class Context:
     def _result(self, res, n, K, diag_arrays):
         ne = res.n_edges
         host = bool(res.from_)
+        owner = _ResultOwner(self._L, res)
 
         def arr(ptr, dt):
-            return np.ctypeslib.as_array(ptr, (ne,)).copy() if (ne and ptr) else np.zeros(0, dt)
+            return _owned_array(owner, ptr, ne) if (ne and ptr) else np.zeros(0, dt)
 
         out = {
             "n_edges": ne,
             "from": arr(res.from_, np.uint32) if host else None,
             "to": arr(res.to, np.uint32) if host else None,
             "scomp": arr(res.scomp, np.float32) if host else None,
-            "score": (np.ctypeslib.as_array(res.score, (ne,)).copy() if (ne and res.score) else None),
+            "score": _owned_array(owner, res.score, ne) if (ne and res.score) else None,
             "sim_cutoff": res.sim_cutoff, "b0": res.b0, "regime": res.regime,
             "gated_pairs": res.gated_pairs,
             "candidates": res.candidates,
@@ -507,8 +528,7 @@ class Context:
         }
         if diag_arrays is not None:
             out.update(diag_arrays)
-        self._L.rbg_graph_result_free(C.byref(res))
-        return out
+        return out  # the library's arrays are freed with the last array built on them
 
     def _diag(self, n, K, m, want):
         if not want:

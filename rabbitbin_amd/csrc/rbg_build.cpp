@@ -346,7 +346,7 @@ size_t rbg_asm::device_bytes() const {
   const DevBuf* all[] = {&seqs, &offs, &sk_rows, &asm_of, &unit_pad, &raw_pad, &colnorm, &anynz,
                          &q8, &perm, &pi, &pj, &pcnt, &merge, &desc, &bscal, &svd, &svf, &level,
                          &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
-                         &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt,
+                         &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt, &selq,
                          &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
                          &ecnt, &escomp, &escore, &scan_tmp, &bounds, &near_e, &near_w, &cls_in};
   size_t s = 0;
@@ -359,7 +359,7 @@ void rbg_asm::release_buffers() {
   DevBuf* all[] = {&seqs, &offs, &sk_rows, &asm_of, &unit_pad, &raw_pad, &colnorm, &anynz,
                    &q8, &perm, &pi, &pj, &pcnt, &merge, &desc, &bscal, &svd, &svf, &level,
                    &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
-                   &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt,
+                   &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt, &selq,
                    &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
                    &ecnt, &escomp, &escore, &scan_tmp, &bounds, &near_e, &near_w, &cls_in};
   for (DevBuf* d : all) d->release();
@@ -465,8 +465,23 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
   // Host preparation of each assembly's new rows (depth features, column
   // norms, pair-pass order, int8 rows, b0 pair draws), assemblies in parallel;
   // then the copies, in order.
+  // Host rows: 2 MiB aligned with transparent huge pages requested
+  // (alloc_host_result), first touched by the threads that fill them.
+  struct HostRows {
+    float* p = nullptr;
+    size_t count = 0;
+    HostRows() = default;
+    HostRows(const HostRows&) = delete;
+    HostRows& operator=(const HostRows&) = delete;
+    ~HostRows() { std::free(p); }
+    void alloc(size_t c) {
+      p = (float*)alloc_host_result(c * sizeof(float));
+      if (!p && c) throw std::bad_alloc();
+      count = c;
+    }
+  };
   struct Prep {
-    std::vector<float> upad, rpad;  // new rows, padded to 16 * NC
+    HostRows upad, rpad;            // new rows, padded to 16 * NC
     std::vector<int8_t> q8;         // new rows in pair-pass order, kpad bytes each
     std::vector<uint32_t> ai, aj;   // b0 pair draws (local rows)
   };
@@ -478,39 +493,40 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     const Resident* src = h.src;
     const SketchInput* sk = in[a].sk;
     const uint32_t n_old = h.n_old, n_new = x.n, S = x.S;
-    // depth features of the new rows (row-local), colnorm from running column
-    // sums (the same additions in the same order as over all rows at once)
+    const size_t P = 16 * NC;  // padded row
+    // depth features of the new rows (row-local) written straight into the
+    // padded rows, colnorm from running column sums (the same additions in
+    // the same order as over all rows at once)
     h.anynz_local.assign(h.n, 1);
     if (src) std::copy(src->anynz.begin(), src->anynz.end(), h.anynz_local.begin());
-    std::vector<float> unit, raw;
+    const bool rows = S > 1 && n_new;
+    if (rows) {
+      pr.upad.alloc((size_t)n_new * P);
+      pr.rpad.alloc((size_t)n_new * P);
+    }
     if (sk) {  // the caller's rows (g_depth_unit, g_depth_raw, g_depth_colnorm, g_anynz)
       if (sk->anynz) std::copy(sk->anynz, sk->anynz + n_new, h.anynz_local.begin());
       if (S > 1) {
-        unit.assign(sk->unit, sk->unit + (size_t)n_new * S);
-        raw.assign(sk->raw, sk->raw + (size_t)n_new * S);
+#pragma omp parallel for schedule(static) if (parallel)
+        for (uint32_t r = 0; r < n_new; ++r) {
+          float* u = pr.upad.p + (size_t)r * P;
+          float* w = pr.rpad.p + (size_t)r * P;
+          std::copy(sk->unit + (size_t)r * S, sk->unit + (size_t)(r + 1) * S, u);
+          std::copy(sk->raw + (size_t)r * S, sk->raw + (size_t)(r + 1) * S, w);
+          std::fill(u + S, u + P, 0.0f);
+          std::fill(w + S, w + P, 0.0f);
+        }
         for (uint32_t c = 0; c < S; ++c) colnorm[h.colnorm0 + c] = sk->colnorm ? sk->colnorm[c] : 1.0;
       }
     } else if (S > 0) {
-      unit.assign((size_t)n_new * S, 0.f);
-      raw.assign((size_t)n_new * S, 0.f);
-      depth_features(x.depth, n_new, S, min_cv, nullptr, unit.data(), raw.data(), nullptr,
-                     h.anynz_local.data() + n_old, parallel);
+      depth_features(x.depth, n_new, S, min_cv, nullptr, rows ? pr.upad.p : nullptr, rows ? pr.rpad.p : nullptr,
+                     nullptr, h.anynz_local.data() + n_old, parallel, P);
       h.colsum.assign(S, 0.0);
       if (src) h.colsum = src->colsum;
+      column_sums(x.depth, n_new, S, h.colsum.data());
       for (uint32_t c = 0; c < S; ++c) {
-        double sum = h.colsum[c];
-        for (uint32_t r = 0; r < n_new; ++r) sum += x.depth[(size_t)r * S + c];
-        h.colsum[c] = sum;
-        const double mean = sum / (double)std::max<uint32_t>(h.n, 1);
+        const double mean = h.colsum[c] / (double)std::max<uint32_t>(h.n, 1);
         colnorm[h.colnorm0 + c] = S > 1 ? ((mean > 1e-12) ? 1.0 / mean : 0.0) : 0.0;
-      }
-    }
-    if (S > 1 && n_new) {
-      pr.upad.assign((size_t)n_new * 16 * NC, 0.f);
-      pr.rpad.assign((size_t)n_new * 16 * NC, 0.f);
-      for (uint32_t r = 0; r < n_new; ++r) {
-        std::memcpy(&pr.upad[(size_t)r * 16 * NC], &unit[(size_t)r * S], S * 4);
-        std::memcpy(&pr.rpad[(size_t)r * 16 * NC], &raw[(size_t)r * S], S * 4);
       }
     }
     // pair-pass order and int8 rows.  The pass visits the assembly's rows in
@@ -518,23 +534,24 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     // order and appends the new rows in their own order.
     h.perm_local.resize(h.n);
     if (src) std::copy(src->perm.begin(), src->perm.end(), h.perm_local.begin());
-    if (S > 1 && n_new) {
+    if (rows) {
       std::vector<uint32_t> lp;
-      pair_pass_order(unit.data(), n_new, S, lp);
-      std::vector<float> sorted((size_t)n_new * S);
-      for (uint32_t r = 0; r < n_new; ++r) {
-        std::memcpy(&sorted[(size_t)r * S], &unit[(size_t)lp[r] * S], S * 4);
-        h.perm_local[n_old + r] = n_old + lp[r];
-      }
+      pair_pass_order(pr.upad.p, n_new, S, lp, parallel, P);
+      for (uint32_t r = 0; r < n_new; ++r) h.perm_local[n_old + r] = n_old + lp[r];
       if (src) {
         h.gq = src->gq;
-        gate_quantize_more(sorted.data(), n_new, S, h.gq);
+        gate_quantize_more(pr.upad.p, n_new, S, h.gq, parallel, lp.data(), P);
       } else {
-        gate_quantize(sorted.data(), n_new, S, h.gq);
+        gate_quantize(pr.upad.p, n_new, S, h.gq, parallel, lp.data(), P);
       }
-      pr.q8.assign((size_t)n_new * kpad, 0);
-      for (uint32_t r = 0; r < n_new; ++r)
-        std::memcpy(&pr.q8[(size_t)r * kpad], &h.gq.q[(size_t)r * h.gq.kpad], h.gq.kpad);
+      if (h.gq.kpad == kpad) {
+        pr.q8.swap(h.gq.q);
+      } else {
+        pr.q8.assign((size_t)n_new * kpad, 0);
+#pragma omp parallel for schedule(static) if (parallel)
+        for (uint32_t r = 0; r < n_new; ++r)
+          std::memcpy(&pr.q8[(size_t)r * kpad], &h.gq.q[(size_t)r * h.gq.kpad], h.gq.kpad);
+      }
       std::vector<int8_t>().swap(h.gq.q);
     } else {
       for (uint32_t r = 0; r < n_new; ++r) h.perm_local[n_old + r] = n_old + r;
@@ -542,35 +559,50 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     }
     if (!(sk && sk->b0 == sk->b0)) baseline_pairs(h.n, pr.ai, pr.aj);  // b0 pair draws (they depend on n only)
   };
-  if (count == 1) {
-    prepare(0, true);
-  } else {
-    std::atomic<uint32_t> next{0};
-    auto work = [&] {
-      for (uint32_t a; (a = next.fetch_add(1)) < count;) prepare(a, false);
-    };
-    const uint32_t nt = std::min<uint32_t>(count, std::max(1u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (uint32_t t = 1; t < nt; ++t) pool.emplace_back(work);
-    work();
-    for (auto& th : pool) th.join();
-  }
-  // every assembly's new contigs in one pass through the staging
-  {
+  // Every assembly's new contigs in one pass through the staging, on a
+  // thread of its own while this one prepares the rows (the preparation makes
+  // no HIP calls).
+  rbg_status seq_status = RBG_OK;
+  std::thread seq_upload([&] {
     std::vector<H2DPart> seq_parts;
     uint64_t at = 0;
     for (uint32_t a = 0; a < count; ++a) {
       const rbg_assembly_in& x = in[a].in;
       const uint64_t nb = (x.n && !in[a].sk) ? x.offsets[x.n] - x.offsets[0] : 0;
-      if (nb && in[a].seqs_on_device)
-        RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->seqs.p + at, x.seqs + x.offsets[0], nb, hipMemcpyDeviceToDevice,
-                                        st));
-      else if (nb)
+      if (nb && in[a].seqs_on_device) {
+        const hipError_t e = hipMemcpyAsync((char*)b->seqs.p + at, x.seqs + x.offsets[0], nb,
+                                            hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) {
+          seq_status = ctx->fail(RBG_ERR_HIP, std::string("contig copy: ") + hipGetErrorString(e));
+          return;
+        }
+      } else if (nb) {
         seq_parts.push_back(H2DPart{(char*)b->seqs.p + at, x.seqs + x.offsets[0], nb});
+      }
       at += nb;
     }
-    if (!seq_parts.empty()) RBG_TRY(h2d_parts(ctx, seq_parts.data(), seq_parts.size()));
+    if (!seq_parts.empty()) seq_status = h2d_parts(ctx, seq_parts.data(), seq_parts.size());
+  });
+  try {
+    if (count == 1) {
+      prepare(0, true);
+    } else {
+      std::atomic<uint32_t> next{0};
+      auto work = [&] {
+        for (uint32_t a; (a = next.fetch_add(1)) < count;) prepare(a, false);
+      };
+      const uint32_t nt = std::min<uint32_t>(count, std::max(1u, std::thread::hardware_concurrency()));
+      std::vector<std::thread> pool;
+      for (uint32_t t = 1; t < nt; ++t) pool.emplace_back(work);
+      work();
+      for (auto& th : pool) th.join();
+    }
+  } catch (...) {
+    seq_upload.join();  // never leave the thread running
+    throw;
   }
+  seq_upload.join();
+  RBG_TRY(seq_status);
   uint64_t seq0 = 0, sk = 0;
   for (uint32_t a = 0; a < count; ++a) {
     const rbg_assembly_in& x = in[a].in;
@@ -597,11 +629,11 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     for (uint32_t r = 0; r < h.n; ++r) asm_of[h.row0 + r] = (uint16_t)a;
     std::copy(h.anynz_local.begin(), h.anynz_local.end(), anynz.begin() + h.row0);
     h.n_zero = (uint32_t)std::count(h.anynz_local.begin(), h.anynz_local.end(), (uint8_t)0);
-    if (!pr.upad.empty()) {
+    if (pr.upad.count) {
       const size_t o = (size_t)(h.row0 + n_old) * 16 * NC * 4;
-      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->unit_pad.p + o, pr.upad.data(), pr.upad.size() * 4,
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->unit_pad.p + o, pr.upad.p, pr.upad.count * 4,
                                       hipMemcpyHostToDevice, st));
-      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->raw_pad.p + o, pr.rpad.data(), pr.rpad.size() * 4,
+      RBG_TRY_HIP(ctx, hipMemcpyAsync((char*)b->raw_pad.p + o, pr.rpad.p, pr.rpad.count * 4,
                                       hipMemcpyHostToDevice, st));
     }
     if (!pr.q8.empty())
@@ -801,6 +833,7 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   RBG_TRY_HIP(ctx, b->topk_id.ensure(((size_t)N * K + 16) * 4));
   RBG_TRY_HIP(ctx, b->topk_cnt.ensure((size_t)N * K * 2));
   RBG_TRY_HIP(ctx, b->topk_len.ensure((size_t)N * 4));
+  RBG_TRY_HIP(ctx, b->selq.ensure((size_t)(N + 1) * 4));  // rows for the workgroup selection + count
   RBG_TRY_HIP(ctx, b->rowmax.ensure((size_t)N * 4));
   RBG_TRY_HIP(ctx, b->fence.ensure((size_t)N * FW * 4));
   RBG_TRY_HIP(ctx, b->keep.ensure((size_t)N * KW * 8));
@@ -852,7 +885,7 @@ size_t estimate_build_bytes(const AsmInput* in, uint32_t count, const rbg_graph_
   s += N * (2 + 1 + 2 * 16 * NC * 4 + kpad + 4);            // asm_of, anynz, unit/raw, q8, perm
   s += N * 8 * W * 4;                                       // planes
   s += mat * 2 + cap * 16 + (2 * cap + extra) * 8;          // mat, cand + slots, list
-  s += N * (4 + 8 + (uint64_t)K * 6 + 4 + 4 + FW * 4 + KW * 8 + 4 + 8);  // per-row arrays
+  s += N * (4 + 8 + 4 + (uint64_t)K * 6 + 4 + 4 + FW * 4 + KW * 8 + 4 + 8);  // per-row arrays
   s += N * (uint64_t)K * (4 + 4 + 2 + 4 + 4);               // edges
   s += (uint64_t)count * (64 * 4 + (m + 1) * 14 + kCalibRandDev * 4 + 30000 * 12);
   return (size_t)s + (64u << 20);                           // small tables and slack
@@ -1003,12 +1036,13 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
   }
   RBG_TRY(mark(3));
   RBG_TRY(mark(9));
-  RBG_TRY(kbeg("select_topk_kernel"));
+  RBG_TRY(kbeg("select_wave_kernel"));  // with the workgroup-per-row pass over the rows it queues
   RBG_TRY_HIP(ctx, launch_select((const uint64_t*)b->list.p, (const uint64_t*)b->off.p,
                                  (const uint16_t*)b->mat.p, (const uint8_t*)b->anynz.p, desc, asm_of,
                                  N, (const uint16_t*)b->level.p, m, K, (uint32_t*)b->topk_id.p,
                                  (uint16_t*)b->topk_cnt.p, (uint32_t*)b->topk_len.p,
-                                 (uint32_t*)b->rowmax.p, (uint32_t*)b->fence.p, FW, st));
+                                 (uint32_t*)b->rowmax.p, (uint32_t*)b->fence.p, FW,
+                                 (uint32_t*)b->selq.p, st));
   RBG_TRY(kend());
   RBG_TRY(mark(10));
   // calibration (the glibc rand() draws are prepared per (n, seed))

@@ -214,7 +214,7 @@ struct rbg_asm {
   uint64_t mat_total = 0, cand_cap = 0, edge_cap = 0;
   rbg_schedule sched{16, 64};
   rbg::DevBuf desc, bscal, svd, svf, level, cal_rnd, cal_members, cal_roff, dense_items, gate_items;
-  rbg::DevBuf wx, planes, mat, cand, counters, deg, slot, off, list, topk_id, topk_cnt, topk_len,
+  rbg::DevBuf wx, planes, mat, cand, counters, deg, slot, off, list, topk_id, topk_cnt, topk_len, selq,
       rowmax, fence, keep, row_edges, row_off, efrom, eto, ecnt, escomp, escore, scan_tmp, bounds,
       near_e, near_w;
   uint64_t near_cap = 0;  // near-cut list entries (edge index, FP64 weight)

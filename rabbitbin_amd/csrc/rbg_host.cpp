@@ -247,79 +247,105 @@ void sv_tables(uint32_t m, double b0, std::vector<double>& svd, std::vector<floa
   }
 }
 
+void column_sums(const float* depth, size_t n, size_t S, double* sum) {
+  // each column's sum runs over the rows in order (the reference's loop);
+  // blocks of rows keep the rows in cache while every column passes them
+  constexpr size_t kBlock = 4096;
+  for (size_t r0 = 0; r0 < n; r0 += kBlock) {
+    const size_t r1 = std::min(n, r0 + kBlock);
+    for (size_t c = 0; c < S; ++c) {
+      double v = sum[c];
+      for (size_t r = r0; r < r1; ++r) v += depth[r * S + c];
+      sum[c] = v;
+    }
+  }
+}
+
 void depth_features(const float* depth, size_t n, size_t S, double min_cv, float* ranked,
-                    float* unit, float* raw, double* colnorm, uint8_t* anynz, bool parallel) {
-  std::vector<float> rk(depth, depth + n * S);
-  if (S > 1) {
-    if (raw) std::memcpy(raw, depth, n * S * sizeof(float));
-    if (colnorm)
-      for (size_t c = 0; c < S; ++c) {
-        double sum = 0.0;
-        for (size_t r = 0; r < n; ++r) sum += depth[r * S + c];
-        const double mean = sum / (double)std::max<size_t>(n, 1);
-        colnorm[c] = (mean > 1e-12) ? 1.0 / mean : 0.0;
-      }
-    // average ranks, ties share the mean position (src/ranker.h:55-66)
-#pragma omp parallel for schedule(static) if (parallel)
-    for (size_t r = 0; r < n; ++r) {
-      float* row = &rk[r * S];
-      std::vector<uint32_t> order(S);
-      std::iota(order.begin(), order.end(), 0u);
-      std::sort(order.begin(), order.end(),
-                [row](uint32_t a, uint32_t b) { return row[a] < row[b]; });
-      std::vector<float> rank(S);
-      size_t c = 0;
-      while (c < S) {
-        size_t tie = 1;
-        while (c + tie < S && row[order[c]] == row[order[c + tie]]) ++tie;
-        const float v = (float)(2 * c + tie - 1) / 2 + 1;
-        for (size_t q = 0; q < tie; ++q) rank[order[c + q]] = v;
-        c += tie;
-      }
-      std::copy(rank.begin(), rank.end(), row);
+                    float* unit, float* raw, double* colnorm, uint8_t* anynz, bool parallel,
+                    size_t stride) {
+  if (stride == 0) stride = S;
+  if (S > 1 && colnorm) {
+    std::vector<double> sum(S, 0.0);
+    column_sums(depth, n, S, sum.data());
+    for (size_t c = 0; c < S; ++c) {
+      const double mean = sum[c] / (double)std::max<size_t>(n, 1);
+      colnorm[c] = (mean > 1e-12) ? 1.0 / mean : 0.0;
     }
   }
-  if (ranked) std::memcpy(ranked, rk.data(), n * S * sizeof(float));
-  if (unit) {
-    std::fill(unit, unit + n * S, 0.0f);
-    if (S > 1) {
-#pragma omp parallel for schedule(static) if (parallel)
-      for (size_t r = 0; r < n; ++r) {
-        const float* row = &rk[r * S];
-        double mean = 0.0;
-        for (size_t k = 0; k < S; ++k) mean += row[k];
-        mean /= (double)S;
-        double ss = 0.0;
-        for (size_t k = 0; k < S; ++k) {
-          const double d = (double)row[k] - mean;
-          // exact with or without fusion: ranks make d a multiple of 1/2
-          ss = std::fma(d, d, ss);
+  // Row by row: the raw row, average ranks (ties share the mean position,
+  // src/ranker.h:55-66), the centred unit row, is_nz.  Every output element
+  // (padding included) is written by the thread that owns the row.
+#pragma omp parallel if (parallel)
+  {
+    std::vector<uint32_t> order(S);
+    std::vector<float> rk(S);
+#pragma omp for schedule(static)
+    for (size_t r = 0; r < n; ++r) {
+      const float* row = depth + r * S;
+      if (S > 1 && raw) {
+        std::copy(row, row + S, raw + r * stride);
+        std::fill(raw + r * stride + S, raw + (r + 1) * stride, 0.0f);
+      }
+      if (S > 1) {
+        std::iota(order.begin(), order.end(), 0u);
+        std::sort(order.begin(), order.end(), [row](uint32_t a, uint32_t b) { return row[a] < row[b]; });
+        size_t c = 0;
+        while (c < S) {
+          size_t tie = 1;
+          while (c + tie < S && row[order[c]] == row[order[c + tie]]) ++tie;
+          const float v = (float)(2 * c + tie - 1) / 2 + 1;
+          for (size_t q = 0; q < tie; ++q) rk[order[c + q]] = v;
+          c += tie;
         }
-        if (ss > 0.0) {
-          const double inv = 1.0 / std::sqrt(ss);
-          for (size_t k = 0; k < S; ++k) unit[r * S + k] = (float)(((double)row[k] - mean) * inv);
+      } else {
+        std::copy(row, row + S, rk.begin());
+      }
+      if (ranked) std::copy(rk.begin(), rk.end(), ranked + r * S);
+      if (unit) {
+        float* u = unit + r * stride;
+        std::fill(u, u + stride, 0.0f);
+        if (S > 1) {
+          double mean = 0.0;
+          for (size_t k = 0; k < S; ++k) mean += rk[k];
+          mean /= (double)S;
+          double ss = 0.0;
+          for (size_t k = 0; k < S; ++k) {
+            const double d = (double)rk[k] - mean;
+            // exact with or without fusion: ranks make d a multiple of 1/2
+            ss = std::fma(d, d, ss);
+          }
+          if (ss > 0.0) {
+            const double inv = 1.0 / std::sqrt(ss);
+            for (size_t k = 0; k < S; ++k) u[k] = (float)(((double)rk[k] - mean) * inv);
+          }
         }
+      }
+      if (anynz) {
+        uint8_t any = 0;
+        for (size_t c = 0; c < S && !any; ++c) any = (double)rk[c] > min_cv;
+        anynz[r] = any;
       }
     }
   }
-  if (anynz)
-    for (size_t r = 0; r < n; ++r) {
-      uint8_t any = 0;
-      for (size_t c = 0; c < S && !any; ++c) any = (double)rk[r * S + c] > min_cv;
-      anynz[r] = any;
-    }
 }
 
 namespace {
 
 // q rows of `unit` at g.scale (clamped to [-127, 127]; the clamp error counts
-// in the error norm), folding the rows' norms into g's maxima.
-void quantize_rows(const float* unit, size_t n, uint32_t S, GateQuant& g) {
+// in the error norm), folding the rows' norms into g's maxima.  Rows in
+// parallel: each row's arithmetic is sequential, and maxima do not depend on
+// the order they are taken in.
+void quantize_rows(const float* unit, size_t n, uint32_t S, GateQuant& g, bool parallel,
+                   const uint32_t* order, size_t stride) {
   g.q.assign(n * g.kpad, 0);
+  double mq = g.max_qnorm, me = g.max_err, mu = g.max_unorm;
+#pragma omp parallel for schedule(static) reduction(max : mq, me, mu) if (parallel)
   for (size_t r = 0; r < n; ++r) {
     double qq = 0, ee = 0, uu = 0;
+    const float* row = unit + (order ? order[r] : r) * stride;
     for (uint32_t k = 0; k < S; ++k) {
-      const double u = unit[r * S + k];
+      const double u = row[k];
       long v = g.scale > 0 ? std::lround(u * g.scale) : 0;
       v = std::max(-127L, std::min(127L, v));
       g.q[r * g.kpad + k] = (int8_t)v;
@@ -328,44 +354,58 @@ void quantize_rows(const float* unit, size_t n, uint32_t S, GateQuant& g) {
       ee += (u - x) * (u - x);
       uu += u * u;
     }
-    g.max_qnorm = std::max(g.max_qnorm, std::sqrt(qq));
-    g.max_err = std::max(g.max_err, std::sqrt(ee));
-    g.max_unorm = std::max(g.max_unorm, std::sqrt(uu));
+    mq = std::max(mq, std::sqrt(qq));
+    me = std::max(me, std::sqrt(ee));
+    mu = std::max(mu, std::sqrt(uu));
   }
+  g.max_qnorm = mq;
+  g.max_err = me;
+  g.max_unorm = mu;
 }
 
-double abs_max(const float* unit, size_t count) {
+// max |u| over the n rows (S values each, stride apart)
+double abs_max(const float* unit, size_t n, uint32_t S, size_t stride, bool parallel) {
   double amax = 0;
-  for (size_t t = 0; t < count; ++t) amax = std::max(amax, (double)std::fabs(unit[t]));
+#pragma omp parallel for schedule(static) reduction(max : amax) if (parallel)
+  for (size_t r = 0; r < n; ++r)
+    for (uint32_t k = 0; k < S; ++k) amax = std::max(amax, (double)std::fabs(unit[r * stride + k]));
   return amax;
 }
 
 }  // namespace
 
-void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g) {
+void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g, bool parallel,
+                   const uint32_t* order, size_t stride) {
+  if (stride == 0) stride = S;
   g.S = S;
   g.kpad = 32 * ((S + 31) / 32);
-  const double amax = abs_max(unit, n * S);
+  const double amax = abs_max(unit, n, S, stride, parallel);
   g.scale = amax > 0 ? 127.0 / amax : 0.0;
   g.max_qnorm = g.max_err = g.max_unorm = 0;
-  quantize_rows(unit, n, S, g);
+  quantize_rows(unit, n, S, g, parallel, order, stride);
 }
 
-void gate_quantize_more(const float* unit, size_t n, uint32_t S, GateQuant& g) {
+void gate_quantize_more(const float* unit, size_t n, uint32_t S, GateQuant& g, bool parallel,
+                        const uint32_t* order, size_t stride) {
+  if (stride == 0) stride = S;
   g.S = S;
   g.kpad = 32 * ((S + 31) / 32);
   if (g.scale <= 0) {  // every earlier row is zero: any scale quantises them exactly
-    const double amax = abs_max(unit, n * S);
+    const double amax = abs_max(unit, n, S, stride, parallel);
     g.scale = amax > 0 ? 127.0 / amax : 0.0;
   }
-  quantize_rows(unit, n, S, g);
+  quantize_rows(unit, n, S, g, parallel, order, stride);
 }
 
-void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32_t>& perm) {
-  std::vector<uint32_t> key(n, 0);
+void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32_t>& perm, bool parallel,
+                     size_t stride) {
+  if (stride == 0) stride = S;
+  // (key << 32 | row): sorting these is the stable sort of the rows by key
+  std::vector<uint64_t> kr(n), tmp(n);
   const uint32_t top = std::min<uint32_t>(3, S);
+#pragma omp parallel for schedule(static) if (parallel)
   for (size_t r = 0; r < n; ++r) {
-    const float* u = unit + r * S;
+    const float* u = unit + r * stride;
     uint32_t k = 0;
     uint64_t used = 0;
     for (uint32_t t = 0; t < top; ++t) {
@@ -379,11 +419,20 @@ void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32
       used |= 1ull << best;
       k = (k << 8) | best;
     }
-    key[r] = k;
+    kr[r] = (uint64_t)k << 32 | r;
+  }
+  // least significant digit first over the key's 8 * top bits (the rows are
+  // already in order): each pass is stable
+  for (uint32_t pass = 0; pass < top; ++pass) {
+    const uint32_t shift = 32 + 8 * pass;
+    size_t cnt[257] = {0};
+    for (size_t r = 0; r < n; ++r) ++cnt[((kr[r] >> shift) & 0xFF) + 1];
+    for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+    for (size_t r = 0; r < n; ++r) tmp[cnt[(kr[r] >> shift) & 0xFF]++] = kr[r];
+    kr.swap(tmp);
   }
   perm.resize(n);
-  std::iota(perm.begin(), perm.end(), 0u);
-  std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+  for (size_t r = 0; r < n; ++r) perm[r] = (uint32_t)kr[r];
 }
 
 // Bound on |fl(u_i . u_j) - q_i . q_j / s^2| for every pair of rows of g.

@@ -45,8 +45,13 @@ void sv_tables(uint32_t m, double b0, std::vector<double>& svd, std::vector<floa
 
 // Depth features (src/rabbitbin.cpp:4784-4988).  `parallel`: rows on the
 // host's OpenMP threads (off when the caller already runs one call per thread).
+// `stride` (0: S): unit and raw rows are written `stride` floats apart, the
+// padding after each row's S values zeroed.
 void depth_features(const float* depth, size_t n, size_t S, double min_cv, float* ranked,
-                    float* unit, float* raw, double* colnorm, uint8_t* anynz, bool parallel = true);
+                    float* unit, float* raw, double* colnorm, uint8_t* anynz, bool parallel = true,
+                    size_t stride = 0);
+// sum[c] += the n rows' depth[r][c], rows in order for every column.
+void column_sums(const float* depth, size_t n, size_t S, double* sum);
 
 // int8 image of the unit rows for the MFMA prefilter of the abundance gate.
 // q = round(u * scale) with scale = 127 / max|u| (zero padded to kpad).  The
@@ -63,17 +68,23 @@ struct GateQuant {
   double max_unorm = 0;   // max_i ||u_i||
   uint32_t S = 0;
 };
-void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g);
+// `parallel` (here and below): rows on the host's OpenMP threads.  Row r of
+// the quantisation is unit row order[r] (r when order is null); rows are
+// `stride` floats apart (0: S).
+void gate_quantize(const float* unit, size_t n, uint32_t S, GateQuant& g, bool parallel = true,
+                   const uint32_t* order = nullptr, size_t stride = 0);
 // Rows appended to an existing quantisation: the same scale (values beyond
 // +-127 clamp, and the clamp error counts in max_err), g.q = the new rows only,
 // the norm maxima taken over the earlier and the new rows.  The cut stays a
 // rigorous bound for every pair of rows quantised under g.
-void gate_quantize_more(const float* unit, size_t n, uint32_t S, GateQuant& g);
+void gate_quantize_more(const float* unit, size_t n, uint32_t S, GateQuant& g, bool parallel = true,
+                        const uint32_t* order = nullptr, size_t stride = 0);
 // Row order of the fused pair pass: rows sorted by the indices of their three
 // largest unit-vector entries (ties to the smaller sample, then by row), so
 // contigs with the same abundance pattern are adjacent and the pairs that
 // pass the abundance gate fall into few tiles.  perm[r] = original row.
-void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32_t>& perm);
+void pair_pass_order(const float* unit, size_t n, uint32_t S, std::vector<uint32_t>& perm,
+                     bool parallel = true, size_t stride = 0);
 // Smallest integer dot a kept pair can have; INT32_MIN disables the prefilter.
 int32_t gate_qcut(const GateQuant& g, float thr);
 // Smallest integer dot that proves a pair kept: the same error terms taken the

@@ -1579,8 +1579,10 @@ __device__ void select_dense_row(const RowSource& src, uint32_t i, const uint16_
   }
 }
 
-__global__ __launch_bounds__(256) void select_topk_kernel(
-    const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
+// One row with the whole workgroup: dense rows, and candidate lists longer
+// than select_wave_kernel takes.
+__device__ void select_block_row(
+    uint32_t i, const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
     const uint16_t* __restrict__ mat, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
     const uint16_t* __restrict__ level_all, uint32_t m, uint32_t K,
@@ -1592,7 +1594,6 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
   __shared__ uint16_t sel_cnt[1024];
   __shared__ uint32_t nsel, tot, tmp_bin, tmp_below, rmax;
   __shared__ uint32_t wave_tot[4];
-  const uint32_t i = blockIdx.x;
   const uint32_t tid = threadIdx.x;
   const uint32_t a = asm_of[i];
   const RowSource src = row_source(list, off, mat, anynz, desc[a], i);
@@ -1744,6 +1745,176 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
   if (tid == 0) {
     topk_len[i] = ns;
     rowmax_out[i] = rmax;
+  }
+}
+
+// The rows select_wave_kernel queued, one workgroup at a time.
+__global__ __launch_bounds__(256) void select_topk_kernel(
+    const uint32_t* __restrict__ queue, const uint32_t* __restrict__ qcount,
+    const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
+    const uint16_t* __restrict__ mat, const uint8_t* __restrict__ anynz,
+    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
+    const uint16_t* __restrict__ level_all, uint32_t m, uint32_t K,
+    uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
+    uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out,
+    uint32_t* __restrict__ fence, uint32_t FW) {
+  const uint32_t nq = *qcount;
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    select_block_row(queue[q], list, off, mat, anynz, desc, asm_of, level_all, m, K, topk_id, topk_cnt,
+                     topk_len, rowmax_out, fence, FW);
+    __syncthreads();  // the row's LDS is read before the next row writes it
+  }
+}
+
+// Candidate-list rows of at most kWaveSelect entries, one wave per row (four
+// rows per workgroup) and no workgroup barrier: the same selection as
+// select_staged_row.  Lane l holds entries l, l + 64, ... (id, level << 16 |
+// count) in registers.  When the row has more than K entries, a level
+// histogram in the wave's LDS gives the threshold level L* (the largest
+// level whose suffix count reaches K): lane l sums bins 16l..16l+15, a
+// suffix scan over the lanes finds the lane holding L*, and that lane walks
+// its bins.  Entries above L* are kept; of those at L*, all when they fit,
+// else the need smallest ids (rank among the tied ids).  The kept entries
+// are compacted with ballots, and each one's output slot is the number of
+// kept ids below it, so the list comes out sorted by id.  Other rows (dense
+// rows, longer lists) are appended to `queue` for select_topk_kernel.
+constexpr uint32_t kWaveSelect = 512;
+constexpr uint32_t kWsPer = kWaveSelect / 64;  // entries per lane
+
+__global__ __launch_bounds__(256) void select_wave_kernel(
+    const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
+    const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
+    const uint16_t* __restrict__ level_all, uint32_t m, uint32_t nrows, uint32_t K,
+    uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
+    uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out,
+    uint32_t* __restrict__ fence, uint32_t FW, uint32_t* __restrict__ queue,
+    uint32_t* __restrict__ qcount) {
+  // per wave: the level histogram (1024 bins), then kept ids [0, 512) and
+  // tied ids [512, 1024); kept counts
+  __shared__ __align__(16) uint32_t wlds[4][1024];
+  __shared__ uint16_t wcnt[4][kWaveSelect];
+  const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
+  const uint32_t i = blockIdx.x * 4 + wv;
+  if (i >= nrows) return;  // wave-uniform
+  const uint32_t a = asm_of[i];
+  const uint64_t o0 = off[i];
+  const uint32_t len = (uint32_t)(off[i + 1] - o0);
+  if (desc[a].dense || len > kWaveSelect) {
+    if (ln == 0) queue[atomicAdd(qcount, 1u)] = i;
+    return;
+  }
+  const uint16_t* __restrict__ level = level_all + (size_t)a * (m + 1);
+  const uint32_t nlevels = m + 1;
+  uint32_t* __restrict__ h = wlds[wv];
+  uint32_t id[kWsPer], cl[kWsPer];
+  uint32_t my_max = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kWsPer; ++k) {
+    const uint32_t t = ln + 64 * k;
+    id[k] = 0xFFFFFFFFu;
+    cl[k] = 0;
+    if (t < len) {
+      const uint64_t e = list[o0 + t];
+      const uint32_t c = (uint32_t)(e >> 32);
+      id[k] = (uint32_t)e;
+      cl[k] = (uint32_t)level[c] << 16 | c;
+      my_max = max(my_max, c);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, (uint32_t)__shfl_xor((int)my_max, o));
+  uint32_t Ls = 0, need = 0xFFFFFFFFu;
+  bool tie_cut = false;
+  if (len > K) {
+    for (uint32_t b = ln; b < 1024; b += 64) h[b] = 0;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t k = 0; k < kWsPer; ++k)
+      if (ln + 64 * k < len) atomicAdd(&h[cl[k] >> 16], 1u);
+    __builtin_amdgcn_wave_barrier();
+    uint32_t part = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 16; ++q) part += h[16 * ln + q];  // bins >= nlevels are zero
+    // entries in the bins of the lanes above this one
+    uint32_t incl = part;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_down((int)incl, d);
+      if (ln + d < 64) incl += t;
+    }
+    const uint32_t above = incl - part;
+    const bool mine = above < K && K <= incl;  // exactly one lane
+    uint32_t lb = 0, lbelow = 0;
+    if (mine) {
+      uint32_t acc = above;
+      for (int q = 15; q >= 0; --q) {
+        const uint32_t b = 16 * ln + (uint32_t)q;
+        if (acc + h[b] >= K) {
+          lb = b;
+          lbelow = acc;
+          break;
+        }
+        acc += h[b];
+      }
+    }
+    const uint64_t owner = __ballot(mine);  // one lane (the row has more than K entries)
+    const int src = owner ? __ffsll((long long)owner) - 1 : 0;
+    Ls = (uint32_t)__shfl((int)lb, src) & 1023u;
+    need = K - (uint32_t)__shfl((int)lbelow, src);
+    tie_cut = need < h[Ls];
+    (void)nlevels;
+    __builtin_amdgcn_wave_barrier();  // the histogram is read before it is reused
+  }
+  uint32_t* __restrict__ kid = h;          // kept ids
+  uint32_t* __restrict__ tid_ = h + 512;   // tied ids (tie_cut only)
+  uint16_t* __restrict__ kc = wcnt[wv];
+  uint32_t nk = 0, nt = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kWsPer; ++k) {
+    const bool v = ln + 64 * k < len;
+    const uint32_t lv = cl[k] >> 16;
+    const bool keep = v && (len <= K || lv > Ls || (lv == Ls && !tie_cut));
+    const bool tie = v && len > K && tie_cut && lv == Ls;
+    const uint64_t km = __ballot(keep), tm = __ballot(tie);
+    if (keep) {
+      const uint32_t p = nk + lanes_below(km);
+      kid[p] = id[k];
+      kc[p] = (uint16_t)(cl[k] & 0xFFFFu);
+    }
+    if (tie) tid_[nt + lanes_below(tm)] = id[k];
+    nk += (uint32_t)__popcll(km);
+    nt += (uint32_t)__popcll(tm);
+  }
+  if (tie_cut) {
+    for (uint32_t u = nt + ln; u < ((nt + 3) & ~3u); u += 64) tid_[u] = 0xFFFFFFFFu;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t k = 0; k < kWsPer; ++k) {
+      const bool tie = ln + 64 * k < len && (cl[k] >> 16) == Ls;
+      const bool take = tie && count_below(tid_, (nt + 3) / 4, id[k]) < need;
+      const uint64_t km = __ballot(take);
+      if (take) {
+        const uint32_t p = nk + lanes_below(km);
+        kid[p] = id[k];
+        kc[p] = (uint16_t)(cl[k] & 0xFFFFu);
+      }
+      nk += (uint32_t)__popcll(km);
+    }
+  }
+  const uint32_t ns = nk;  // = min(len, K)
+  for (uint32_t u = ns + ln; u < ((ns + 3) & ~3u); u += 64) kid[u] = 0xFFFFFFFFu;
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t t = ln; t < ns; t += 64) {
+    const uint32_t me = kid[t];
+    const uint32_t pos = count_below(kid, (ns + 3) / 4, me);
+    topk_id[(size_t)i * K + pos] = me;
+    topk_cnt[(size_t)i * K + pos] = kc[t];
+    if ((pos & 15u) == 0) fence[(size_t)i * FW + pos / 16] = me;
+  }
+  for (uint32_t q = (ns + 15) / 16 + ln; q < FW; q += 64) fence[(size_t)i * FW + q] = 0xFFFFFFFFu;
+  if (ln == 0) {
+    topk_len[i] = ns;
+    rowmax_out[i] = my_max;
   }
 }
 
@@ -2633,11 +2804,19 @@ hipError_t launch_select(const uint64_t* list, const uint64_t* off, const uint16
                          const uint8_t* anynz, const AsmDesc* desc, const uint16_t* asm_of,
                          uint32_t nrows, const uint16_t* level, uint32_t m, uint32_t K,
                          uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
-                         uint32_t* rowmax, uint32_t* fence, uint32_t FW, hipStream_t st) {
+                         uint32_t* rowmax, uint32_t* fence, uint32_t FW, uint32_t* queue,
+                         hipStream_t st) {
   if (nrows == 0) return hipSuccess;
   if (m + 1 > 1024 || K > 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(select_topk_kernel, dim3(nrows), dim3(256), 0, st, list, off, mat, anynz, desc,
-                     asm_of, level, m, K, topk_id, topk_cnt, topk_len, rowmax, fence, FW);
+  uint32_t* qcount = queue + nrows;
+  const hipError_t e = launch_fill_u32(qcount, 1, 0u, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(select_wave_kernel, dim3((nrows + 3) / 4), dim3(256), 0, st, list, off, desc, asm_of,
+                     level, m, nrows, K, topk_id, topk_cnt, topk_len, rowmax, fence, FW, queue, qcount);
+  // the queued rows: a grid-stride loop over them (the count is on the device)
+  hipLaunchKernelGGL(select_topk_kernel, dim3(std::min<uint32_t>(nrows, 2048)), dim3(256), 0, st, queue,
+                     qcount, list, off, mat, anynz, desc, asm_of, level, m, K, topk_id, topk_cnt, topk_len,
+                     rowmax, fence, FW);
   return hipGetLastError();
 }
 
