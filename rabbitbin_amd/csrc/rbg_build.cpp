@@ -833,7 +833,18 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   RBG_TRY_HIP(ctx, b->topk_id.ensure(((size_t)N * K + 16) * 4));
   RBG_TRY_HIP(ctx, b->topk_cnt.ensure((size_t)N * K * 2));
   RBG_TRY_HIP(ctx, b->topk_len.ensure((size_t)N * 4));
-  RBG_TRY_HIP(ctx, b->selq.ensure((size_t)(N + 1) * 4));  // rows for the workgroup selection + count
+  // rows for the workgroup selection: the dense assemblies' rows, then the
+  // long candidate lists the wave selection queues; two counters
+  RBG_TRY_HIP(ctx, b->selq.ensure((size_t)(N + 2) * 4));
+  {
+    std::vector<uint32_t> srows;
+    for (uint32_t a = 0; a < A; ++a)
+      if (D[a].dense)
+        for (uint32_t r = 0; r < D[a].n; ++r) srows.push_back(D[a].row0 + r);
+    b->n_sel_static = (uint32_t)srows.size();
+    if (!srows.empty())
+      RBG_TRY_HIP(ctx, hipMemcpyAsync(b->selq.p, srows.data(), srows.size() * 4, hipMemcpyHostToDevice, st));
+  }
   RBG_TRY_HIP(ctx, b->rowmax.ensure((size_t)N * 4));
   RBG_TRY_HIP(ctx, b->fence.ensure((size_t)N * FW * 4));
   RBG_TRY_HIP(ctx, b->keep.ensure((size_t)N * KW * 8));
@@ -942,17 +953,23 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
   RBG_TRY_HIP(ctx, launch_fill_u32((uint32_t*)b->counters.p, 8, 0u, st));
   unsigned long long* ncand = (unsigned long long*)b->counters.p;
   // The int8 prefilter of the pair pass reads only the int8 rows, the
-  // pair-pass order and the gate items, which the plan has uploaded: it runs
-  // on the side stream beside the sketch and b0 (they are bound by LDS and
-  // VALU, it by the MFMA tile cadence) and joins before the exact gate.
-  if (b->has_gate_items) {
+  // pair-pass order and the gate items, which the plan has uploaded.  In a
+  // build of several assemblies it runs on the side stream beside the sketch,
+  // b0 and the small assemblies' dense counts, and joins before the exact
+  // gate (44.0 against 46.5 ms for 64 assemblies).  A single assembly has no
+  // dense work beside it, and the sketch and the prefilter compete for the
+  // same workgroup slots (together 4.5 ms, one after the other 4.6): it runs
+  // on the main stream, so every kernel's time is its own.
+  auto prefilter = [&](hipStream_t s) {
+    return launch_gate_mfma((const int8_t*)b->q8.p, b->kpad, desc, (const uint32_t*)b->perm.p,
+                            (const uint32_t*)b->gate_items.p, b->n_gate_items, (uint64_t*)b->cand.p, ncand,
+                            b->cand_cap, s);
+  };
+  const bool side = b->has_gate_items && A > 1;
+  if (side) {
     RBG_TRY_HIP(ctx, hipEventRecord(ctx->fork_ev, st));
     RBG_TRY_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
-    RBG_TRY(kside("gate_mfma_kernel", [&] {
-      return launch_gate_mfma((const int8_t*)b->q8.p, b->kpad, desc, (const uint32_t*)b->perm.p,
-                              (const uint32_t*)b->gate_items.p, b->n_gate_items, (uint64_t*)b->cand.p, ncand,
-                              b->cand_cap, ctx->side);
-    }));
+    RBG_TRY(kside("gate_mfma_kernel", [&] { return prefilter(ctx->side); }));
     RBG_TRY_HIP(ctx, hipEventRecord(ctx->join_ev, ctx->side));
   }
   const uint32_t mpad = 32 * W;
@@ -1001,8 +1018,13 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                          (uint16_t*)b->mat.p, st));
     RBG_TRY(kend());
   }
-  if (b->has_gate_items)
+  if (side) {
     RBG_TRY_HIP(ctx, hipStreamWaitEvent(st, ctx->join_ev, 0));  // the candidates are in
+  } else if (b->has_gate_items) {
+    RBG_TRY(kbeg("gate_mfma_kernel"));
+    RBG_TRY_HIP(ctx, prefilter(st));
+    RBG_TRY(kend());
+  }
   RBG_TRY(mark(6));
   RBG_TRY(mark(7));
   if (b->has_gate_items) {
@@ -1042,7 +1064,7 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
                                  N, (const uint16_t*)b->level.p, m, K, (uint32_t*)b->topk_id.p,
                                  (uint16_t*)b->topk_cnt.p, (uint32_t*)b->topk_len.p,
                                  (uint32_t*)b->rowmax.p, (uint32_t*)b->fence.p, FW,
-                                 (uint32_t*)b->selq.p, st));
+                                 (uint32_t*)b->selq.p, b->n_sel_static, st));
   RBG_TRY(kend());
   RBG_TRY(mark(10));
   // calibration (the glibc rand() draws are prepared per (n, seed))

@@ -148,12 +148,14 @@ hipError_t launch_scatter(const uint64_t* cand, uint64_t cap, const unsigned lon
 // ---- stage (c)
 // Per-row top-K (sv desc, id asc) over the row's partners: regime 1 rows read
 // the dense count matrix, regime 2 rows their candidate list (off, list).
+// queue: nrows + 2 words, its first nstatic entries the rows of the dense
+// assemblies (written by the plan).
 hipError_t launch_select(const uint64_t* list, const uint64_t* off, const uint16_t* mat,
                          const uint8_t* anynz, const AsmDesc* desc, const uint16_t* asm_of,
                          uint32_t nrows, const uint16_t* level, uint32_t m, uint32_t K,
                          uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
                          uint32_t* rowmax, uint32_t* fence, uint32_t FW, uint32_t* queue,
-                         hipStream_t st);
+                         uint32_t nstatic, hipStream_t st);
 // Fence rows of the selected lists: FW words per row (16-id stride, padded
 // with 0xFFFFFFFF); FW = 16 * ceil(ceil(K/16) / 16).
 inline uint32_t fence_words(uint32_t K) { return 16u * (((K + 15u) / 16u + 15u) / 16u); }

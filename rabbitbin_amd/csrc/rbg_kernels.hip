@@ -1748,9 +1748,12 @@ __device__ void select_block_row(
   }
 }
 
-// The rows select_wave_kernel queued, one workgroup at a time.
+// One workgroup per row over queue[0, nstatic + qcount[0]): the dense rows
+// the plan listed, then the long candidate lists select_wave_kernel queued.
+// Rows are taken one at a time from the counter qcount[1] (dense rows differ
+// in length from one assembly to the next).
 __global__ __launch_bounds__(256) void select_topk_kernel(
-    const uint32_t* __restrict__ queue, const uint32_t* __restrict__ qcount,
+    const uint32_t* __restrict__ queue, uint32_t nstatic, uint32_t* __restrict__ qcount,
     const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
     const uint16_t* __restrict__ mat, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
@@ -1758,8 +1761,14 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
     uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
     uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out,
     uint32_t* __restrict__ fence, uint32_t FW) {
-  const uint32_t nq = *qcount;
-  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+  __shared__ uint32_t next;
+  const uint32_t nq = nstatic + qcount[0];
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(&qcount[1], 1u);
+    __syncthreads();
+    const uint32_t q = next;
+    __syncthreads();  // read before thread 0 takes the next row
+    if (q >= nq) break;  // workgroup-uniform
     select_block_row(queue[q], list, off, mat, anynz, desc, asm_of, level_all, m, K, topk_id, topk_cnt,
                      topk_len, rowmax_out, fence, FW);
     __syncthreads();  // the row's LDS is read before the next row writes it
@@ -1776,8 +1785,9 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
 // its bins.  Entries above L* are kept; of those at L*, all when they fit,
 // else the need smallest ids (rank among the tied ids).  The kept entries
 // are compacted with ballots, and each one's output slot is the number of
-// kept ids below it, so the list comes out sorted by id.  Other rows (dense
-// rows, longer lists) are appended to `queue` for select_topk_kernel.
+// kept ids below it, so the list comes out sorted by id.  Longer lists are
+// appended to `queue` (after its nstatic dense rows) for select_topk_kernel;
+// dense rows are left to it.
 constexpr uint32_t kWaveSelect = 512;
 constexpr uint32_t kWsPer = kWaveSelect / 64;  // entries per lane
 
@@ -1787,7 +1797,7 @@ __global__ __launch_bounds__(256) void select_wave_kernel(
     const uint16_t* __restrict__ level_all, uint32_t m, uint32_t nrows, uint32_t K,
     uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
     uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out,
-    uint32_t* __restrict__ fence, uint32_t FW, uint32_t* __restrict__ queue,
+    uint32_t* __restrict__ fence, uint32_t FW, uint32_t* __restrict__ queue, uint32_t nstatic,
     uint32_t* __restrict__ qcount) {
   // per wave: the level histogram (1024 bins), then kept ids [0, 512) and
   // tied ids [512, 1024); kept counts
@@ -1799,8 +1809,9 @@ __global__ __launch_bounds__(256) void select_wave_kernel(
   const uint32_t a = asm_of[i];
   const uint64_t o0 = off[i];
   const uint32_t len = (uint32_t)(off[i + 1] - o0);
-  if (desc[a].dense || len > kWaveSelect) {
-    if (ln == 0) queue[atomicAdd(qcount, 1u)] = i;
+  if (desc[a].dense) return;  // listed by the plan
+  if (len > kWaveSelect) {
+    if (ln == 0) queue[nstatic + atomicAdd(qcount, 1u)] = i;
     return;
   }
   const uint16_t* __restrict__ level = level_all + (size_t)a * (m + 1);
@@ -2805,18 +2816,18 @@ hipError_t launch_select(const uint64_t* list, const uint64_t* off, const uint16
                          uint32_t nrows, const uint16_t* level, uint32_t m, uint32_t K,
                          uint32_t* topk_id, uint16_t* topk_cnt, uint32_t* topk_len,
                          uint32_t* rowmax, uint32_t* fence, uint32_t FW, uint32_t* queue,
-                         hipStream_t st) {
+                         uint32_t nstatic, hipStream_t st) {
   if (nrows == 0) return hipSuccess;
-  if (m + 1 > 1024 || K > 1024) return hipErrorInvalidValue;
-  uint32_t* qcount = queue + nrows;
-  const hipError_t e = launch_fill_u32(qcount, 1, 0u, st);
+  if (m + 1 > 1024 || K > 1024 || nstatic > nrows) return hipErrorInvalidValue;
+  uint32_t* qcount = queue + nrows;  // queued long lists, rows taken by the workgroup pass
+  const hipError_t e = launch_fill_u32(qcount, 2, 0u, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(select_wave_kernel, dim3((nrows + 3) / 4), dim3(256), 0, st, list, off, desc, asm_of,
-                     level, m, nrows, K, topk_id, topk_cnt, topk_len, rowmax, fence, FW, queue, qcount);
-  // the queued rows: a grid-stride loop over them (the count is on the device)
+                     level, m, nrows, K, topk_id, topk_cnt, topk_len, rowmax, fence, FW, queue, nstatic, qcount);
+  // the listed and queued rows (their number is on the device)
   hipLaunchKernelGGL(select_topk_kernel, dim3(std::min<uint32_t>(nrows, 2048)), dim3(256), 0, st, queue,
-                     qcount, list, off, mat, anynz, desc, asm_of, level, m, K, topk_id, topk_cnt, topk_len,
-                     rowmax, fence, FW);
+                     nstatic, qcount, list, off, mat, anynz, desc, asm_of, level, m, K, topk_id, topk_cnt,
+                     topk_len, rowmax, fence, FW);
   return hipGetLastError();
 }
 
