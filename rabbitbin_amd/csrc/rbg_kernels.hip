@@ -1581,7 +1581,7 @@ __device__ void select_dense_row(const RowSource& src, uint32_t i, const uint16_
 
 // One row with the whole workgroup: dense rows, and candidate lists longer
 // than select_wave_kernel takes.
-__device__ void select_block_row(
+__device__ __forceinline__ void select_block_row(
     uint32_t i, const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
     const uint16_t* __restrict__ mat, const uint8_t* __restrict__ anynz,
     const AsmDesc* __restrict__ desc, const uint16_t* __restrict__ asm_of,
@@ -1748,11 +1748,24 @@ __device__ void select_block_row(
   }
 }
 
-// One workgroup per row over queue[0, nstatic + qcount[0]): the dense rows
-// the plan listed, then the long candidate lists select_wave_kernel queued.
-// Rows are taken one at a time from the counter qcount[1] (dense rows differ
-// in length from one assembly to the next).
+// One workgroup per row: workgroup b takes queue[b], a dense row the plan
+// listed (one row per workgroup and no loop, so the row index stays in
+// scalar registers and the kernel keeps the row function's occupancy).
 __global__ __launch_bounds__(256) void select_topk_kernel(
+    const uint32_t* __restrict__ queue, const uint64_t* __restrict__ list,
+    const uint64_t* __restrict__ off, const uint16_t* __restrict__ mat,
+    const uint8_t* __restrict__ anynz, const AsmDesc* __restrict__ desc,
+    const uint16_t* __restrict__ asm_of, const uint16_t* __restrict__ level_all, uint32_t m,
+    uint32_t K, uint32_t* __restrict__ topk_id, uint16_t* __restrict__ topk_cnt,
+    uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out, uint32_t* __restrict__ fence,
+    uint32_t FW) {
+  select_block_row(queue[blockIdx.x], list, off, mat, anynz, desc, asm_of, level_all, m, K, topk_id, topk_cnt,
+                   topk_len, rowmax_out, fence, FW);
+}
+
+// The long candidate lists select_wave_kernel queued (queue[nstatic + q],
+// q < qcount[0]), taken one at a time from the counter qcount[1].
+__global__ __launch_bounds__(256) void select_long_kernel(
     const uint32_t* __restrict__ queue, uint32_t nstatic, uint32_t* __restrict__ qcount,
     const uint64_t* __restrict__ list, const uint64_t* __restrict__ off,
     const uint16_t* __restrict__ mat, const uint8_t* __restrict__ anynz,
@@ -1762,15 +1775,15 @@ __global__ __launch_bounds__(256) void select_topk_kernel(
     uint32_t* __restrict__ topk_len, uint32_t* __restrict__ rowmax_out,
     uint32_t* __restrict__ fence, uint32_t FW) {
   __shared__ uint32_t next;
-  const uint32_t nq = nstatic + qcount[0];
+  const uint32_t nq = qcount[0];
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(&qcount[1], 1u);
     __syncthreads();
-    const uint32_t q = next;
+    const uint32_t q = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();  // read before thread 0 takes the next row
     if (q >= nq) break;  // workgroup-uniform
-    select_block_row(queue[q], list, off, mat, anynz, desc, asm_of, level_all, m, K, topk_id, topk_cnt,
-                     topk_len, rowmax_out, fence, FW);
+    select_block_row(queue[nstatic + q], list, off, mat, anynz, desc, asm_of, level_all, m, K, topk_id,
+                     topk_cnt, topk_len, rowmax_out, fence, FW);
     __syncthreads();  // the row's LDS is read before the next row writes it
   }
 }
@@ -2824,10 +2837,14 @@ hipError_t launch_select(const uint64_t* list, const uint64_t* off, const uint16
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(select_wave_kernel, dim3((nrows + 3) / 4), dim3(256), 0, st, list, off, desc, asm_of,
                      level, m, nrows, K, topk_id, topk_cnt, topk_len, rowmax, fence, FW, queue, nstatic, qcount);
-  // the listed and queued rows (their number is on the device)
-  hipLaunchKernelGGL(select_topk_kernel, dim3(std::min<uint32_t>(nrows, 2048)), dim3(256), 0, st, queue,
-                     nstatic, qcount, list, off, mat, anynz, desc, asm_of, level, m, K, topk_id, topk_cnt,
-                     topk_len, rowmax, fence, FW);
+  // the listed dense rows, then the queued long lists (their number is on
+  // the device)
+  if (nstatic)
+    hipLaunchKernelGGL(select_topk_kernel, dim3(nstatic), dim3(256), 0, st, queue, list, off, mat, anynz, desc,
+                       asm_of, level, m, K, topk_id, topk_cnt, topk_len, rowmax, fence, FW);
+  hipLaunchKernelGGL(select_long_kernel, dim3(std::min<uint32_t>(nrows, 2048)), dim3(256), 0, st, queue, nstatic,
+                     qcount, list, off, mat, anynz, desc, asm_of, level, m, K, topk_id, topk_cnt, topk_len,
+                     rowmax, fence, FW);
   return hipGetLastError();
 }
 
