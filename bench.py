@@ -443,9 +443,11 @@ def plain_form(args, ctx):
     full = n == w["n"]
     kernels = [kernel_entry(k, kms[k], kms[k] / total_k, stats, profiled_traffic(args.workload, full, k))
                for k in sorted(names, key=lambda k: -kms[k]) if kms[k] >= 0.10 * total_k]
-    dom = kernels[0]
-    roofline = {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
-                "frac": dom["frac"], "traffic": dom["traffic"], "kernel": dom["kernel"], "kernel_ms": dom["ms"]}
+    # the largest kernel that has a roofline (kernels are sorted by time)
+    dom = next((k for k in kernels if "bound" in k), None)
+    roofline = None if dom is None else {
+        "bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
+        "frac": dom["frac"], "traffic": dom["traffic"], "kernel": dom["kernel"], "kernel_ms": dom["ms"]}
     # host inputs to host edges: upload, depth features, int8 rows and
     # pair-pass order, the step, and the edge list back (rbg_build_graph)
     t0 = time.perf_counter()
