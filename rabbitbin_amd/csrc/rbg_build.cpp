@@ -345,7 +345,7 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 size_t rbg_asm::device_bytes() const {
   const DevBuf* all[] = {&seqs, &offs, &sk_rows, &asm_of, &unit_pad, &raw_pad, &colnorm, &anynz,
                          &q8, &perm, &pi, &pj, &pcnt, &merge, &desc, &bscal, &svd, &svf, &level,
-                         &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
+                         &cal_rnd, &cal_members, &cal_roff, &cal_hist, &dense_items, &gate_items, &wx, &planes,
                          &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt, &selq,
                          &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
                          &ecnt, &escomp, &escore, &scan_tmp, &bounds, &near_e, &near_w, &cls_in};
@@ -358,7 +358,7 @@ void rbg_asm::release_buffers() {
   drop_graph();
   DevBuf* all[] = {&seqs, &offs, &sk_rows, &asm_of, &unit_pad, &raw_pad, &colnorm, &anynz,
                    &q8, &perm, &pi, &pj, &pcnt, &merge, &desc, &bscal, &svd, &svf, &level,
-                   &cal_rnd, &cal_members, &cal_roff, &dense_items, &gate_items, &wx, &planes,
+                   &cal_rnd, &cal_members, &cal_roff, &cal_hist, &dense_items, &gate_items, &wx, &planes,
                    &mat, &cand, &counters, &deg, &slot, &off, &list, &topk_id, &topk_cnt, &selq,
                    &topk_len, &rowmax, &fence, &keep, &row_edges, &row_off, &efrom, &eto,
                    &ecnt, &escomp, &escore, &scan_tmp, &bounds, &near_e, &near_w, &cls_in};
@@ -798,6 +798,10 @@ rbg_status batch_plan(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params* P) {
   RBG_TRY_HIP(ctx, b->cal_rnd.ensure(rnd.size() * 4));
   RBG_TRY_HIP(ctx, b->cal_members.ensure(std::max<size_t>(members.size(), 1) * 4));
   RBG_TRY_HIP(ctx, b->cal_roff.ensure(roff.size() * 4));
+  RBG_TRY_HIP(ctx, b->cal_hist.ensure((size_t)A * kCalibHistWords * 4));
+  b->cal_max_items = 0;
+  for (uint32_t a = 0; a < A; ++a)
+    b->cal_max_items = std::max(b->cal_max_items, D[a].regime == 1 ? D[a].n : roff[(size_t)a * 11 + 10] - roff[(size_t)a * 11]);
   RBG_TRY_HIP(ctx, b->merge.ensure(std::max<size_t>(md.size(), 1) * sizeof(MergeDesc)));
   RBG_TRY_HIP(ctx, hipMemcpyAsync(b->desc.p, D.data(), A * sizeof(AsmDesc), hipMemcpyHostToDevice, st));
   if (!ditems.empty())
@@ -898,7 +902,7 @@ size_t estimate_build_bytes(const AsmInput* in, uint32_t count, const rbg_graph_
   s += mat * 2 + cap * 16 + (2 * cap + extra) * 8;          // mat, cand + slots, list
   s += N * (4 + 8 + 4 + (uint64_t)K * 6 + 4 + 4 + FW * 4 + KW * 8 + 4 + 8);  // per-row arrays
   s += N * (uint64_t)K * (4 + 4 + 2 + 4 + 4);               // edges
-  s += (uint64_t)count * (64 * 4 + (m + 1) * 14 + kCalibRandDev * 4 + 30000 * 12);
+  s += (uint64_t)count * (64 * 4 + (m + 1) * 14 + kCalibRandDev * 4 + kCalibHistWords * 4 + 30000 * 12);
   return (size_t)s + (64u << 20);                           // small tables and slack
 }
 
@@ -1072,7 +1076,8 @@ static rbg_status enqueue_build(rbg_ctx* ctx, rbg_asm* b, const rbg_graph_params
   RBG_TRY_HIP(ctx, launch_calibrate((const uint32_t*)b->rowmax.p, desc, A, m,
                                     (const double*)b->svd.p, (const float*)b->svf.p,
                                     (const int32_t*)b->cal_rnd.p, (const uint32_t*)b->cal_members.p,
-                                    (const uint32_t*)b->cal_roff.p, P->coverage, bs, st));
+                                    (const uint32_t*)b->cal_roff.p, b->cal_max_items,
+                                    (uint32_t*)b->cal_hist.p, P->coverage, bs, st));
   RBG_TRY(kend());
   RBG_TRY(kbeg("emit_mark_kernel"));
   RBG_TRY_HIP(ctx, launch_emit_mark((const uint32_t*)b->topk_id.p, (const uint16_t*)b->topk_cnt.p,

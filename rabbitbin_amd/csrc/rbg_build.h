@@ -213,8 +213,9 @@ struct rbg_asm {
   uint32_t n_dense_items = 0, n_gate_items = 0;
   uint64_t mat_total = 0, cand_cap = 0, edge_cap = 0;
   uint32_t n_sel_static = 0;  // rows of dense assemblies at the head of selq
+  uint32_t cal_max_items = 0;  // most contigs one assembly samples for calibration
   rbg_schedule sched{16, 64};
-  rbg::DevBuf desc, bscal, svd, svf, level, cal_rnd, cal_members, cal_roff, dense_items, gate_items;
+  rbg::DevBuf desc, bscal, svd, svf, level, cal_rnd, cal_members, cal_roff, cal_hist, dense_items, gate_items;
   rbg::DevBuf wx, planes, mat, cand, counters, deg, slot, off, list, topk_id, topk_cnt, topk_len, selq,
       rowmax, fence, keep, row_edges, row_off, efrom, eto, ecnt, escomp, escore, scan_tmp, bounds,
       near_e, near_w;

@@ -51,6 +51,8 @@ struct AsmDesc {
 // calibration plan holds kCalibRandDev rand() values and 11 round offsets
 // per assembly.
 constexpr uint32_t kCalibRandDev = 2048;
+// words of calibration histogram scratch per assembly (10 rounds x 520 counts)
+constexpr uint32_t kCalibHistWords = 10 * 520;
 
 // ---- frozen-core recruitment: out_id[r] = the id of the one core whose
 // threshold unit row r's FP64 score reaches (sample order, fused multiply-
@@ -82,10 +84,12 @@ hipError_t launch_baseline_tables(const uint32_t* pcnt, const AsmDesc* desc, uin
                                   uint16_t* level, hipStream_t st);
 // simCutoff and cmin from the per-row maxima (regime 1: all rows of the
 // assembly; regime 2: its round members), consuming its rand() values in order.
+// max_items: the most contigs one assembly samples (its rows, or its members);
+// hist: nasm x 10 x 520 words of scratch for the rounds' count histograms.
 hipError_t launch_calibrate(const uint32_t* rowmax, const AsmDesc* desc, uint32_t nasm, uint32_t m,
                             const double* svd, const float* svf, const int32_t* rnd,
-                            const uint32_t* members, const uint32_t* round_off, double coverage,
-                            BuildScalars* bs, hipStream_t st);
+                            const uint32_t* members, const uint32_t* round_off, uint32_t max_items,
+                            uint32_t* hist, double coverage, BuildScalars* bs, hipStream_t st);
 
 // ---- stage (b), regime 1: n x n counts per assembly (each unordered pair
 // computed once and written at both places; the diagonal is not written)

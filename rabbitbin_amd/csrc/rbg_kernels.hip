@@ -2456,33 +2456,72 @@ __global__ __launch_bounds__(256) void baseline_tables_kernel(const uint32_t* __
 // Block-wide exclusive scan (256 threads; defined with the scan kernels).
 __device__ uint64_t block_scan_excl_u64(uint64_t v, uint64_t* wave_tot, uint64_t* total);
 
-// calib_converge over a table cov_at[p] = (contigs whose maxsim >= p / 1000)
-// / total, computed for every p beforehand in parallel, so the serial walk
-// only reads it.
-__device__ uint32_t calib_converge_dev(const double* cov_at, double coverage,
-                                       const int32_t* rnd, uint32_t nrnd, uint32_t& ri) {
-  uint32_t p = 999, pp = 1000;
-  double pcov = 0.0;
+// calib_converge over the suffix histogram of one round: cov(p) = (contigs
+// whose maxsim >= p / 1000) / denom = suf[lo_at[p]] / denom, with lo_at[p] the
+// first count whose maxsim reaches p / 1000 (m + 1 when none does:
+// suf[m + 1] = 0).  fl(c / denom) increases with c, so cov >= coverage exactly
+// when c >= cstar (the first such c): the walk compares integers and divides
+// only where it stops.  rmod[i] = rand() value i % 3.
+__device__ uint32_t calib_converge_dev(const uint32_t* suf, const uint16_t* lo_at, double denom,
+                                       uint32_t cstar, double coverage, const uint8_t* rmod, uint32_t nrnd,
+                                       uint32_t& ri) {
+  uint32_t p = 999, pp = 1000, pc = 0;
   while (p > 50) {
-    const double cov = cov_at[p];
-    if (cov >= coverage) {
+    const uint32_t c = suf[lo_at[p]];
+    if (c >= cstar) {
+      const double cov = (double)c / denom, pcov = (double)pc / denom;
       if (cov - coverage > coverage - pcov) p = pp;
       break;
     }
     pp = p;
-    pcov = cov;
-    const int r = (ri < nrnd ? rnd[ri] : 0) % 3;
+    pc = c;
+    const int r = ri < nrnd ? (int)rmod[ri] : 0;
     ++ri;
     p -= (uint32_t)(p > 990 ? r + 1 : p > 900 ? r + 3 : r + 9);
   }
   return p;
 }
 
+constexpr uint32_t kCalibRounds = 10;
+constexpr uint32_t kCalibBins = 520;  // counts 0..m+1 (m <= 512), per round
+static_assert(kCalibRounds * kCalibBins == kCalibHistWords, "calibration histogram scratch");
+
+// Every round's count histogram of every assembly (hist: kCalibRounds x
+// kCalibBins per assembly, zeroed before), one thread per sampled contig:
+// regime 1 every row of the assembly (n > 1), regime 2 members[t] for t in
+// [round_off[0], round_off[10]), round r holding [round_off[r],
+// round_off[r + 1]) (members are global rows).  Spread over the device, the
+// contigs' maxima are gathered in parallel instead of by one workgroup.
+__global__ __launch_bounds__(256) void calib_hist_kernel(const uint32_t* __restrict__ rowmax_all,
+                                                         const AsmDesc* __restrict__ desc,
+                                                         const uint32_t* __restrict__ members,
+                                                         const uint32_t* __restrict__ round_off_all,
+                                                         uint32_t* __restrict__ hist_all) {
+  const uint32_t a = blockIdx.y;
+  const AsmDesc d = desc[a];
+  const uint32_t* __restrict__ round_off = round_off_all + (size_t)a * 11;
+  const bool small = d.regime == 1;
+  const uint32_t t_beg = small ? 0u : round_off[0];
+  const uint32_t t_end = small ? (d.n > 1 ? d.n : 0u) : round_off[kCalibRounds];
+  const uint32_t t = t_beg + blockIdx.x * 256 + threadIdx.x;
+  if (t >= t_end) return;
+  const uint32_t c = small ? rowmax_all[d.row0 + t] : rowmax_all[members[t]];
+  uint32_t r = 0;
+  if (!small)
+    while (t >= round_off[r + 1]) ++r;
+  atomicAdd(&hist_all[((size_t)a * kCalibRounds + r) * kCalibBins + c], 1u);
+}
+
+// One workgroup per assembly.  Each wave turns its rounds' histograms
+// (calib_hist_kernel) into suffix sums, and thread 0 then runs the rounds'
+// convergence walks back to back (each reads the rand() values where the
+// previous one stopped); no workgroup barrier between rounds.
 __global__ __launch_bounds__(256) void calibrate_kernel(
     const uint32_t* __restrict__ rowmax_all, const AsmDesc* __restrict__ desc, uint32_t m,
     const double* __restrict__ svd_all, const float* __restrict__ svf_all,
     const int32_t* __restrict__ rnd_all, const uint32_t* __restrict__ members,
-    const uint32_t* __restrict__ round_off_all, double coverage, BuildScalars* __restrict__ bs_all) {
+    const uint32_t* __restrict__ round_off_all, const uint32_t* __restrict__ hist_all, double coverage,
+    BuildScalars* __restrict__ bs_all) {
   const uint32_t a = blockIdx.x;  // one assembly per block
   const uint32_t n = desc[a].n;
   const int regime = (int)desc[a].regime;
@@ -2492,86 +2531,92 @@ __global__ __launch_bounds__(256) void calibrate_kernel(
   const uint32_t nrnd = kCalibRandDev;
   const uint32_t* __restrict__ round_off = round_off_all + (size_t)a * 11;
   BuildScalars* __restrict__ bs = bs_all + a;
-  __shared__ uint32_t hist[514];
-  __shared__ double cov_at[1000];
+  __shared__ uint32_t hist[kCalibRounds][kCalibBins];
+  __shared__ uint16_t lo_at[1000];
   __shared__ double val[513];
-  __shared__ uint64_t wave_tot[4], tot;
-  // the rand() values, read one after another by the serial convergence walk
-  __shared__ int32_t rnd[kCalibRandDev];
+  // the rand() values % 3, read one after another by the serial convergence walks
+  __shared__ uint8_t rmod[kCalibRandDev];
   __shared__ uint32_t ri_s, cmin_s;
   __shared__ double sum_p;
-  __shared__ bool stop;
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, ln = tid & 63u, wv = tid >> 6;
+  const uint32_t rounds = regime == 1 ? 1 : kCalibRounds;
   for (uint32_t c = tid; c <= m; c += 256) val[c] = regime == 1 ? svd[c] : (double)svf[c];
-  for (uint32_t q = tid; q < kCalibRandDev; q += 256) rnd[q] = rnd_all[(size_t)a * kCalibRandDev + q];
-  if (tid == 0) {
-    ri_s = 0;
-    sum_p = 0.0;
-    stop = false;
+  for (uint32_t q = tid; q < kCalibRandDev; q += 256) rmod[q] = (uint8_t)(rnd_all[(size_t)a * kCalibRandDev + q] % 3);
+  for (uint32_t c = tid; c < rounds * kCalibBins; c += 256)
+    (&hist[0][0])[c] = hist_all[(size_t)a * kCalibRounds * kCalibBins + c];
+  // lo_at[p]: the first count whose maxsim reaches p / 1000 (val increases
+  // with the count)
+  for (uint32_t p = tid; p < 1000; p += 256) {
+    const double cut = (double)p / 1000.;
+    uint32_t lo = 0, hi = m + 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (val[mid] >= cut)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    lo_at[p] = (uint16_t)lo;
   }
-  const uint32_t rounds = regime == 1 ? 1 : 10;
-  const uint32_t per = (m + 1 + 255) / 256;  // histogram bins per thread (<= 3)
-  for (uint32_t r = 0; r < rounds; ++r) {
-    __syncthreads();
-    if (stop) break;
-    for (uint32_t c = tid; c <= m + 1; c += 256) hist[c] = 0;
-    __syncthreads();
-    if (regime == 1) {
-      if (n > 1)
-        for (uint32_t t = tid; t < n; t += 256) atomicAdd(&hist[rowmax[t]], 1u);
-    } else {
-      for (uint32_t t = round_off[r] + tid; t < round_off[r + 1]; t += 256)
-        atomicAdd(&hist[rowmax_all[members[t]]], 1u);  // members are global rows
+  __syncthreads();
+  // suffix sums, one wave per round: hist[r][c] <- contigs with count >= c
+  // (lane l takes bins 9l .. 9l+8; counts run to m <= 512, bin m + 1 stays 0)
+  for (uint32_t r = wv; r < rounds; r += 4) {
+    uint32_t* __restrict__ h = hist[r];
+    uint32_t loc[9], part = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 9; ++k) {
+      const uint32_t c = 9 * ln + k;
+      loc[k] = c <= m + 1 ? h[c] : 0u;
+      part += loc[k];
     }
-    __syncthreads();
-    // suffix sums: hist[c] <- contigs with count >= c (hist[m+1] = 0)
-    uint32_t loc[3] = {0, 0, 0}, mine = 0;
-    for (uint32_t k = 0; k < per; ++k) {
-      const uint32_t c = tid * per + k;
-      loc[k] = c <= m ? hist[c] : 0u;
-      mine += loc[k];
+    uint32_t incl = part;  // lanes >= ln
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_down((int)incl, d);
+      if (ln + d < 64) incl += t;
     }
-    const uint64_t before = block_scan_excl_u64(mine, wave_tot, &tot);  // bins < tid*per
-    __syncthreads();
-    const uint64_t total_in = tot;
-    uint64_t run = before;
-    for (uint32_t k = 0; k < per; ++k) {
-      const uint32_t c = tid * per + k;
-      if (c <= m) hist[c] = (uint32_t)(total_in - run);
+    uint32_t run = incl - part;  // bins above this lane's
+#pragma unroll
+    for (int k = 8; k >= 0; --k) {
+      const uint32_t c = 9 * ln + (uint32_t)k;
       run += loc[k];
+      if (c <= m + 1) h[c] = run;
     }
-    __syncthreads();
-    // cov_at[p]: the suffix count at the first count whose maxsim reaches
-    // p / 1000, over the contigs sampled
-    for (uint32_t p = tid; p < 1000; p += 256) {
-      const double cut = (double)p / 1000.;
-      uint32_t lo = 0, hi = m + 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (val[mid] >= cut)
-          hi = mid;
-        else
-          lo = mid + 1;
-      }
-      cov_at[p] = (double)hist[lo] / (regime == 1 ? (double)n : 2500.0);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t ri = 0;
+    double sp = 0.0;
+    const double denom = regime == 1 ? (double)n : 2500.0;
+    // cstar: the first count c with fl(c / denom) >= coverage (bisection over
+    // the counts a round can hold; maxc + 1 when none reaches it)
+    const uint32_t maxc = regime == 1 ? n : 2500u;
+    uint32_t lo = 0, hi = maxc + 1;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if ((double)mid / denom >= coverage)
+        hi = mid;
+      else
+        lo = mid + 1;
     }
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t ri = ri_s;
-      const uint32_t p = calib_converge_dev(cov_at, coverage, rnd, nrnd, ri);
-      ri_s = ri;
+    const uint32_t cstar = lo;
+    for (uint32_t r = 0; r < rounds; ++r) {
+      const uint32_t p = calib_converge_dev(hist[r], lo_at, denom, cstar, coverage, rmod, nrnd, ri);
       if (regime == 1) {
-        sum_p = (double)p;
+        sp = (double)p;
       } else {
         double mp = (double)p;
         if (mp < 51.0) mp = 50.0;
-        sum_p += mp;
-        if (r == 1 && sum_p / 2 < 51.0) {
-          sum_p = 50.0 * 10.0;
-          stop = true;
+        sp += mp;
+        if (r == 1 && sp / 2 < 51.0) {
+          sp = 50.0 * 10.0;
+          break;
         }
       }
     }
+    ri_s = ri;
+    sum_p = sp;
   }
   if (tid == 0) cmin_s = m + 1;
   __syncthreads();
@@ -2934,12 +2979,17 @@ hipError_t launch_baseline_tables(const uint32_t* pcnt, const AsmDesc* desc, uin
 
 hipError_t launch_calibrate(const uint32_t* rowmax, const AsmDesc* desc, uint32_t nasm, uint32_t m,
                             const double* svd, const float* svf, const int32_t* rnd,
-                            const uint32_t* members, const uint32_t* round_off, double coverage,
-                            BuildScalars* bs, hipStream_t st) {
+                            const uint32_t* members, const uint32_t* round_off, uint32_t max_items,
+                            uint32_t* hist, double coverage, BuildScalars* bs, hipStream_t st) {
   if (m < 2 || m > 512) return hipErrorInvalidValue;
   if (nasm == 0) return hipSuccess;
+  hipError_t e = launch_fill_u32(hist, (uint64_t)nasm * kCalibRounds * kCalibBins, 0u, st);
+  if (e != hipSuccess) return e;
+  if (max_items)
+    hipLaunchKernelGGL(calib_hist_kernel, dim3((max_items + 255) / 256, nasm), dim3(256), 0, st, rowmax, desc,
+                       members, round_off, hist);
   hipLaunchKernelGGL(calibrate_kernel, dim3(nasm), dim3(256), 0, st, rowmax, desc, m, svd, svf, rnd,
-                     members, round_off, coverage, bs);
+                     members, round_off, hist, coverage, bs);
   return hipGetLastError();
 }
 
