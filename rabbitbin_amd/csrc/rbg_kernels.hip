@@ -870,10 +870,12 @@ __device__ __forceinline__ void unpack_pair(uint64_t e, uint32_t& i, uint32_t& j
 // rows' planes (32 bytes each: one group's 8 planes, in two 16-byte halves),
 // so each load instruction of the eight lanes of a candidate reads one
 // contiguous 128-byte line of a row (the same half of eight groups).  The
-// candidate e comes in from the previous step, and every load that depends
-// on it (plane rows, unit rows, anynz, and the assembly's gate parameters
-// when the build holds several assemblies) is issued before any is used, so
-// a step waits for one memory round trip: nearly every candidate survives.
+// candidate e comes in from the previous step, and its plane rows are loaded
+// before anything is used, so a step waits for one memory round trip: nearly
+// every candidate survives.  The FP32 gate (unit rows, anynz, the assembly's
+// parameters when the build holds several, the accumulation and reductions)
+// runs only in steps where a candidate of the wave was not proven by the
+// prefilter (about 6% of them at 300k contigs).
 // Returns true on lane 0 of a survivor's eight, with its rows in ri, rj.
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
@@ -901,22 +903,10 @@ __device__ __forceinline__ bool gate_compare_one(
   const uint32_t i = (uint32_t)i64, j = (uint32_t)j64;
   // proven by the prefilter (its dot reached qcut_hi): no FP32 gate needed
   const bool sure = live && ((e >> 15) & 1u);
+  const bool need = live && !sure;
   const uint32_t q4 = q & 3u;
-  // The plane loads have no branch (a dead lane reads row 0's planes); the
-  // unit rows of a proven candidate are not read.
-  float4 ua[NC], ub[NC];
+  // The plane loads have no branch (a dead lane reads row 0's planes).
   uint4 pa[2 * GPL], pb[2 * GPL];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) ua[c] = ub[c] = float4{0.f, 0.f, 0.f, 0.f};
-  if (live && !sure) {
-    const float4* ra = reinterpret_cast<const float4*>(unit + i64 * 16 * NC);
-    const float4* rb = reinterpret_cast<const float4*>(unit + j64 * 16 * NC);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      ua[c] = ra[4 * c + q4];
-      ub[c] = rb[4 * c + q4];
-    }
-  }
 #pragma unroll
   for (int x = 0; x < GPL; ++x) {
     const uint32_t w = 8u * x + q;
@@ -928,19 +918,38 @@ __device__ __forceinline__ bool gate_compare_one(
     pb[2 * x] = wb[0];
     pb[2 * x + 1] = wb[W];
   }
-  // is_nz inputs (rows 0 stand in for a dead lane; its result is unused)
-  const uint32_t nzi = anynz ? anynz[i] : 1u, nzj = anynz ? anynz[j] : 1u;
-  float thr = thr0;
-  uint32_t acc1_end = acc1_end0;
-  if (!single) {
-    const AsmDesc& ad = desc[asm_of[i]];
-    thr = ad.thr;
-    acc1_end = ad.acc1_end;
-  }
-  // gate value
-  float acc0[4] = {0.f, 0.f, 0.f, 0.f}, acc1[4] = {0.f, 0.f, 0.f, 0.f};
-  bool used1 = false;
-  if (live) {
+  bool pass = sure;
+  // The FP32 gate, only when a candidate of the wave needs it (nearly every
+  // candidate is proven): its unit rows, is_nz inputs and the assembly's
+  // parameters are read by the lanes that need them, and the arithmetic and
+  // lane reductions run on the wave (zeros elsewhere).
+  if (__ballot(need)) {  // wave-uniform
+    float4 ua[NC], ub[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ua[c] = ub[c] = float4{0.f, 0.f, 0.f, 0.f};
+    uint32_t nzi = 1u, nzj = 1u;
+    float thr = thr0;
+    uint32_t acc1_end = acc1_end0;
+    if (need) {
+      const float4* ra = reinterpret_cast<const float4*>(unit + i64 * 16 * NC);
+      const float4* rb = reinterpret_cast<const float4*>(unit + j64 * 16 * NC);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        ua[c] = ra[4 * c + q4];
+        ub[c] = rb[4 * c + q4];
+      }
+      if (anynz) {
+        nzi = anynz[i];
+        nzj = anynz[j];
+      }
+      if (!single) {
+        const AsmDesc& ad = desc[asm_of[i]];
+        thr = ad.thr;
+        acc1_end = ad.acc1_end;
+      }
+    }
+    float acc0[4] = {0.f, 0.f, 0.f, 0.f}, acc1[4] = {0.f, 0.f, 0.f, 0.f};
+    bool used1 = false;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const float a4[4] = {ua[c].x, ua[c].y, ua[c].z, ua[c].w};
@@ -954,19 +963,18 @@ __device__ __forceinline__ bool gate_compare_one(
         for (int l = 0; l < 4; ++l) acc0[l] = __builtin_fmaf(a4[l], b4[l], acc0[l]);
       }
     }
+    if (used1) {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) acc0[l] = acc0[l] + acc1[l];
+    }
+    float t8[4], t4[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) t8[l] = dpp_f32<kDppXor2>(acc0[l]) + acc0[l];  // l+8 with l
+#pragma unroll
+    for (int l = 0; l < 4; ++l) t4[l] = dpp_f32<kDppXor1>(t8[l]) + t8[l];  // l+4 with l
+    const float dot = (t4[0] + t4[2]) + (t4[1] + t4[3]);
+    pass = sure || (need && (nzi | nzj) != 0 && !(dot < thr));
   }
-  if (used1) {
-#pragma unroll
-    for (int l = 0; l < 4; ++l) acc0[l] = acc0[l] + acc1[l];
-  }
-  float t8[4], t4[4];
-#pragma unroll
-  for (int l = 0; l < 4; ++l) t8[l] = dpp_f32<kDppXor2>(acc0[l]) + acc0[l];  // l+8 with l
-#pragma unroll
-  for (int l = 0; l < 4; ++l) t4[l] = dpp_f32<kDppXor1>(t8[l]) + t8[l];  // l+4 with l
-  const float dot = (t4[0] + t4[2]) + (t4[1] + t4[3]);
-  const bool nz = sure || (nzi | nzj) != 0;
-  const bool pass = live && (sure || (nz && !(dot < thr)));
   // register compare
   uint32_t mism = 0;
   if (pass) {
