@@ -424,6 +424,11 @@ rbg_status batch_upload(rbg_ctx* ctx, rbg_asm* b, const AsmInput* in, uint32_t c
     rows += h.n;
     ncol += x.S;
     if (!sk) {
+      // the sketch addresses a workgroup's four contigs with 32-bit offsets
+      for (uint32_t r = 0; r < x.n; ++r)
+        if (x.offsets[r + 1] - x.offsets[r] >= (1ull << 29))
+          return ctx->fail(RBG_ERR_UNSUPPORTED, "assembly " + std::to_string(a) + ": contig " +
+                                                    std::to_string(r) + " is 512 MiB or longer (or offsets decrease)");
       nsk += x.n;
       if (x.n) bytes += x.offsets[x.n] - x.offsets[0];
     }

@@ -195,11 +195,12 @@ __device__ __forceinline__ uint32_t window_slot(uint64_t c40, uint32_t i) {
 // counts two windows.  Every lane counts all eight 5-mers (nearly every chunk
 // is all bases and inside its contig); a lane whose chunk is not takes each
 // 5-mer with an invalid window off again and adds its valid window, if any,
-// to the 4-mer bins h4.  prev is the dword at 16j-4, `first` the buffer
-// position 16j-4 and [beg, end) the contig.
+// to the 4-mer bins h4.  prev is the dword at 16j-4, `first` the position
+// 16j-4 and [beg, end) the contig, both relative to the workgroup's first
+// chunk (a workgroup's contigs span less than 2^31 bytes).
 __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ h5, uint32_t* __restrict__ h4,
-                                            uint32_t prev, uint4 cur, int64_t first, int64_t beg,
-                                            int64_t end, KmerTally& tl) {
+                                            uint32_t prev, uint4 cur, int32_t first, int32_t beg,
+                                            int32_t end, KmerTally& tl) {
   uint32_t x[5];
   const uint32_t c0 = word_codes(prev, x[0]), c1 = word_codes(cur.x, x[1]),
                  c2 = word_codes(cur.y, x[2]), c3 = word_codes(cur.z, x[3]),
@@ -219,9 +220,8 @@ __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ h5, uint32_t*
   if (!full) {
     uint32_t vm = word_valid4(x[0]) | (word_valid4(x[1]) << 4) | (word_valid4(x[2]) << 8) |
                   (word_valid4(x[3]) << 12) | (word_valid4(x[4]) << 16);
-    const int64_t lo = beg - first, hi = end - first;
-    const uint32_t lo_c = (uint32_t)(lo < 0 ? 0 : (lo > 20 ? 20 : lo));
-    const uint32_t hi_c = (uint32_t)(hi < 0 ? 0 : (hi > 20 ? 20 : hi));
+    const uint32_t lo_c = (uint32_t)min(max(beg - first, 0), 20);
+    const uint32_t hi_c = (uint32_t)min(max(end - first, 0), 20);
     vm &= ((1u << hi_c) - 1u) & ~((1u << lo_c) - 1u);
     // bit i: the 4-mer ending at chunk byte i (buffer bytes 16j+i-3 .. 16j+i) is valid
     km = ((vm & (vm >> 1) & (vm >> 2) & (vm >> 3)) >> 1) & 0xFFFFu;
@@ -273,17 +273,6 @@ __device__ __forceinline__ uint32_t rev4(uint32_t v) {
   return ((v & 3u) << 6) | ((v & 0xCu) << 2) | ((v >> 2) & 0xCu) | ((v >> 6) & 3u);
 }
 
-// Ranks r0 .. r0+7 of register k (the last rank repeated past the end).
-__device__ __forceinline__ void load_rank_group(const double* __restrict__ sT, const uint8_t* __restrict__ sX,
-                                                uint32_t m, uint32_t k, int r0, double* t, uint32_t* x) {
-#pragma unroll
-  for (int g = 0; g < kRankGroup; ++g) {
-    const int r = min(r0 + g, kElemCount - 1);
-    t[g] = sT[(size_t)r * m + k];
-    x[g] = sX[(size_t)r * m + k];
-  }
-}
-
 template <int CPB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void sketch_kernel(
     const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
@@ -315,11 +304,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
   if (tid < 4 * CPB) (&bases[0][0])[tid] = 0;
   if (tid <= CPB) soff[tid] = offs[min(c0 + tid, n)];  // contigs past n are empty
   __syncthreads();
-  // chunk ranges of the four contigs: contig q owns chunks cj[q] .. cj[q] +
-  // nch - 1 of the buffer, numbered cpre[q] .. cpre[q+1] - 1 in the block
-  __shared__ int64_t cbeg[CPB], cend[CPB], cj[CPB];
+  // Chunks are numbered from the workgroup's first chunk (the one holding its
+  // first contig's first byte, at buffer byte 16 * (soff[0] / 16)), with
+  // 32-bit offsets from there.  Contig q owns the chunks counted cpre[q] ..
+  // cpre[q+1] - 1 in the workgroup; count g of contig q is chunk g + cdj[q],
+  // and its bytes are [cbe[q].x, cbe[q].y).  A chunk holding the end of one
+  // contig and the start of the next is counted once for each.
+  __shared__ int2 cbe[CPB];
+  __shared__ int32_t cdj[CPB];
   __shared__ uint32_t cpre[CPB + 1];
   if (tid <= CPB) {
+    const int64_t c16 = (int64_t)(soff[0] >> 4);
     uint32_t acc = 0;
     for (uint32_t q = 0; q < tid; ++q) {
       const int64_t b = (int64_t)soff[q], e = (int64_t)soff[q + 1];
@@ -327,58 +322,81 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
     }
     cpre[tid] = acc;
     if (tid < CPB) {
-      cbeg[tid] = (int64_t)soff[tid];
-      cend[tid] = (int64_t)soff[tid + 1];
-      cj[tid] = (int64_t)soff[tid] >> 4;
+      cbe[tid] = make_int2((int32_t)((int64_t)soff[tid] - 16 * c16), (int32_t)((int64_t)soff[tid + 1] - 16 * c16));
+      cdj[tid] = (int32_t)(((int64_t)soff[tid] >> 4) - c16) - (int32_t)acc;
     }
   }
   __syncthreads();
   {
-    const uint32_t nall = cpre[CPB];
+    // workgroup-uniform values, held in scalar registers
+    const uint32_t nall = (uint32_t)__builtin_amdgcn_readfirstlane((int)cpre[CPB]);
+    const uint32_t p1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)cpre[1]);
+    const uint32_t p2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)cpre[2]);
+    const uint32_t p3 = (uint32_t)__builtin_amdgcn_readfirstlane((int)cpre[3]);
+    const int32_t d1 = __builtin_amdgcn_readfirstlane(cdj[1]);
+    const int32_t d2 = __builtin_amdgcn_readfirstlane(cdj[2]);
+    const int32_t d3 = __builtin_amdgcn_readfirstlane(cdj[3]);
+    const uint64_t blk_at = soff[0] & ~15ull;
+    const uint8_t* blk = seqs + (((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(blk_at >> 32)) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)blk_at));
+    // contig and chunk of count g; the chunk and the dword before it (for the
+    // workgroup's first chunk its own first dword: those bytes stand before
+    // every contig of the workgroup, so no window or base takes them)
+    auto locate = [p1, p2, p3, d1, d2, d3](uint32_t g, uint32_t& q, int32_t& jr) {
+      const bool a1 = g >= p1, a2 = g >= p2, a3 = g >= p3;
+      q = (uint32_t)a1 + (uint32_t)a2 + (uint32_t)a3;
+      int32_t d = a1 ? d1 : 0;  // selects of values (a select of references becomes a load)
+      d = a2 ? d2 : d;
+      d = a3 ? d3 : d;
+      jr = (int32_t)g + d;
+    };
+    auto load = [&](int32_t jr, uint4& cur, uint32_t& prev) {
+      const uint32_t at = 16u * (uint32_t)jr;
+      cur = *reinterpret_cast<const uint4*>(blk + at);
+      prev = *reinterpret_cast<const uint32_t*>(blk + (max(at, 4u) - 4u));
+    };
     KmerTally tl;
     uint32_t tq = 0;  // contig of the tally (a thread's chunks come in contig order)
-    uint32_t g = tid, q = 0;
-    int64_t j = 0;
-    uint4 cur = make_uint4(0, 0, 0, 0);
-    uint32_t prev = 0;
-    if (g < nall) {
-      q = (g >= cpre[1]) + (g >= cpre[2]) + (g >= cpre[3]);
-      j = cj[q] + (int64_t)(g - cpre[q]);
-      cur = *reinterpret_cast<const uint4*>(seqs + 16 * j);
-      prev = *reinterpret_cast<const uint32_t*>(seqs + (j > 0 ? 16 * j - 4 : 0));
-    }
-    while (g < nall) {
-      const uint32_t gn = g + 256;
-      uint32_t qn = 0;
-      int64_t jn = 0;
-      uint4 nxt = make_uint4(0, 0, 0, 0);
-      uint32_t nprev = 0;
-      if (gn < nall) {  // next chunk in flight while this one is counted
-        qn = (gn >= cpre[1]) + (gn >= cpre[2]) + (gn >= cpre[3]);
-        jn = cj[qn] + (int64_t)(gn - cpre[qn]);
-        nxt = *reinterpret_cast<const uint4*>(seqs + 16 * jn);
-        nprev = *reinterpret_cast<const uint32_t*>(seqs + (jn > 0 ? 16 * jn - 4 : 0));
-      }
-      if (q != tq) {  // the tally so far belongs to contig tq
-        if (tl.kmers | tl.base[0] | tl.base[1] | tl.base[2] | tl.base[3]) {
-          atomicAdd(&total[tq], tl.kmers);
+    auto flush = [&]() {
+      if (tl.kmers | tl.base[0] | tl.base[1] | tl.base[2] | tl.base[3]) {
+        atomicAdd(&total[tq], tl.kmers);
 #pragma unroll
-          for (int b = 0; b < 4; ++b) atomicAdd(&bases[tq][b], tl.base[b]);
-        }
+        for (int b = 0; b < 4; ++b) atomicAdd(&bases[tq][b], tl.base[b]);
+      }
+    };
+    // count one chunk while the thread's next one (count g + 256; past the end
+    // the last chunk again, unused) is loaded
+    auto step = [&](uint32_t g, uint32_t q, int32_t jr, uint4 cur, uint32_t prev, uint32_t& qn,
+                    int32_t& jn, uint4& nxt, uint32_t& nprev) {
+      locate(min(g + 256u, nall - 1u), qn, jn);
+      load(jn, nxt, nprev);
+      if (q != tq) {  // the tally so far belongs to contig tq
+        flush();
         tl = KmerTally();
         tq = q;
       }
-      chunk_kmers(hbuf + q * 1024u, hbuf + CPB * 1024u + q * 256u, prev, cur, 16 * j - 4, cbeg[q], cend[q], tl);
-      cur = nxt;
-      prev = nprev;
-      g = gn;
-      q = qn;
-      j = jn;
-    }
-    if (tl.kmers | tl.base[0] | tl.base[1] | tl.base[2] | tl.base[3]) {
-      atomicAdd(&total[tq], tl.kmers);
-#pragma unroll
-      for (int b = 0; b < 4; ++b) atomicAdd(&bases[tq][b], tl.base[b]);
+      const int2 be = cbe[q];
+      chunk_kmers(hbuf + q * 1024u, hbuf + CPB * 1024u + q * 256u, prev, cur, 16 * jr - 4, be.x, be.y, tl);
+    };
+    uint32_t g = tid;
+    if (g < nall) {
+      uint32_t qa, qb;
+      int32_t ja, jb;
+      uint4 ca, cb;
+      uint32_t pa, pb;
+      locate(g, qa, ja);
+      load(ja, ca, pa);
+      tq = qa;
+      // two chunk buffers used in turn, so that no registers are copied
+      while (true) {
+        step(g, qa, ja, ca, pa, qb, jb, cb, pb);
+        g += 256;
+        if (g >= nall) break;
+        step(g, qb, jb, cb, pb, qa, ja, ca, pa);
+        g += 256;
+        if (g >= nall) break;
+      }
+      flush();
     }
   }
   __syncthreads();
@@ -454,8 +472,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
     orow[q] = c < n ? (out_row ? out_row[c] : c) : 0u;
   }
 
-  const uint32_t kspan = ((32u * W + 255u) / 256u) * 256u;
-  for (uint32_t k = tid; k < kspan; k += 256) {
+  // Registers k = 64v + l + 256s (s < kspan / 256) belong to wave v.  A lane
+  // walks one register at a time; when its walk stops it records the four
+  // winners in wbx[k] (byte q: contig q's winner, 0xFF for none) and takes
+  // the wave's next unwalked register, so that the wave's lanes stay busy
+  // until its registers run out (walk lengths vary: about 1.8 rank groups per
+  // register on average, but the longest of 64 registers is several times
+  // that).  The winners are turned into output bytes and plane words
+  // afterwards, with lane l of wave v on registers 64v + l + 256s again.
+  const uint32_t kspan = ((32u * W + 255u) / 256u) * 256u;  // <= 512 (m <= 512)
+  uint32_t* wbx = hbuf + 2 * CPB * kElemCount;                 // after winv
+  static_assert(2 * CPB * kElemCount + 512 <= CPB * kBins, "winners fit the bins");
+  static_assert(kElemCount % kRankGroup == 0, "whole rank groups");
+  {
+    const uint32_t wv = tid >> 6;
+    const uint32_t npool = kspan / 4u;  // registers per wave
+    auto kof = [&](uint32_t p) { return 64u * wv + 256u * (p >> 6) + (p & 63u); };
+    uint32_t p = ln, pn = 64;  // this lane's pool entry, the wave's next one
+    uint32_t k = kof(p);
+    bool act = p < npool && k < m;  // kof grows with p: once k >= m, every later k too
+    int r0 = 0;
+    uint32_t at = k;  // table entry of rank r0 at register k (< 136 * 512)
     double best[CPB];
     int bx[CPB];
 #pragma unroll
@@ -463,18 +500,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
       best[q] = __builtin_huge_val();
       bx[q] = -1;
     }
-    if (k < m) {
-      // Ranks in groups of kRankGroup: applying an element past the break
-      // point changes nothing (it can neither win nor tie), so the break is
-      // tested once per group, on the group's last rank.
-      for (int r0 = 0; r0 < kElemCount; r0 += kRankGroup) {
+    while (__ballot(act)) {
+      bool fin = false;
+      if (act) {
+        // Ranks in groups of kRankGroup: applying an element past the break
+        // point changes nothing (it can neither win nor tie), so the break
+        // is tested once per group, on the group's last rank.
         double t[kRankGroup];
         uint32_t x[kRankGroup];
-        load_rank_group(sT, sX, m, k, r0, t, x);
-        // Branch-free update, so that the LDS reads of several ranks are in
-        // flight together: an absent element has 1/w = inf, hence hv = inf,
-        // which neither beats nor ties best (best = inf only while bx = -1,
-        // and x > -1).
+#pragma unroll
+        for (int g = 0; g < kRankGroup; ++g) {  // rank r0 + g at element (r0 + g) * m + k
+          const uint32_t e = at + (uint32_t)g * m;
+          t[g] = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(sT) + 8u * e);
+          x[g] = sX[e];
+        }
+        // An absent element has 1/w = inf, hence hv = inf, which neither
+        // beats nor ties best (best = inf only while bx = -1, and x > -1).
 #pragma unroll
         for (int g = 0; g < kRankGroup; ++g) {
           double wq[CPB];
@@ -492,8 +533,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
 #pragma unroll
         for (int q = 0; q < CPB; ++q)  // wmin = +inf: contig without elements
           done = done && (wmin[q] == __builtin_huge_val() || wmin[q] * t[kRankGroup - 1] > best[q]);
-        if (done) break;
+        fin = done || r0 + kRankGroup >= kElemCount;
+        if (fin) {
+          uint32_t packed = 0;
+#pragma unroll
+          for (int q = 0; q < CPB; ++q) packed |= ((uint32_t)bx[q] & 0xFFu) << (8 * q);
+          wbx[k] = packed;
+        }
+        r0 += kRankGroup;
+        at += kRankGroup * m;
       }
+      const uint64_t fm = __ballot(fin);
+      if (fin) {  // the next registers, in lane order
+        p = pn + lanes_below(fm);
+        k = kof(p);
+        act = p < npool && k < m;
+        r0 = 0;
+        at = k;
+#pragma unroll
+        for (int q = 0; q < CPB; ++q) {
+          best[q] = __builtin_huge_val();
+          bx[q] = -1;
+        }
+      }
+      pn += (uint32_t)__popcll(fm);
+    }
+  }
+  __syncthreads();  // winners recorded (a register's walker is any lane of its wave)
+  for (uint32_t k = tid; k < kspan; k += 256) {
+    const uint32_t wb = k < m ? wbx[k] : 0xFFFFFFFFu;
+    int bx[CPB];
+#pragma unroll
+    for (int q = 0; q < CPB; ++q) {
+      const uint32_t b = (wb >> (8 * q)) & 0xFFu;
+      bx[q] = b == 0xFFu ? -1 : (int)b;
     }
     // class bytes of the four winners, loaded together
     uint32_t code[CPB];
@@ -2719,6 +2792,7 @@ hipError_t launch_sketch(const uint8_t* seqs, const uint64_t* offs, uint32_t n, 
   constexpr int CPB = 4;
   const uint32_t blocks = (n + CPB - 1) / CPB;
   if (blocks == 0) return hipSuccess;
+  if (W > 16) return hipErrorInvalidValue;  // the kernel holds the winners of at most 512 registers
   hipLaunchKernelGGL(sketch_kernel<CPB>, dim3(blocks), dim3(256), 0, st, seqs, offs, n, m, W, sT,
                      sX, cls, codes, wx, mpad, planes, out_row);
   return hipGetLastError();
