@@ -122,7 +122,7 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
 // (src/rabbitbin.cpp:1491-1516), kept as 1/w per element; the base
 // probabilities and 1/total are computed once per contig.
 //
-// Phase 2: thread t owns registers k = t, t+256, ...; for each register the
+// Phase 2: wave v owns registers k = 64v + l + 256s; for each register the
 // winner is argmin_x fl((1/w_x) * T[x][k]) over present elements, ties to the
 // smaller x (the sketch's strict-less update, src/probmh.cpp:417-431).
 // Elements are visited in ascending T[x][k] (sT/sX, shared by the four
@@ -130,18 +130,18 @@ __device__ __forceinline__ uint32_t pair_count_rows(const uint32_t* __restrict__
 // after rank r has fl((1/w_x) * T_x) >= fl(wmin * T_r) (both factors are no
 // smaller, and rounding is monotone), so once fl(wmin * T_r) > best no later
 // element can win or tie and the walk stops: typically after a few ranks.
-// Ranks are loaded and applied in groups of eight with the stop tested per
-// group (applying elements past the stop changes nothing); a register's
-// first group is loaded before it is needed (the first register's while the
-// weights are formed, the next register's while the current one's planes
-// are written).  A wave's 64 registers give each contig two plane-word
-// groups: the eight ballots of a contig are gathered into lanes 0..15 with
-// writelane and written by one store (two 32-byte segments).
+// Ranks are loaded and applied in groups of four with the stop tested per
+// group (applying elements past the stop changes nothing).  A lane walks one
+// register at a time and takes the wave's next register when its walk stops.
+// The winners go to LDS; then a wave's 64 registers give each contig two
+// plane-word groups: the eight ballots of a contig are gathered into lanes
+// 0..15 by selects and written by one store (two 32-byte segments).
 //
 // The sequence buffer must be readable 16 bytes past its last contig (the
-// last chunk of a contig is read whole).  The dword before chunk 0 of the
-// buffer does not exist; the first dword is read in its place, and its bytes
-// stand before every contig, so no window or base takes them.
+// last chunk of a contig is read whole).  A thread reads the dword before
+// its chunk; for the first chunk of a workgroup it reads the chunk's first
+// dword instead (those bytes stand before every contig of the workgroup, so
+// no window or base takes them).
 // ---------------------------------------------------------------------------
 constexpr int kSketchWaves = 4;
 constexpr int kRankGroup = 4;  // proposal ranks loaded together in phase 2
@@ -195,10 +195,12 @@ __device__ __forceinline__ uint32_t window_slot(uint64_t c40, uint32_t i) {
 // counts two windows.  Every lane counts all eight 5-mers (nearly every chunk
 // is all bases and inside its contig); a lane whose chunk is not takes each
 // 5-mer with an invalid window off again and adds its valid window, if any,
-// to the 4-mer bins h4.  prev is the dword at 16j-4, `first` the position
+// to the 4-mer bins h4 (one per element: elem maps a window value E to the
+// element of its canonical 4-mer).  prev is the dword at 16j-4, `first` the position
 // 16j-4 and [beg, end) the contig, both relative to the workgroup's first
 // chunk (a workgroup's contigs span less than 2^31 bytes).
 __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ h5, uint32_t* __restrict__ h4,
+                                            const uint8_t* __restrict__ elem,
                                             uint32_t prev, uint4 cur, int32_t first, int32_t beg,
                                             int32_t end, KmerTally& tl) {
   uint32_t x[5];
@@ -230,8 +232,8 @@ __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ h5, uint32_t*
     for (uint32_t pm = (inv | (inv >> 1)) & 0x5555u; pm != 0; pm &= pm - 1u) {
       const uint32_t i = (uint32_t)__ffs(pm) - 1u;  // = 2t
       atomicSub(&h5[(uint32_t)(c40 >> (2 * i + 2)) & 0x3FFu], 1u);
-      if ((km >> i) & 1u) atomicAdd(&h4[window_slot(c40, i)], 1u);
-      if ((km >> (i + 1)) & 1u) atomicAdd(&h4[window_slot(c40, i + 1)], 1u);
+      if ((km >> i) & 1u) atomicAdd(&h4[elem[window_slot(c40, i)]], 1u);
+      if ((km >> (i + 1)) & 1u) atomicAdd(&h4[elem[window_slot(c40, i + 1)]], 1u);
     }
     valid = spread16((vm >> 4) & 0xFFFFu);
   }
@@ -244,12 +246,11 @@ __device__ __forceinline__ void chunk_kmers(uint32_t* __restrict__ h5, uint32_t*
   }
 }
 
-// Windows with value E (earliest base lowest) in a contig: its 4-mer bin
-// plus the 5-mers whose first four bases (E | x << 8) or last four
+// Windows with value E (earliest base lowest) in a contig counted by the
+// 5-mer bins: those whose first four bases (E | x << 8) or last four
 // (x | E << 2) are E.
-__device__ __forceinline__ uint32_t window_count(const uint32_t* __restrict__ h5,
-                                                 const uint32_t* __restrict__ h4, uint32_t e) {
-  uint32_t c = h4[e];
+__device__ __forceinline__ uint32_t window_count(const uint32_t* __restrict__ h5, uint32_t e) {
+  uint32_t c = 0;
 #pragma unroll
   for (uint32_t x = 0; x < 4; ++x) c += h5[e | (x << 8)] + h5[x | (e << 2)];
   return c;
@@ -274,15 +275,17 @@ __device__ __forceinline__ uint32_t rev4(uint32_t v) {
 }
 
 template <int CPB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void sketch_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void sketch_kernel(
     const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs, uint32_t n, uint32_t m,
     uint32_t W, const double* __restrict__ sT, const uint8_t* __restrict__ sX,
     const uint8_t* __restrict__ cls, const uint8_t* __restrict__ codes, uint8_t* __restrict__ wx,
     uint32_t mpad, uint32_t* __restrict__ planes, const uint32_t* __restrict__ out_row) {
   static_assert(CPB == kSketchWaves, "four contigs per workgroup");
-  // Phase-1 bins per contig: 1024 5-mer bins, then 256 4-mer bins.  After
-  // the counts are read the same bytes hold winv (1/w per element).
-  constexpr uint32_t kBins = 1024 + 256;
+  // Phase-1 bins per contig: 1024 5-mer bins, then one bin per element for
+  // the windows counted alone (next to a non-base or a contig end).  After
+  // the counts are read the same bytes hold winv (1/w per element).  At
+  // 19.2 KB of LDS per workgroup, eight workgroups fit a CU.
+  constexpr uint32_t kBins = 1024 + kElemCount;
   __shared__ __align__(16) uint32_t hbuf[CPB * kBins];
   double (*winv)[kElemCount] = reinterpret_cast<double (*)[kElemCount]>(hbuf);
   static_assert(sizeof(double) * CPB * kElemCount <= sizeof(hbuf), "winv fits the bins");
@@ -292,6 +295,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
   __shared__ double tinv[CPB];         // per contig: 1 / total
   __shared__ unsigned long long wmin_bits[CPB];  // positive doubles order as integers
   __shared__ uint64_t soff[CPB + 1];   // contig q is [soff[q], soff[q+1])
+  __shared__ uint8_t elem[256];        // window value E -> element of its canonical 4-mer
   const uint32_t tid = threadIdx.x;
   const uint32_t ln = tid & 63u;
   const uint32_t c0 = blockIdx.x * CPB;
@@ -303,6 +307,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
   }
   if (tid < 4 * CPB) (&bases[0][0])[tid] = 0;
   if (tid <= CPB) soff[tid] = offs[min(c0 + tid, n)];  // contigs past n are empty
+  if (tid < kElemCount) {
+    const uint32_t v = codes[tid];
+    elem[rev4(v)] = (uint8_t)tid;
+    elem[255u - v] = (uint8_t)tid;
+  }
   __syncthreads();
   // Chunks are numbered from the workgroup's first chunk (the one holding its
   // first contig's first byte, at buffer byte 16 * (soff[0] / 16)), with
@@ -376,7 +385,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
         tq = q;
       }
       const int2 be = cbe[q];
-      chunk_kmers(hbuf + q * 1024u, hbuf + CPB * 1024u + q * 256u, prev, cur, 16 * jr - 4, be.x, be.y, tl);
+      chunk_kmers(hbuf + q * 1024u, hbuf + CPB * 1024u + q * kElemCount, elem, prev, cur, 16 * jr - 4, be.x, be.y, tl);
     };
     uint32_t g = tid;
     if (g < nall) {
@@ -421,8 +430,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void s
       const uint32_t v = codes[x];
       const uint32_t sa = rev4(v), sb = 255u - v;
       const uint32_t* h5 = hbuf + q * 1024u;
-      const uint32_t* h4 = hbuf + CPB * 1024u + q * 256u;
-      ecnt[r] = window_count(h5, h4, sa) + (sa != sb ? window_count(h5, h4, sb) : 0u);
+      ecnt[r] = hbuf[CPB * 1024u + q * kElemCount + x] + window_count(h5, sa) + (sa != sb ? window_count(h5, sb) : 0u);
     }
   }
   __syncthreads();  // bins read: their bytes now take winv
