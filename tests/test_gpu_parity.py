@@ -119,6 +119,19 @@ def test_sketch_edge_cases(ctx, m):
     assert not w32[:3].any() and not w32[4].any()  # no 4-mer: every register unset
 
 
+def test_contig_length_limit(ctx):
+    """The sketch addresses a workgroup's four contigs with 32-bit offsets:
+    a contig of 512 MiB or more, or decreasing offsets, is refused before any
+    byte is read or uploaded (the buffer here is 64 bytes)."""
+    seqs = b"ACGT" * 16
+    for off in ([0, 1 << 29], [0, 40, 20]):
+        with pytest.raises(api.RbgError, match="512 MiB"):
+            ctx.build_pmh_winners(seqs, np.array(off, dtype=np.uint64), m=500, seed=42)
+    off = np.array([0, 64], dtype=np.uint64)
+    w32, _ = ctx.build_pmh_winners(seqs, off, m=500, seed=42)
+    assert np.array_equal(w32, orc.sketch_batch(seqs, off, m=500, seed=42))
+
+
 def test_graph_natural_fused_regime_26k(ctx):
     """n = 26,000 > 25,000 selects the fused regime by itself (S = 10: one
     masked 16-lane chunk in the gate); the resident assembly with the edge
