@@ -292,7 +292,9 @@ def served_form(count, rate):
     the graphs returned: n(n-1)/2 per result) at the offered rate, the same at
     saturation (every session arriving at once), and the plain path on the
     same request sequence (every request a full rebuild from host input, one
-    at a time, FIFO latencies simulated from its measured service times)."""
+    at a time, FIFO latencies simulated from its measured service times; a
+    request's service time is the median of three builds, since single
+    builds vary with the host's state)."""
     import heapq
     import threading
     import numpy as np
@@ -367,9 +369,12 @@ def served_form(count, rate):
                     base += int(p[1][-1])
                 offs = np.concatenate(offs)
                 depth = np.concatenate([p[2] for p in parts])
-                t1 = time.perf_counter()
-                ctx.build_graph(seqs, offs, depth, params)
-                service[(k, t)] = time.perf_counter() - t1
+                ts = []
+                for _ in range(3):
+                    t1 = time.perf_counter()
+                    ctx.build_graph(seqs, offs, depth, params)
+                    ts.append(time.perf_counter() - t1)
+                service[(k, t)] = float(np.median(ts))
     T = len(pieces[0])
     ns = {(k, t): sum(len(p[1]) - 1 for p in pieces[k][:t + 1]) for k in range(count) for t in range(T)}
 
