@@ -291,6 +291,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
   static_assert(sizeof(double) * CPB * kElemCount <= sizeof(hbuf), "winv fits the bins");
   __shared__ uint32_t bases[CPB][4];
   __shared__ uint32_t total[CPB];
+  // Phase-1 tallies per contig (windows, then bases A, C, G, T) in eight
+  // copies: lane l adds to copy l % 8, so the lanes of a wave that add to one
+  // counter hit eight words in eight banks instead of one word.
+  constexpr uint32_t kTallyCopies = 8;
+  __shared__ uint32_t tallies[CPB][5][kTallyCopies];
   __shared__ double pbase[CPB][4];     // per contig: base probabilities
   __shared__ double tinv[CPB];         // per contig: 1 / total
   __shared__ unsigned long long wmin_bits[CPB];  // positive doubles order as integers
@@ -303,9 +308,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
   for (uint32_t v = tid; v < CPB * kBins / 4; v += 256) reinterpret_cast<uint4*>(hbuf)[v] = make_uint4(0, 0, 0, 0);
   if (tid < CPB) {
     wmin_bits[tid] = 0x7FF0000000000000ull;  // +inf: no element present
-    total[tid] = 0;
   }
-  if (tid < 4 * CPB) (&bases[0][0])[tid] = 0;
+  if (tid < CPB * 5 * kTallyCopies) (&tallies[0][0][0])[tid] = 0;
   if (tid <= CPB) soff[tid] = offs[min(c0 + tid, n)];  // contigs past n are empty
   if (tid < kElemCount) {
     const uint32_t v = codes[tid];
@@ -368,9 +372,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
     uint32_t tq = 0;  // contig of the tally (a thread's chunks come in contig order)
     auto flush = [&]() {
       if (tl.kmers | tl.base[0] | tl.base[1] | tl.base[2] | tl.base[3]) {
-        atomicAdd(&total[tq], tl.kmers);
+        atomicAdd(&tallies[tq][0][ln % kTallyCopies], tl.kmers);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) atomicAdd(&bases[tq][b], tl.base[b]);
+        for (int b = 0; b < 4; ++b) atomicAdd(&tallies[tq][1 + b][ln % kTallyCopies], tl.base[b]);
       }
     };
     // count one chunk while the thread's next one (count g + 256; past the end
@@ -411,6 +415,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
   __syncthreads();
   if (tid < CPB) {  // per-contig factors, in the reference's operations
     const uint32_t q = tid;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      uint32_t sum = 0;
+#pragma unroll
+      for (uint32_t r = 0; r < kTallyCopies; ++r) sum += tallies[q][v][r];
+      if (v == 0)
+        total[q] = sum;
+      else
+        bases[q][v - 1] = sum;
+    }
     const uint64_t vb = (uint64_t)bases[q][0] + bases[q][1] + bases[q][2] + bases[q][3];
 #pragma unroll
     for (int b = 0; b < 4; ++b) pbase[q][b] = vb > 0 ? (double)bases[q][b] / (double)vb : 0.25;
